@@ -1,0 +1,112 @@
+/* Handle-based C API of the XC quadrature engine (libg2g.so).
+ *
+ * The g2g Fortran ABI (g2g.h) is a process-global singleton, like the
+ * reference.  This API exposes the same engine per context so that callers
+ * can hold several molecules at once and receive status codes instead of a
+ * terminated process.  It has no counterpart in the reference; the g2g_*
+ * entry points are implemented on top of one default context.
+ *
+ * All functions return 0 on success and a negative code on failure; the
+ * message of the last failure on the calling thread is xcq_last_error().
+ */
+#ifndef LIO_AMD_XCQ_H
+#define LIO_AMD_XCQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xcq_context xcq_context;
+
+/* The arrays of g2g_parameter_init_ in their Fortran layouts plus the
+ * g2g_set_options_ knobs.  Referenced, not copied: they must outlive the
+ * context or the next xcq_set_problem. */
+typedef struct {
+  unsigned natom, max_atoms, M, ngaussians;
+  int normalize;            /* NORM: d xx/yy/zz carry 1/sqrt(3) */
+  unsigned iexch;           /* 1..3 LDA, 9 PBE */
+  const double* r;          /* max_atoms x 3 column-major, bohr */
+  const double* Rm;         /* Becke size parameter by Z (0..54) */
+  const unsigned* Iz;       /* natom */
+  const unsigned* Nr;       /* radial shells by Z, grid type 0 */
+  const unsigned* Nr2;      /* radial shells by Z, grid types 1, 2 */
+  const unsigned* Nuc;      /* M, 1-based atom of each function */
+  const unsigned* ncont;    /* M */
+  const unsigned* nshell;   /* 3: #s, #p functions, #d functions */
+  const double* c;          /* M x 13 column-major (ld = ngaussians) */
+  const double* a;          /* M x 13 column-major (ld = ngaussians) */
+  const double* e[3];       /* angular rules 50/116/194: n x 3 column-major */
+  const double* wang[3];    /* their weights (sum 4 pi) */
+  double little_cube_size;  /* default 8.0 */
+  double sphere_radius;     /* default 0.6 */
+  int assign_all_functions; /* default 0 */
+  int remove_zero_weights;  /* default 1 */
+  unsigned min_points_per_cube;   /* default 1 */
+  unsigned max_function_exponent; /* default 10 */
+} xcq_problem;
+
+/* Partition inventory of one point group. */
+typedef struct {
+  int kind;      /* 0 cube, 1 sphere */
+  int npoints;   /* after zero-weight removal */
+  int nfuncs;    /* expanded functions */
+  int s, p, d;   /* selected shells per kind */
+  int nnuc;      /* atoms owning selected functions */
+  double wsum;   /* sum of point weights */
+  double xsum;   /* sum of point x coordinates */
+  unsigned long long fhash; /* FNV-1a of the selected shells' first indices */
+} xcq_group_info;
+
+/* Grid summary after xcq_build_grid. */
+typedef struct {
+  int ngroups;
+  long long npoints;          /* real points */
+  long long npoints_padded;
+  double flops;               /* algorithmic 10 n m^2 summed over groups */
+  double bytes_values;        /* algorithmic 2*10*n*m*8 evaluation bytes */
+  double grid_seconds;        /* host partition build */
+  double upload_seconds;
+  unsigned long long arena_bytes;
+  int density_tiles, fock_items;
+} xcq_grid_info;
+
+typedef struct {
+  double energy, electrons;
+  double seconds;             /* host wall time of the call */
+  double kernel_ms[4];        /* expand, density, fock, pack (HIP events) */
+} xcq_solve_info;
+
+const char* xcq_last_error(void);
+
+int xcq_create(int device, xcq_context** out);
+int xcq_destroy(xcq_context* ctx);
+int xcq_set_problem(xcq_context* ctx, const xcq_problem* problem);
+/* Builds grid and partition for grid_type (0/1/2) and uploads them. */
+int xcq_build_grid(xcq_context* ctx, unsigned grid_type);
+int xcq_query_grid(xcq_context* ctx, xcq_grid_info* out);
+/* Group inventory of the built grid; returns the count through *ngroups. */
+int xcq_groups(xcq_context* ctx, xcq_group_info* out, int max, int* ngroups);
+/* One XC Fock build from a packed host density; fock_accum (+=, may be NULL). */
+int xcq_solve(xcq_context* ctx, const double* rho_packed, double* fock_accum,
+              xcq_solve_info* info);
+/* Same with device pointers (rho and F packed, in device memory); no PCIe
+ * copies and no synchronisation: the work is queued on the context stream
+ * (xcq_stream); energy and electrons are read back by xcq_sync_energy. */
+int xcq_solve_device(xcq_context* ctx, const double* d_rho_packed, double* d_fock);
+int xcq_sync_energy(xcq_context* ctx, double* energy, double* electrons);
+/* Waits for the context stream and returns the HIP-event durations (ms) of
+ * the last solve's phases: [0] density expansion, [1] density + functional,
+ * [2] Fock tiles + slice sums, [3] Fock packing + energy reduction. */
+int xcq_kernel_times(xcq_context* ctx, double ms[4]);
+/* HIP stream (hipStream_t) the context launches on. */
+void* xcq_stream(xcq_context* ctx);
+
+/* Host-only partition build (no device needed): inventory of the groups the
+ * engine would build for this problem and grid type. */
+int xcq_host_partition(const xcq_problem* problem, unsigned grid_type,
+                       xcq_group_info* out, int max, int* ngroups);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

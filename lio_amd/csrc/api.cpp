@@ -1,0 +1,391 @@
+// C entry points: the handle API (xcq_*) and the g2g Fortran ABI on top of a
+// default context.  The g2g semantics follow g2g/init.cpp:38-433 and
+// g2g/partition.cpp:426-591 (energy assigned, Fock accumulated, NaN energy
+// terminates).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/g2g.h"
+#include "../../include/xcq.h"
+#include "xcq.h"
+
+namespace xcq {
+// Named Fortran timers (timers.cpp) need the options' timer level.
+void timers_configure(unsigned level);
+}
+
+struct xcq_context {
+  std::unique_ptr<xcq::Engine> engine;
+  xcq::ProblemView pv;
+  xcq::Options opt;
+  bool have_problem = false;
+  int grid_type = -1;
+  xcq::Molecule mol;
+  xcq::GridPartition part;
+  double upload_seconds = 0.0;
+  long long npts_pad = 0;
+  double flops = 0.0;
+  int density_tiles = 0, fock_items = 0;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+double wall() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  } catch (...) {
+    g_error = "unknown error";
+    return -2;
+  }
+}
+
+void view_from_problem(const xcq_problem* p, xcq::ProblemView& pv, xcq::Options& opt) {
+  pv.natom = p->natom;
+  pv.max_atoms = p->max_atoms ? p->max_atoms : p->natom;
+  pv.M = p->M;
+  pv.ngaussians = p->ngaussians ? p->ngaussians : p->M;
+  pv.normalize = p->normalize != 0;
+  pv.iexch = p->iexch;
+  pv.r = p->r;
+  pv.Rm = p->Rm;
+  pv.Iz = p->Iz;
+  pv.Nr = p->Nr;
+  pv.Nr2 = p->Nr2;
+  pv.Nuc = p->Nuc;
+  pv.ncont = p->ncont;
+  pv.nshell = p->nshell;
+  pv.c = p->c;
+  pv.a = p->a;
+  for (int k = 0; k < 3; ++k) {
+    pv.e[k] = p->e[k];
+    pv.wang[k] = p->wang[k];
+  }
+  opt.little_cube_size = p->little_cube_size;
+  opt.sphere_radius = p->sphere_radius;
+  opt.assign_all_functions = p->assign_all_functions != 0;
+  opt.remove_zero_weights = p->remove_zero_weights != 0;
+  opt.min_points_per_cube = p->min_points_per_cube;
+  opt.max_function_exponent = p->max_function_exponent;
+}
+
+void fill_groups(const xcq::GridPartition& part, xcq_group_info* out, int max) {
+  for (int g = 0; g < part.ngroups && g < max; ++g) {
+    xcq_group_info& o = out[g];
+    o.kind = part.kind[g];
+    o.npoints = part.npts[g];
+    o.nfuncs = part.m[g];
+    o.s = part.ns[g];
+    o.p = part.np[g];
+    o.d = part.nd[g];
+    o.nnuc = part.nnuc[g];
+    o.wsum = 0.0;
+    o.xsum = 0.0;
+    for (int k = 0; k < part.npts[g]; ++k) {
+      o.wsum += part.w[part.pt_off[g] + k];
+      o.xsum += part.x[part.pt_off[g] + k];
+    }
+    o.fhash = part.fhash[g];
+  }
+}
+
+void check_problem(const xcq::ProblemView& pv) {
+  if (!pv.r || !pv.Rm || !pv.Iz || !pv.Nuc || !pv.ncont || !pv.nshell || !pv.a || !pv.c)
+    throw std::runtime_error("g2g: problem arrays missing");
+  if (pv.natom == 0 || pv.M == 0) throw std::runtime_error("g2g: empty problem");
+  if (pv.open) throw std::runtime_error("g2g: open-shell solve is not provided by this build");
+}
+
+void build(xcq_context* ctx, unsigned grid_type) {
+  if (!ctx->have_problem) throw std::runtime_error("g2g: grid requested before parameter init");
+  check_problem(ctx->pv);
+  ctx->mol = xcq::capture(ctx->pv, grid_type);
+  ctx->part = xcq::build_partition(ctx->mol, ctx->opt);
+  const double t0 = wall();
+  ctx->engine->load(ctx->mol, ctx->part);
+  ctx->upload_seconds = wall() - t0;
+  ctx->grid_type = (int)grid_type;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xcq_last_error(void) { return g_error.c_str(); }
+
+int xcq_create(int device, xcq_context** out) {
+  return guarded([&] {
+    auto* ctx = new xcq_context;
+    try {
+      ctx->engine.reset(new xcq::Engine(device));
+    } catch (...) {
+      delete ctx;
+      throw;
+    }
+    *out = ctx;
+  });
+}
+
+int xcq_destroy(xcq_context* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int xcq_set_problem(xcq_context* ctx, const xcq_problem* p) {
+  return guarded([&] {
+    view_from_problem(p, ctx->pv, ctx->opt);
+    ctx->have_problem = true;
+    ctx->grid_type = -1;
+  });
+}
+
+int xcq_build_grid(xcq_context* ctx, unsigned grid_type) {
+  return guarded([&] { build(ctx, grid_type); });
+}
+
+int xcq_query_grid(xcq_context* ctx, xcq_grid_info* o) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: no grid built");
+    const xcq::GridPartition& part = ctx->part;
+    o->ngroups = part.ngroups;
+    o->npoints = part.total_points();
+    long long pad = 0;
+    double fl = 0.0, by = 0.0;
+    for (int g = 0; g < part.ngroups; ++g) {
+      pad += (part.npts[g] + 15) / 16 * 16;
+      fl += 10.0 * part.npts[g] * (double)part.m[g] * part.m[g];
+      by += 2.0 * 10.0 * part.npts[g] * (double)part.m[g] * 8.0;
+    }
+    o->npoints_padded = pad;
+    o->flops = fl;
+    o->bytes_values = by;
+    o->grid_seconds = part.seconds;
+    o->upload_seconds = ctx->upload_seconds;
+    o->arena_bytes = ctx->engine->arena_bytes();
+    o->density_tiles = (int)(pad / 16);
+    o->fock_items = 0;
+  });
+}
+
+int xcq_groups(xcq_context* ctx, xcq_group_info* out, int max, int* n) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: no grid built");
+    if (out) fill_groups(ctx->part, out, max);
+    if (n) *n = ctx->part.ngroups;
+  });
+}
+
+int xcq_solve(xcq_context* ctx, const double* rho, double* fock_accum, xcq_solve_info* info) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
+    xcq::SolveResult r = ctx->engine->solve(rho, fock_accum, true);
+    if (info) {
+      info->energy = r.energy;
+      info->electrons = r.electrons;
+      info->seconds = r.seconds;
+      for (int k = 0; k < 4; ++k) info->kernel_ms[k] = r.kernel_ms[k];
+    }
+  });
+}
+
+int xcq_solve_device(xcq_context* ctx, const double* d_rho, double* d_fock) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
+    ctx->engine->solve_device(d_rho, d_fock, true);
+  });
+}
+
+int xcq_sync_energy(xcq_context* ctx, double* energy, double* electrons) {
+  return guarded([&] { ctx->engine->read_energy(energy, electrons); });
+}
+
+int xcq_kernel_times(xcq_context* ctx, double ms[4]) {
+  return guarded([&] { ctx->engine->kernel_times(ms); });
+}
+
+void* xcq_stream(xcq_context* ctx) { return ctx ? ctx->engine->stream() : nullptr; }
+
+int xcq_host_partition(const xcq_problem* p, unsigned grid_type, xcq_group_info* out, int max,
+                       int* n) {
+  return guarded([&] {
+    xcq::ProblemView pv;
+    xcq::Options opt;
+    view_from_problem(p, pv, opt);
+    check_problem(pv);
+    xcq::Molecule mol = xcq::capture(pv, grid_type);
+    xcq::GridPartition part = xcq::build_partition(mol, opt);
+    if (out) fill_groups(part, out, max);
+    if (n) *n = part.ngroups;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// g2g Fortran ABI on the default context
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+xcq_context* g_ctx = nullptr;
+xcq::Options g_opt;  // set by g2g_set_options_, possibly before g2g_init_
+
+[[noreturn]] void die(const char* where, const std::string& msg) {
+  std::fprintf(stderr, "g2g: %s failed: %s\n", where, msg.c_str());
+  std::fflush(stderr);
+  std::exit(1);
+}
+
+template <class F>
+void abi(const char* where, F&& f) {
+  try {
+    f();
+  } catch (const std::exception& e) {
+    die(where, e.what());
+  } catch (...) {
+    die(where, "unknown error");
+  }
+}
+
+xcq_context* ctx_or_die(const char* where) {
+  if (!g_ctx) die(where, "g2g_init_ was not called");
+  return g_ctx;
+}
+
+}  // namespace
+
+extern "C" {
+
+void g2g_init_(void) {
+  abi("g2g_init_", [] {
+    if (!g_ctx) {
+      g_ctx = new xcq_context;
+      g_ctx->engine.reset(new xcq::Engine(-1));
+    }
+    g_ctx->opt = g_opt;
+    if (g_opt.verbose > 2) std::printf("  Using HIP device %d for the XC quadrature.\n", g_ctx->engine->device());
+  });
+}
+
+void g2g_set_options_(double* fgm, double* lcs, double* sr, bool* aaf, bool* eai, bool* rzw,
+                      unsigned* mppc, unsigned* mfe, unsigned* tl, unsigned* verbose) {
+  g_opt.free_global_memory = *fgm;
+  g_opt.little_cube_size = *lcs;
+  g_opt.sphere_radius = *sr;
+  g_opt.assign_all_functions = *aaf;
+  g_opt.energy_all_iterations = *eai;
+  g_opt.remove_zero_weights = *rzw;
+  g_opt.min_points_per_cube = *mppc;
+  g_opt.max_function_exponent = *mfe;
+  g_opt.timer_level = *tl;
+  g_opt.verbose = *verbose;
+  xcq::timers_configure(*tl);
+  if (g_ctx) g_ctx->opt = g_opt;
+}
+
+void g2g_parameter_init_(const unsigned* norm, const unsigned* natom, const unsigned* max_atoms,
+                         const unsigned* ngaussians, double* r, double* Rm, const unsigned* Iz,
+                         const unsigned* Nr, const unsigned* Nr2, unsigned* Nuc, const unsigned* M,
+                         unsigned* ncont, const unsigned* nshell, double* c, double* a,
+                         double* rho_vec, double* fock_vec, double* fockb_vec, double* rhoalpha,
+                         double* rhobeta, const unsigned* nco, bool* OPEN, const unsigned* nunp,
+                         const unsigned* nopt, const unsigned* Iexch, double* e, double* e2,
+                         double* e3, double* wang, double* wang2, double* wang3, bool* use_libxc,
+                         const unsigned* ex_functional_id, const unsigned* ec_functional_id) {
+  (void)fockb_vec;
+  (void)rhoalpha;
+  (void)rhobeta;
+  (void)ex_functional_id;
+  (void)ec_functional_id;
+  abi("g2g_parameter_init_", [&] {
+    xcq_context* ctx = ctx_or_die("g2g_parameter_init_");
+    if (*Iexch < 1 || *Iexch > 9) throw std::runtime_error("Iexch must be in 1..9");
+    if (*use_libxc) throw std::runtime_error("libxc functionals are not provided by this build");
+    xcq::ProblemView& pv = ctx->pv;
+    pv = xcq::ProblemView{};
+    pv.natom = *natom;
+    pv.max_atoms = *max_atoms;
+    pv.M = *M;
+    pv.ngaussians = *ngaussians;
+    pv.normalize = *norm != 0;
+    pv.iexch = *Iexch;
+    pv.open = *OPEN;
+    pv.nco = *nco;
+    pv.nunp = *nunp;
+    pv.nopt = *nopt;
+    pv.r = r;
+    pv.Rm = Rm;
+    pv.Iz = Iz;
+    pv.Nr = Nr;
+    pv.Nr2 = Nr2;
+    pv.Nuc = Nuc;
+    pv.ncont = ncont;
+    pv.nshell = nshell;
+    pv.c = c;
+    pv.a = a;
+    pv.rho = rho_vec;
+    pv.fock = fock_vec;
+    pv.e[0] = e;
+    pv.e[1] = e2;
+    pv.e[2] = e3;
+    pv.wang[0] = wang;
+    pv.wang[1] = wang2;
+    pv.wang[2] = wang3;
+    ctx->opt = g_opt;
+    ctx->have_problem = true;
+    ctx->grid_type = -1;
+  });
+}
+
+void g2g_reload_atom_positions_(const unsigned* grid_type) {
+  abi("g2g_reload_atom_positions_", [&] { build(ctx_or_die("g2g_reload_atom_positions_"), *grid_type); });
+}
+
+void g2g_new_grid_(const unsigned* grid_type) {
+  abi("g2g_new_grid_", [&] {
+    xcq_context* ctx = ctx_or_die("g2g_new_grid_");
+    if ((int)*grid_type != ctx->grid_type) build(ctx, *grid_type);
+  });
+}
+
+void g2g_solve_groups_(const unsigned* computation_type, double* energy, double* forces) {
+  (void)forces;
+  abi("g2g_solve_groups_", [&] {
+    xcq_context* ctx = ctx_or_die("g2g_solve_groups_");
+    const unsigned ct = *computation_type;
+    if (ct > 3) throw std::runtime_error("unknown computation type");
+    if (ct >= 2) throw std::runtime_error("XC forces are not provided by this build");
+    if (ctx->grid_type < 0) throw std::runtime_error("solve before g2g_reload_atom_positions_");
+    const bool compute_rmm = (ct == 0);
+    const bool compute_energy = (ct == 1) || g_opt.energy_all_iterations;
+    xcq::SolveResult r = ctx->engine->solve(ctx->pv.rho, compute_rmm ? ctx->pv.fock : nullptr, compute_energy);
+    const double e = compute_energy ? r.energy : 0.0;
+    *energy = e;
+    if (std::isnan(e)) die("g2g_solve_groups_", "XC energy is NaN");
+  });
+}
+
+void g2g_deinit_(void) {
+  abi("g2g_deinit_", [] {
+    delete g_ctx;
+    g_ctx = nullptr;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+// Internal C++ interface of the MI355X XC quadrature engine.
+//
+// Layers:
+//   g2g_abi.cpp  : the Fortran-facing g2g C ABI (drop-in for libg2g.so)
+//   grid.cpp     : host grid + point-group partition (cubes and spheres of
+//                  significant basis functions, group-local Becke weights)
+//   engine.hip   : device arena, uploads, fused HIP kernels, solve
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace xcq {
+
+constexpr int kMaxContractions = 13;
+
+// g2g_set_options_ knobs and their defaults (g2g/init.cpp:394-406).
+struct Options {
+  double free_global_memory = 0.0;
+  double little_cube_size = 8.0;
+  double sphere_radius = 0.6;
+  bool assign_all_functions = false;
+  bool energy_all_iterations = false;
+  bool remove_zero_weights = true;
+  unsigned min_points_per_cube = 1;
+  unsigned max_function_exponent = 10;
+  unsigned timer_level = 0;
+  unsigned verbose = 0;
+};
+
+// The caller's arrays (Fortran layouts), referenced, not owned, exactly as
+// g2g_parameter_init_ receives them (g2g/init.cpp:69-219).
+struct ProblemView {
+  unsigned natom = 0, max_atoms = 0, M = 0, ngaussians = 0;
+  bool normalize = true;
+  unsigned iexch = 9;
+  bool open = false;
+  unsigned nco = 0, nunp = 0, nopt = 0;
+  const double* r = nullptr;        // natom x 3, ld = max_atoms
+  const double* Rm = nullptr;       // by Z
+  const unsigned* Iz = nullptr;     // natom
+  const unsigned* Nr = nullptr;     // shells by Z (grid type 0)
+  const unsigned* Nr2 = nullptr;    // shells by Z (grid types 1, 2)
+  const unsigned* Nuc = nullptr;    // M, 1-based
+  const unsigned* ncont = nullptr;  // M
+  const unsigned* nshell = nullptr; // 3
+  const double* c = nullptr;        // M x 13, ld = ngaussians
+  const double* a = nullptr;        // M x 13, ld = ngaussians
+  const double* rho = nullptr;      // packed density
+  double* fock = nullptr;           // packed Fock, accumulated (+=)
+  const double* e[3] = {nullptr, nullptr, nullptr};
+  const double* wang[3] = {nullptr, nullptr, nullptr};
+};
+
+// Basis in the engine's form: one record per expanded function.
+struct BasisFn {
+  double a[kMaxContractions];
+  double c[kMaxContractions];  // includes the NORM factor of d xx/yy/zz
+  int atom;                    // 0-based
+  int ncont;
+  int lx, ly, lz;
+  int shell_len;               // 1 (s), 3 (p), 6 (d)
+};
+
+struct Molecule {
+  int natom = 0, M = 0, ns = 0, np = 0, nd = 0;  // np/nd count shells
+  int iexch = 9;
+  bool gga = true;
+  std::vector<double> x, y, z;  // atom positions (bohr)
+  std::vector<int> Z;
+  std::vector<double> rm;       // Becke size parameter per atom
+  std::vector<int> shells;      // radial shells per atom
+  std::vector<BasisFn> fn;      // M records
+  int ngrid = 0;
+  std::vector<double> ex, ey, ez, ew;  // angular rule
+};
+
+// Builds a Molecule from the caller's arrays for grid type gt (0, 1, 2).
+// Throws std::runtime_error on inconsistent input.
+Molecule capture(const ProblemView& pv, unsigned grid_type);
+
+// The point-group partition, flattened for upload.  Groups are cubes (in
+// prism order) followed by spheres (in atom order); points keep generation
+// order inside each group.
+struct GridPartition {
+  int ngroups = 0;
+  std::vector<int> kind;     // 0 cube, 1 sphere
+  std::vector<int> npts;     // points after zero-weight removal
+  std::vector<int> pt_off;   // offset into x/y/z/w
+  std::vector<int> m;        // expanded functions
+  std::vector<int> fn_off;   // offset into fn
+  std::vector<int> nshells;  // selected shells (s + p + d)
+  std::vector<int> ns, np, nd;
+  std::vector<int> nnuc;
+  std::vector<uint64_t> fhash;  // FNV-1a of the first-component indices
+  std::vector<double> x, y, z, w;
+  std::vector<int> fn;       // global expanded function indices per group
+  double seconds = 0.0;
+  long long total_points() const { return (long long)x.size(); }
+};
+
+GridPartition build_partition(const Molecule& mol, const Options& opt);
+
+// ---------------------------------------------------------------------------
+// Device engine
+// ---------------------------------------------------------------------------
+enum class Precision : int { FP64 = 0 };
+
+struct SolveResult {
+  double energy = 0.0;
+  double electrons = 0.0;
+  double seconds = 0.0;      // wall time of the device solve incl. copies
+  double kernel_ms[4] = {0, 0, 0, 0};  // expand, density, fock, pack (events)
+};
+
+class Engine;
+
+// One molecule resident on the device: its grid, work lists and buffers.
+struct DeviceGrid;
+
+class Engine {
+ public:
+  explicit Engine(int device = -1);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Uploads basis, grid and work lists; replaces any previous grid.
+  void load(const Molecule& mol, const GridPartition& part);
+  // One XC Fock build from packed rho (host memory).  fock_accum (host) gets
+  // += F_xc; pass nullptr to skip the copy-back.
+  SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy);
+  // Same with device-resident input/output (no PCIe copies), used for timing.
+  SolveResult solve_device(const double* d_rho_packed, double* d_fock, bool want_energy);
+  // Waits for the stream and reads the last solve's energy and electron count.
+  void read_energy(double* energy, double* electrons);
+  // Waits for the stream; HIP-event durations (ms) of the last solve's phases.
+  void kernel_times(double ms[4]);
+
+  int device() const { return device_; }
+  size_t arena_bytes() const;
+  void* stream() const { return stream_; }
+  const DeviceGrid* grid() const { return grid_; }
+
+ private:
+  int device_ = 0;
+  void* stream_ = nullptr;
+  DeviceGrid* grid_ = nullptr;
+  void* events_[8] = {};
+};
+
+}  // namespace xcq

@@ -1,0 +1,305 @@
+"""Python front end of ``libg2g.so`` (ctypes; no torch types cross the boundary).
+
+Two ways in, mirroring how the library is used:
+
+* :class:`G2G` drives the Fortran-style g2g ABI exactly as lioamber does
+  (``g2g_set_options_`` -> ``g2g_init_`` -> ``g2g_parameter_init_`` ->
+  ``g2g_reload_atom_positions_`` -> ``g2g_solve_groups_``), with the caller
+  owning every array, energy assigned and the packed Fock accumulated.
+* :class:`Context` uses the handle API (``xcq_*``) for several molecules at once
+  and for timing with device-resident buffers.
+
+The shared library must be built (``__graft_entry__.build()`` or
+``make -C lio_amd/csrc``); there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libg2g.so")
+
+_dp = C.POINTER(C.c_double)
+_up = C.POINTER(C.c_uint)
+_bp = C.POINTER(C.c_bool)
+
+
+class XcqProblem(C.Structure):
+    _fields_ = [("natom", C.c_uint), ("max_atoms", C.c_uint), ("M", C.c_uint),
+                ("ngaussians", C.c_uint), ("normalize", C.c_int), ("iexch", C.c_uint),
+                ("r", _dp), ("Rm", _dp), ("Iz", _up), ("Nr", _up), ("Nr2", _up),
+                ("Nuc", _up), ("ncont", _up), ("nshell", _up), ("c", _dp), ("a", _dp),
+                ("e", _dp * 3), ("wang", _dp * 3),
+                ("little_cube_size", C.c_double), ("sphere_radius", C.c_double),
+                ("assign_all_functions", C.c_int), ("remove_zero_weights", C.c_int),
+                ("min_points_per_cube", C.c_uint), ("max_function_exponent", C.c_uint)]
+
+
+class XcqGroupInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("npoints", C.c_int), ("nfuncs", C.c_int),
+                ("s", C.c_int), ("p", C.c_int), ("d", C.c_int), ("nnuc", C.c_int),
+                ("wsum", C.c_double), ("xsum", C.c_double), ("fhash", C.c_ulonglong)]
+
+
+class XcqGridInfo(C.Structure):
+    _fields_ = [("ngroups", C.c_int), ("npoints", C.c_longlong), ("npoints_padded", C.c_longlong),
+                ("flops", C.c_double), ("bytes_values", C.c_double),
+                ("grid_seconds", C.c_double), ("upload_seconds", C.c_double),
+                ("arena_bytes", C.c_ulonglong), ("density_tiles", C.c_int),
+                ("fock_items", C.c_int)]
+
+
+class XcqSolveInfo(C.Structure):
+    _fields_ = [("energy", C.c_double), ("electrons", C.c_double), ("seconds", C.c_double),
+                ("kernel_ms", C.c_double * 4)]
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libg2g.so once; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            "%s is not built: run __graft_entry__.build() or make -C lio_amd/csrc" % path)
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.xcq_last_error.restype = C.c_char_p
+    L.xcq_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.xcq_destroy.argtypes = [vp]
+    L.xcq_set_problem.argtypes = [vp, C.POINTER(XcqProblem)]
+    L.xcq_build_grid.argtypes = [vp, C.c_uint]
+    L.xcq_query_grid.argtypes = [vp, C.POINTER(XcqGridInfo)]
+    L.xcq_groups.argtypes = [vp, C.POINTER(XcqGroupInfo), C.c_int, C.POINTER(C.c_int)]
+    L.xcq_solve.argtypes = [vp, _dp, _dp, C.POINTER(XcqSolveInfo)]
+    L.xcq_solve_device.argtypes = [vp, vp, vp]
+    L.xcq_sync_energy.argtypes = [vp, _dp, _dp]
+    L.xcq_kernel_times.argtypes = [vp, _dp]
+    L.xcq_stream.argtypes = [vp]
+    L.xcq_stream.restype = vp
+    L.xcq_host_partition.argtypes = [C.POINTER(XcqProblem), C.c_uint, C.POINTER(XcqGroupInfo),
+                                     C.c_int, C.POINTER(C.c_int)]
+    _LIB = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError("%s failed: %s" % (what, load_library().xcq_last_error().decode()))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+DEFAULT_OPTIONS = dict(little_cube_size=8.0, sphere_radius=0.6, assign_all_functions=0,
+                       remove_zero_weights=1, min_points_per_cube=1, max_function_exponent=10)
+
+
+def make_problem(system, **options):
+    """xcq_problem for a ``lio_amd.systems.System``; returns (struct, keepalive)."""
+    ab = system.abi_arrays()
+    keep = {}
+
+    def d(k, x):
+        keep[k] = np.ascontiguousarray(x, np.float64)
+        return _ptr(keep[k], C.c_double)
+
+    def u(k, x):
+        keep[k] = np.ascontiguousarray(x, np.uint32)
+        return _ptr(keep[k], C.c_uint)
+
+    p = XcqProblem()
+    p.natom = p.max_atoms = ab["natom"]
+    p.M = p.ngaussians = ab["M"]
+    p.normalize = ab["norm"]
+    p.iexch = ab["iexch"]
+    p.r, p.Rm, p.Iz = d("r", ab["r"]), d("Rm", ab["Rm"]), u("Iz", ab["Iz"])
+    p.Nr, p.Nr2 = u("Nr", ab["Nr"]), u("Nr2", ab["Nr2"])
+    p.Nuc, p.ncont, p.nshell = u("Nuc", ab["Nuc"]), u("ncont", ab["ncont"]), u("nshell", ab["nshell"])
+    p.c, p.a = d("c", ab["c"]), d("a", ab["a"])
+    for k, (en, wn) in enumerate((("e1", "w1"), ("e2", "w2"), ("e3", "w3"))):
+        p.e[k] = d(en, ab[en])
+        p.wang[k] = d(wn, ab[wn])
+    opt = dict(DEFAULT_OPTIONS)
+    opt.update(options)
+    for k, v in opt.items():
+        setattr(p, k, v)
+    return p, keep
+
+
+def _groups_to_dicts(arr):
+    return [dict(kind=g.kind, npoints=g.npoints, nfuncs=g.nfuncs, s=g.s, p=g.p, d=g.d,
+                 nnuc=g.nnuc, wsum=g.wsum, xsum=g.xsum, fhash=g.fhash) for g in arr]
+
+
+def host_partition(system, grid_type=None, **options):
+    """Group inventory of the partition the engine builds (host only, no GPU)."""
+    L = load_library()
+    p, keep = make_problem(system, **options)
+    gt = system.grid_type if grid_type is None else grid_type
+    n = C.c_int(0)
+    _check(L.xcq_host_partition(C.byref(p), gt, None, 0, C.byref(n)), "xcq_host_partition")
+    arr = (XcqGroupInfo * n.value)()
+    _check(L.xcq_host_partition(C.byref(p), gt, arr, n.value, C.byref(n)), "xcq_host_partition")
+    return _groups_to_dicts(arr)
+
+
+class Context:
+    """One molecule on the device through the handle API."""
+
+    def __init__(self, system=None, device: int = -1, **options):
+        self.L = load_library()
+        h = C.c_void_p()
+        _check(self.L.xcq_create(device, C.byref(h)), "xcq_create")
+        self.h = h
+        self.M = 0
+        if system is not None:
+            self.set_system(system, **options)
+
+    def set_system(self, system, **options):
+        self._p, self._keep = make_problem(system, **options)
+        self.M = system.M
+        self.system = system
+        _check(self.L.xcq_set_problem(self.h, C.byref(self._p)), "xcq_set_problem")
+
+    def build_grid(self, grid_type=None):
+        gt = self.system.grid_type if grid_type is None else grid_type
+        _check(self.L.xcq_build_grid(self.h, gt), "xcq_build_grid")
+        return self.grid_info()
+
+    def grid_info(self) -> dict:
+        gi = XcqGridInfo()
+        _check(self.L.xcq_query_grid(self.h, C.byref(gi)), "xcq_query_grid")
+        return {k: getattr(gi, k) for k, _ in XcqGridInfo._fields_}
+
+    def groups(self):
+        n = C.c_int(0)
+        _check(self.L.xcq_groups(self.h, None, 0, C.byref(n)), "xcq_groups")
+        arr = (XcqGroupInfo * n.value)()
+        _check(self.L.xcq_groups(self.h, arr, n.value, C.byref(n)), "xcq_groups")
+        return _groups_to_dicts(arr)
+
+    def solve(self, rho=None, fock=None):
+        """One XC Fock build; returns (energy, packed F added onto ``fock``, info)."""
+        rho = self.system.rho_packed if rho is None else rho
+        rho = np.ascontiguousarray(rho, np.float64)
+        F = np.zeros(self.M * (self.M + 1) // 2) if fock is None else fock
+        info = XcqSolveInfo()
+        _check(self.L.xcq_solve(self.h, _ptr(rho, C.c_double), _ptr(F, C.c_double), C.byref(info)),
+               "xcq_solve")
+        return info.energy, F, dict(electrons=info.electrons, seconds=info.seconds,
+                                    kernel_ms=list(info.kernel_ms))
+
+    def solve_device(self, d_rho: int, d_fock: int):
+        """Queues a solve on device pointers (integers); no synchronisation."""
+        _check(self.L.xcq_solve_device(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),
+               "xcq_solve_device")
+
+    def sync_energy(self):
+        e, n = C.c_double(), C.c_double()
+        _check(self.L.xcq_sync_energy(self.h, C.byref(e), C.byref(n)), "xcq_sync_energy")
+        return e.value, n.value
+
+    def kernel_times(self):
+        """Waits for the stream; ms of [expand, density, fock, pack] of the last solve."""
+        out = np.zeros(4)
+        _check(self.L.xcq_kernel_times(self.h, _ptr(out, C.c_double)), "xcq_kernel_times")
+        return out
+
+    def stream(self) -> int:
+        return self.L.xcq_stream(self.h) or 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xcq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class G2G:
+    """The g2g Fortran ABI driven as lioamber drives it (process-global state)."""
+
+    def __init__(self):
+        self.L = L = load_library()
+        L.g2g_set_options_.argtypes = [_dp, _dp, _dp, _bp, _bp, _bp, _up, _up, _up, _up]
+        L.g2g_reload_atom_positions_.argtypes = [_up]
+        L.g2g_new_grid_.argtypes = [_up]
+        L.g2g_solve_groups_.argtypes = [_up, _dp, _dp]
+
+    def setup(self, system, energy_all_iterations=True, timer_level=0, verbose=0, **options):
+        L = self.L
+        opt = dict(DEFAULT_OPTIONS)
+        opt.update(options)
+        k = self._keep = {}
+
+        def D(v):
+            a = np.array([v], np.float64)
+            k[id(a)] = a
+            return _ptr(a, C.c_double)
+
+        def U(v):
+            a = np.array([v], np.uint32)
+            k[id(a)] = a
+            return _ptr(a, C.c_uint)
+
+        def B(v):
+            a = (C.c_bool * 1)(bool(v))
+            k[id(a)] = a
+            return C.cast(a, _bp)
+
+        L.g2g_set_options_(D(0.0), D(opt["little_cube_size"]), D(opt["sphere_radius"]),
+                           B(opt["assign_all_functions"]), B(energy_all_iterations),
+                           B(opt["remove_zero_weights"]), U(opt["min_points_per_cube"]),
+                           U(opt["max_function_exponent"]), U(timer_level), U(verbose))
+        L.g2g_init_()
+        ab = system.abi_arrays()
+        self.M = ab["M"]
+        arr = self._arr = {}
+        for name in ("r", "Rm", "a", "c", "rho", "e1", "e2", "e3", "w1", "w2", "w3"):
+            arr[name] = np.ascontiguousarray(ab[name], np.float64)
+        for name in ("Iz", "Nr", "Nr2", "Nuc", "ncont", "nshell"):
+            arr[name] = np.ascontiguousarray(ab[name], np.uint32)
+        packed = self.M * (self.M + 1) // 2
+        arr["fock"] = np.zeros(packed)
+        arr["fockb"] = np.zeros(1)
+        dp = lambda n: _ptr(arr[n], C.c_double)
+        up = lambda n: _ptr(arr[n], C.c_uint)
+        args = [U(ab["norm"]), U(ab["natom"]), U(ab["max_atoms"]), U(ab["ngaussians"]),
+                dp("r"), dp("Rm"), up("Iz"), up("Nr"), up("Nr2"), up("Nuc"), U(ab["M"]),
+                up("ncont"), up("nshell"), dp("c"), dp("a"), dp("rho"), dp("fock"), dp("fockb"),
+                None, None, U(ab["nco"]), B(False), U(0), U(0), U(ab["iexch"]),
+                dp("e1"), dp("e2"), dp("e3"), dp("w1"), dp("w2"), dp("w3"), B(False), U(0), U(0)]
+        L.g2g_parameter_init_(*args)
+        self._grid_type = U(ab["grid_type"])
+        L.g2g_reload_atom_positions_(self._grid_type)
+
+    @property
+    def rho(self) -> np.ndarray:
+        """The caller-owned packed density the library reads at each solve."""
+        return self._arr["rho"]
+
+    @property
+    def fock(self) -> np.ndarray:
+        """The caller-owned packed Fock the library accumulates into."""
+        return self._arr["fock"]
+
+    def solve(self, computation_type: int = 0) -> float:
+        e = np.zeros(1)
+        ct = np.array([computation_type], np.uint32)
+        self.L.g2g_solve_groups_(_ptr(ct, C.c_uint), _ptr(e, C.c_double), None)
+        return float(e[0])
+
+    def deinit(self):
+        self.L.g2g_deinit_()

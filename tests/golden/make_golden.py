@@ -1,0 +1,70 @@
+"""Golden vectors from LIO's own CPU g2g path (FP64).
+
+Runs the reference harness (oracle/_ref/libg2g_ref.so, built from the LIO
+sources by `make -C oracle ref`) on seeded systems from lio_amd.systems and
+stores, per case: the XC energy, the integrated electron count, the packed
+XC Fock (sparse: exact zeros are pairs of functions that share no group) and
+the group inventory.  The inputs are regenerated from the recorded seeds; a
+checksum of the packed density guards against drift.
+
+    python tests/golden/make_golden.py          (takes about a minute)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from lio_amd import systems  # noqa: E402
+
+# name -> (builder, iexch)
+MOLECULES = [(3, 101), (10, 102), (25, 103), (45, 104)]
+
+
+def cases():
+    yield "c0_pbe", lambda: systems.config0()
+    for ix in (1, 2, 3):
+        def b(ix=ix):
+            s = systems.config0()
+            s.iexch = ix
+            return s
+        yield "c0_lda%d" % ix, b
+    yield "c1_pbe", lambda: systems.config1()
+    for n, seed in MOLECULES:
+        yield "mol%d_s%d_pbe" % (n, seed), (lambda n=n, seed=seed: systems.random_molecule(n, seed))
+
+
+GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsum", "fhash")
+
+
+def main():
+    from pyxc import RefG2G
+    ref = RefG2G()
+    for name, builder in cases():
+        s = builder()
+        ref.setup(s)
+        E, F, t = ref.solve()
+        N = ref.electrons()
+        groups = ref.groups()
+        nz = np.nonzero(F)[0]
+        out = dict(
+            energy=E, electrons=N, M=s.M, natom=s.natom, iexch=s.iexch,
+            rho_sum=float(s.rho_packed.sum()), rho_norm=float(np.linalg.norm(s.rho_packed)),
+            fock_index=nz.astype(np.int64), fock_value=F[nz],
+            solve_seconds=t,
+        )
+        for f in GROUP_FIELDS:
+            out["g_" + f] = np.array([g[f] for g in groups],
+                                     dtype=np.uint64 if f == "fhash" else None)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print("%-18s M=%4d groups=%4d Exc=% .12f N=%.9f nnz(F)=%d" %
+              (name, s.M, len(groups), E, N, len(nz)), flush=True)
+    ref.teardown()
+
+
+if __name__ == "__main__":
+    main()

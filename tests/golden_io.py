@@ -1,0 +1,66 @@
+"""Loading the golden vectors (tests/golden/*.npz) and their input systems."""
+import os
+
+import numpy as np
+
+from lio_amd import systems
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsum", "fhash")
+
+
+def system_for(name: str):
+    if name.startswith("c0_"):
+        s = systems.config0()
+        if name.startswith("c0_lda"):
+            s.iexch = int(name[len("c0_lda")])
+        return s
+    if name == "c1_pbe":
+        return systems.config1()
+    if name.startswith("mol"):
+        n, seed = name[3:].split("_pbe")[0].split("_s")
+        return systems.random_molecule(int(n), int(seed))
+    raise KeyError(name)
+
+
+def load(name: str):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    M = int(z["M"])
+    F = np.zeros(M * (M + 1) // 2)
+    F[z["fock_index"]] = z["fock_value"]
+    groups = []
+    for k in range(len(z["g_kind"])):
+        groups.append({f: (int(z["g_" + f][k]) if f != "wsum" and f != "xsum" else float(z["g_" + f][k]))
+                       for f in GROUP_FIELDS})
+    return dict(energy=float(z["energy"]), electrons=float(z["electrons"]), fock=F, M=M,
+                groups=groups, rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]))
+
+
+def names():
+    return sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def group_key(g):
+    return (g["kind"], g["npoints"], g["nfuncs"], g["fhash"], g["nnuc"], g["s"], g["p"], g["d"])
+
+
+def same_partition(a, b, wtol=1e-12):
+    """True when two group inventories hold the same groups (order-free)."""
+    if len(a) != len(b):
+        return False
+    sa = sorted(a, key=lambda g: (group_key(g), g["wsum"]))
+    sb = sorted(b, key=lambda g: (group_key(g), g["wsum"]))
+    for x, y in zip(sa, sb):
+        if group_key(x) != group_key(y):
+            return False
+        if abs(x["wsum"] - y["wsum"]) > wtol * max(1.0, abs(y["wsum"])):
+            return False
+        if abs(x["xsum"] - y["xsum"]) > wtol * max(1.0, abs(y["xsum"])):
+            return False
+    return True
+
+
+def check_inputs(s, gold):
+    """The regenerated density must be the one the fixture was made from."""
+    assert abs(s.rho_packed.sum() - gold["rho_sum"]) <= 1e-9 * max(1.0, abs(gold["rho_sum"]))
+    assert abs(np.linalg.norm(s.rho_packed) - gold["rho_norm"]) <= 1e-9 * gold["rho_norm"]
