@@ -83,9 +83,9 @@ def main():
     ctx = g2g.Context(s)
     t0 = time.time()
     info = ctx.build_grid()
-    log("grid: %d groups, %d points, %.3f TFLOP/build, host partition %.2f s, upload %.2f s"
+    log("grid: %d groups, %d points, %.3f TFLOP/build, host partition %.2f s, upload+eval %.2f s, eval kernel %.3f ms"
         % (info["ngroups"], info["npoints"], info["flops"] / 1e12, info["grid_seconds"],
-           info["upload_seconds"]))
+           info["upload_seconds"], info["eval_ms"]))
     rho = torch.from_numpy(s.rho_packed.copy()).to(dev)
     F = torch.zeros_like(rho)
     torch.cuda.synchronize()

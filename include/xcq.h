@@ -66,6 +66,7 @@ typedef struct {
   double bytes_values;        /* algorithmic 2*10*n*m*8 evaluation bytes */
   double grid_seconds;        /* host partition build */
   double upload_seconds;
+  double eval_ms;             /* basis evaluation kernel (HIP events) */
   unsigned long long arena_bytes;
   int density_tiles, fock_items;
 } xcq_grid_info;

@@ -177,6 +177,7 @@ int xcq_query_grid(xcq_context* ctx, xcq_grid_info* o) {
     o->bytes_values = by;
     o->grid_seconds = part.seconds;
     o->upload_seconds = ctx->upload_seconds;
+    o->eval_ms = ctx->engine->eval_ms();
     o->arena_bytes = ctx->engine->arena_bytes();
     o->density_tiles = (int)(pad / 16);
     o->fock_items = 0;

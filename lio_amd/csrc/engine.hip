@@ -1,21 +1,27 @@
 // Device engine for the XC Fock build on MI355X (gfx950), FP64 path.
 //
-// One solve = five launches on one stream, all reading one device arena:
-//   k_expand_density   packed rho (off-diagonals x2) -> dense symmetric P
-//   k_density          per 16-point tile: Y = [phi; dphi/dx; dphi/dy; dphi/dz] P
-//                      on FP64 MFMA (16x16x4), basis values computed in-kernel,
-//                      row-dot epilogue -> rho, grad rho, Hessian of rho ->
-//                      PBE/LDA -> factor_p = w_p v_p, per-tile Exc and N.
-//                      Also stores phi(p, j) for the Fock pass.
-//   k_fock             F_g tile += phi^T diag(factor) phi over a 256-point
-//                      slice of the group (FP64 MFMA), one partial per slice
+// Grid build (once per geometry, like the reference's cached GPU groups,
+// g2g/cuda/iteration.cu:792-881):
+//   k_eval_basis       phi, grad phi and the Hessian of every significant basis
+//                      function at every point of its group, written once to
+//                      HBM as planes [value][function][point] (points
+//                      contiguous, coalesced).  HBM-bound.
+// Per XC Fock build (one solve):
+//   k_gather_density   packed rho -> dense symmetric block P_g per group
+//   k_density          Y_t^T = P_g Phi_t^T for t = phi, d/dx, d/dy, d/dz on
+//                      FP64 MFMA (16x16x4), 64 points per block; row-dot
+//                      epilogue -> rho, grad rho, Hessian of rho -> PBE/LDA
+//                      -> factor_p = w_p v_p and per-block (Exc, N)
+//   k_fock             F_g tile (64 x 64) over a 128-point slice,
+//                      phi^T diag(factor) phi on FP64 MFMA, one partial/slice
 //   k_fock_group       sums the slices of each tile in fixed order
-//   k_fock_pack        packed F_IJ = sum over groups holding I and J, in fixed
-//                      group order (no float atomics: bitwise reproducible)
-// plus a one-block reduction of the per-tile energies.
+//   k_fock_pack        packed F_IJ = sum over groups holding I and J, in
+//                      ascending group order (no float atomics)
+//   k_energy           fixed-order reduction of the per-block (Exc, N)
+// Every reduction has a fixed order, so repeated solves are bitwise equal.
 //
 // The maths follows g2g/cpu/iteration.cpp:116-291 (closed shell): with P the
-// true symmetric density (packed off-diagonals halved),
+// true symmetric density (packed off-diagonals halved) and Y = Phi P,
 //   rho = sum_i phi_i Y_i,  d_a rho = 2 sum_i d_a phi_i Y_i,
 //   d_a d_b rho = 2 sum_i (d_ab phi_i Y_i + d_a phi_i (Y_b)_i),
 //   F_ij += sum_p phi_i phi_j w_p y2a_p,  Exc = sum_p rho_p w_p (ex + ec).
@@ -45,10 +51,12 @@ namespace xcq {
 
 namespace {
 
-constexpr int kTile = 16;        // points per density tile (x4 derivative rows)
-constexpr int kBlock = 64;       // columns per density column block / Fock tile
-constexpr int kChunk = 16;       // K chunk staged through LDS
-constexpr int kSlice = 256;      // points per Fock slice (split-K)
+constexpr int kPtAlign = 16;     // point padding (one MFMA column tile)
+constexpr int kFnAlign = 16;     // function padding (one MFMA row tile)
+constexpr int kDensPts = 64;     // points per density block (16 per wave)
+constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
+constexpr int kSlice = 128;      // points per Fock slice (split-K)
+constexpr int kEvalPts = 256;    // points per evaluation block
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -60,29 +68,31 @@ struct DevFn {
 };
 
 struct DevGroup {
-  int pt_off;      // first padded point
-  int npts;        // real points
-  int npts_pad;    // multiple of kTile
-  int fn_off;      // first entry of the padded function list
-  int m;           // real functions
-  int m_pad;       // multiple of 16
-  int ntile;       // 64-column tiles per side
-  int nslice;      // Fock slices
-  long long phi_off;   // phi block: npts_pad x m_pad
-  long long fg_off;    // F_g block: m_pad x m_pad
-  int fock_base;   // first Fock item
+  int pt_off;          // first padded point
+  int npts;            // real points
+  int npts_pad;        // multiple of kPtAlign
+  int fn_off;          // first entry of the padded function list
+  int m;               // real functions
+  int m_pad;           // multiple of kFnAlign
+  int ntile;           // kFnChunk tiles per side
+  int nslice;          // Fock slices
+  long long val_off;   // value planes: nval x m_pad x npts_pad
+  long long pg_off;    // P_g / F_g block: m_pad x m_pad
+  int fock_base;       // first Fock item
   int pad_;
 };
 
-struct DensityItem {
-  int group;
-  int p0;  // first point (group-local)
+struct EvalItem {
+  int group, k, p0, pad_;
 };
-
+struct DensityItem {
+  int group, p0;
+};
 struct FockItem {
-  int group;
-  int ti, tj;
-  int slice;
+  int group, ti, tj, slice;
+};
+struct GatherItem {
+  int group, start;  // first element of the m_pad^2 block handled
 };
 
 struct Axis {
@@ -96,8 +106,8 @@ __device__ inline Axis axis(double v, int l) {
 }
 
 // Contracted Cartesian Gaussian x^lx y^ly z^lz sum_k c_k exp(-a_k r^2) and
-// its derivatives (g2g/cpu/functions.cpp:56-354): NV = 1 value, 4 adds the
-// gradient, 10 adds the Hessian (xx, yy, zz, xy, xz, yz).  Primitives with
+// its derivatives (g2g/cpu/functions.cpp:56-354): NV = 1 value, 10 adds the
+// gradient and the Hessian (xx, yy, zz, xy, xz, yz).  Primitives with
 // a r^2 > 70 are skipped as in the reference.
 template <int NV>
 __device__ inline void eval_basis(const DevFn& b, double px, double py, double pz, double* o) {
@@ -110,8 +120,10 @@ __device__ inline void eval_basis(const DevFn& b, double px, double py, double p
     if (a * r2 > 70.0) continue;
     const double e = exp(-a * r2) * b.c[k];
     t += e;
-    if (NV > 1) tg += e * a;
-    if (NV > 4) th += e * (a * a);
+    if (NV > 1) {
+      tg += e * a;
+      th += e * (a * a);
+    }
   }
   const Axis X = axis(vx, b.lx), Y = axis(vy, b.ly), Z = axis(vz, b.lz);
   const double P = X.m * Y.m * Z.m;
@@ -122,7 +134,6 @@ __device__ inline void eval_basis(const DevFn& b, double px, double py, double p
   o[1] = Px * t + P * gx;
   o[2] = Py * t + P * gy;
   o[3] = Pz * t + P * gz;
-  if (NV == 4) return;
   const double Pxx = X.dd * Y.m * Z.m, Pyy = X.m * Y.dd * Z.m, Pzz = X.m * Y.m * Z.dd;
   const double Pxy = X.d * Y.d * Z.m, Pxz = X.d * Y.m * Z.d, Pyz = X.m * Y.d * Z.d;
   o[4] = Pxx * t + 2.0 * Px * gx + P * (4.0 * vx * vx * th - 2.0 * tg);
@@ -133,9 +144,10 @@ __device__ inline void eval_basis(const DevFn& b, double px, double py, double p
   o[9] = Pyz * t + Py * gz + Pz * gy + P * (4.0 * vy * vz * th);
 }
 
-__device__ inline double mfma64(double a, double b, d4& c) {
+// FP64 16x16x4 MFMA.  Operands: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the result D[r][c] sits in lane (r%4)*16 + c, item r/4.
+__device__ inline void mfma64(double a, double b, d4& c) {
   c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  return 0.0;
 }
 
 // Packed index of (i, j), i <= j, column-major lower triangle (partition.cpp:132-134).
@@ -143,276 +155,248 @@ __device__ __host__ inline long long packed_index(long long i, long long j, long
   return i * M - (i * (i - 1)) / 2 + (j - i);
 }
 
-__global__ void k_expand_density(const double* __restrict__ rho, double* __restrict__ P, int M) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)M * M) return;
-  const int i = (int)(idx / M), j = (int)(idx % M);
-  const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const double v = rho[packed_index(lo, hi, M)];
-  P[idx] = (i == j) ? v : 0.5 * v;
+// ---------------------------------------------------------------------------
+// basis evaluation: one function x 256 points per block
+// ---------------------------------------------------------------------------
+template <int NV>
+__global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__ items,
+                                                    const DevGroup* __restrict__ groups,
+                                                    const int* __restrict__ fnl,
+                                                    const DevFn* __restrict__ bas,
+                                                    const double* __restrict__ px,
+                                                    const double* __restrict__ py,
+                                                    const double* __restrict__ pz,
+                                                    double* __restrict__ vals) {
+  const EvalItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int p = it.p0 + threadIdx.x;
+  if (p >= g.npts_pad) return;
+  const int f = fnl[g.fn_off + it.k];
+  double v[NV];
+  if (f >= 0) {
+    const int q = g.pt_off + p;
+    eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
+  } else {
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  }
+  double* out = vals + g.val_off + (long long)it.k * g.npts_pad + p;
+  const long long plane = (long long)g.m_pad * g.npts_pad;
+  for (int k = 0; k < NV; ++k) out[k * plane] = v[k];
 }
 
-// One 16-point tile of one group.  256 threads = 4 waves.  The FP64 16x16x4
-// MFMA leaves D[r][c] in lane (r % 4) * 16 + c, item r / 4 (lane l holds
-// rows l/16 + 4j); A-operand row r of row-tile rt is therefore the
-// derivative t = r / 4 at point 4 rt + r % 4, which puts the 4 derivative
-// rows of one point in the 4 accumulator items of one lane.
-__global__ __launch_bounds__(256) void k_density_gga(
-    const DensityItem* __restrict__ items, const DevGroup* __restrict__ groups,
-    const int* __restrict__ fnl, const DevFn* __restrict__ bas, const double* __restrict__ px,
-    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pw,
-    const double* __restrict__ P, int M, double* __restrict__ phi, double* __restrict__ factor,
-    double* __restrict__ epart) {
-  __shared__ double sA[kTile][4][kChunk + 1];
-  __shared__ double sB[kChunk][kBlock + 1];
-  __shared__ double sPart[4][kTile][10];
-  __shared__ int sCol[kBlock];
+// ---------------------------------------------------------------------------
+// density blocks P_g[k][j] = P(f_k, f_j) with the true symmetric P
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __restrict__ items,
+                                                        const DevGroup* __restrict__ groups,
+                                                        const int* __restrict__ fnl,
+                                                        const double* __restrict__ rho, int M,
+                                                        double* __restrict__ pg) {
+  const GatherItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int e = it.start + threadIdx.x;
+  if (e >= g.m_pad * g.m_pad) return;
+  const int k = e / g.m_pad, j = e % g.m_pad;
+  const int fk = fnl[g.fn_off + k], fj = fnl[g.fn_off + j];
+  double v = 0.0;
+  if (fk >= 0 && fj >= 0) {
+    const int lo = min(fk, fj), hi = max(fk, fj);
+    v = rho[packed_index(lo, hi, M)];
+    if (lo != hi) v *= 0.5;
+  }
+  pg[g.pg_off + e] = v;
+}
 
+// ---------------------------------------------------------------------------
+// density + functional: 64 points per block, 16 per wave
+// ---------------------------------------------------------------------------
+// Wave w owns points p0 = 64 b + 16 w (the 16 columns of its MFMA tiles) and
+// walks the functions in chunks of 64 (4 row tiles).  For each row tile rt
+// and derivative t it accumulates (Y_t)^T[j][p] = sum_k P[j][k] Phi_t[k][p];
+// lane l then holds, for point p0 + l%16, the four functions
+// j = 16 rt + l/16 + 4 i (i = item) of all NT derivatives, and contracts them
+// with the values of those functions at that point (10 row-dot partials).
+template <int NT>
+__global__ __launch_bounds__(256) void k_density(const DensityItem* __restrict__ items,
+                                                 const DevGroup* __restrict__ groups,
+                                                 const double* __restrict__ pg,
+                                                 const double* __restrict__ vals,
+                                                 const double* __restrict__ pw,
+                                                 double* __restrict__ factor,
+                                                 double* __restrict__ epart, int iexch) {
+  __shared__ double sE[4][2];
+  constexpr int NS = (NT == 4) ? 10 : 1;
   const DensityItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lrow = lane & 15, lk = lane >> 4;
-  const int* fl = fnl + g.fn_off;
-  const int pbase = g.pt_off + it.p0;
-
-  for (int k = tid; k < 4 * kTile * 10; k += 256) (&sPart[0][0][0])[k] = 0.0;
-
-  for (int c0 = 0; c0 < g.m_pad; c0 += kBlock) {
-    __syncthreads();
-    if (tid < kBlock) sCol[tid] = (c0 + tid < g.m_pad) ? fl[c0 + tid] : -1;
-    const bool active = (c0 + 16 * wave) < g.m_pad;
-    d4 acc[4];
-    for (int rt = 0; rt < 4; ++rt) acc[rt] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.m_pad; k0 += kChunk) {
-      __syncthreads();
-      {  // A: phi and gradient of function k0+kk at point pp
-        const int pp = tid >> 4, kk = tid & 15;
-        const int f = fl[k0 + kk];
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        if (f >= 0) {
-          const int q = pbase + pp;
-          eval_basis<4>(bas[f], px[q], py[q], pz[q], v);
+  const int p0 = it.p0 + 16 * wave;
+  double e_w = 0.0, n_w = 0.0;
+  if (p0 < g.npts_pad) {
+    const int np = g.npts_pad, mp = g.m_pad;
+    const long long plane = (long long)mp * np;
+    const double* V = vals + g.val_off;
+    const double* Pm = pg + g.pg_off;
+    double s[NS];
+    for (int k = 0; k < NS; ++k) s[k] = 0.0;
+    for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
+      const int nrt = min(4, (mp - c0) / 16);
+      d4 acc[4][NT];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[rt][t] = d4{0.0, 0.0, 0.0, 0.0};
+      const double* Bk = V + (long long)lk * np + p0 + lrow;   // Phi_t[k][p]
+      const double* Ak = Pm + (long long)lk * mp + c0 + lrow;  // P[k][j] = P[j][k]
+      // operands of step k0 in (a, b); the next step's are loaded before the
+      // MFMAs of this one so the loads overlap the matrix pipe
+      double a[4], b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b[t] = Bk[t * plane];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) a[rt] = (rt < nrt) ? Ak[16 * rt] : 0.0;
+      for (int k0 = 0; k0 < mp; k0 += 4) {
+        double an[4], bn[NT];
+        const bool more = k0 + 4 < mp;
+        const long long kb = (long long)(more ? k0 + 4 : k0);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bn[t] = Bk[t * plane + kb * np];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) an[rt] = (rt < nrt) ? Ak[kb * mp + 16 * rt] : 0.0;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          if (rt < nrt) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) mfma64(a[rt], b[t], acc[rt][t]);
+          }
         }
-        for (int t = 0; t < 4; ++t) sA[pp][t][kk] = v[t];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[t] = bn[t];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) a[rt] = an[rt];
       }
-      for (int e = tid; e < kChunk * kBlock; e += 256) {  // B: P rows k, columns of the block
-        const int kk = e / kBlock, cc = e % kBlock;
-        const int fr = fl[k0 + kk], fc = sCol[cc];
-        sB[kk][cc] = (fr >= 0 && fc >= 0) ? P[(long long)fr * M + fc] : 0.0;
-      }
-      __syncthreads();
-      if (active) {
-        for (int ks = 0; ks < kChunk / 4; ++ks) {
-          const double b = sB[4 * ks + lk][16 * wave + lrow];
-          for (int rt = 0; rt < 4; ++rt) {
-            const double a = sA[4 * rt + (lrow & 3)][lrow >> 2][4 * ks + lk];
-            mfma64(a, b, acc[rt]);
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        if (rt >= nrt) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int j = c0 + 16 * rt + lk + 4 * i;
+          const double* vj = V + (long long)j * np + p0 + lrow;
+          if constexpr (NT == 4) {
+            double v[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) v[q] = vj[q * plane];
+            const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
+                         yz = acc[rt][3][i];
+            s[0] += v[0] * y0;
+            s[1] += v[1] * y0;
+            s[2] += v[2] * y0;
+            s[3] += v[3] * y0;
+            s[4] += v[4] * y0 + v[1] * yx;
+            s[5] += v[5] * y0 + v[2] * yy;
+            s[6] += v[6] * y0 + v[3] * yz;
+            s[7] += v[7] * y0 + v[1] * yy;
+            s[8] += v[8] * y0 + v[1] * yz;
+            s[9] += v[9] * y0 + v[2] * yz;
+          } else {
+            s[0] += vj[0] * acc[rt][0][i];
           }
         }
       }
     }
-    if (active) {
-      const int cl = 16 * wave + lrow;
-      const int col = c0 + cl;
-      const int f = sCol[cl];
-      for (int rt = 0; rt < 4; ++rt) {
-        const int pp = 4 * rt + lk;
-        const int q = pbase + pp;
-        double v[10];
-        if (f >= 0) {
-          eval_basis<10>(bas[f], px[q], py[q], pz[q], v);
-        } else {
-          for (int k = 0; k < 10; ++k) v[k] = 0.0;
-        }
-        phi[g.phi_off + (long long)(it.p0 + pp) * g.m_pad + col] = v[0];
-        const double y0 = acc[rt][0], yx = acc[rt][1], yy = acc[rt][2], yz = acc[rt][3];
-        double s[10];
-        s[0] = v[0] * y0;
-        s[1] = v[1] * y0;
-        s[2] = v[2] * y0;
-        s[3] = v[3] * y0;
-        s[4] = v[4] * y0 + v[1] * yx;
-        s[5] = v[5] * y0 + v[2] * yy;
-        s[6] = v[6] * y0 + v[3] * yz;
-        s[7] = v[7] * y0 + v[1] * yy;
-        s[8] = v[8] * y0 + v[1] * yz;
-        s[9] = v[9] * y0 + v[2] * yz;
-        for (int k = 0; k < 10; ++k) {
-          double x = s[k];
-          x += __shfl_xor(x, 1);
-          x += __shfl_xor(x, 2);
-          x += __shfl_xor(x, 4);
-          x += __shfl_xor(x, 8);
-          s[k] = x;
-        }
-        if (lrow == 0)
-          for (int k = 0; k < 10; ++k) sPart[wave][pp][k] += s[k];
-      }
+    for (int k = 0; k < NS; ++k) {  // the 4 lane groups holding the same point
+      s[k] += __shfl_xor(s[k], 16);
+      s[k] += __shfl_xor(s[k], 32);
     }
-  }
-  __syncthreads();
-  if (tid < 64) {
-    double e = 0.0, n = 0.0;
-    if (tid < kTile) {
-      const int pp = tid;
-      double t[10];
-      for (int k = 0; k < 10; ++k)
-        t[k] = sPart[0][pp][k] + sPart[1][pp][k] + sPart[2][pp][k] + sPart[3][pp][k];
-      const double dens = t[0];
-      const double gr[3] = {2.0 * t[1], 2.0 * t[2], 2.0 * t[3]};
-      const double h1[3] = {2.0 * t[4], 2.0 * t[5], 2.0 * t[6]};
-      const double h2[3] = {2.0 * t[7], 2.0 * t[8], 2.0 * t[9]};
+    if (lk == 0) {
+      const int q = g.pt_off + p0 + lrow;
+      const double dens = s[0];
       double ex, ec, y2a;
-      xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
-      const int q = pbase + pp;
+      if constexpr (NT == 4) {
+        const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
+        const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
+        const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
+        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+      } else {
+        xc::lda_point(dens, iexch, ex, ec, y2a);
+      }
       const double w = pw[q];
       factor[q] = w * y2a;
-      e = (dens * w) * (ex + ec);
-      n = dens * w;
+      e_w = (dens * w) * (ex + ec);
+      n_w = dens * w;
     }
     for (int o = 8; o >= 1; o >>= 1) {
-      e += __shfl_xor(e, o);
-      n += __shfl_xor(n, o);
+      e_w += __shfl_xor(e_w, o);
+      n_w += __shfl_xor(n_w, o);
     }
-    if (tid == 0) {
-      epart[2 * blockIdx.x] = e;
-      epart[2 * blockIdx.x + 1] = n;
-    }
+  }
+  if (lane == 0) {
+    sE[wave][0] = e_w;
+    sE[wave][1] = n_w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    epart[2 * blockIdx.x] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
+    epart[2 * blockIdx.x + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
   }
 }
 
-// LDA variant: only phi enters; rows of the A operand are 16 points.
-__global__ __launch_bounds__(256) void k_density_lda(
-    const DensityItem* __restrict__ items, const DevGroup* __restrict__ groups,
-    const int* __restrict__ fnl, const DevFn* __restrict__ bas, const double* __restrict__ px,
-    const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ pw,
-    const double* __restrict__ P, int M, double* __restrict__ phi, double* __restrict__ factor,
-    double* __restrict__ epart, int iexch) {
-  __shared__ double sA[kTile][kChunk + 1];
-  __shared__ double sB[kChunk][kBlock + 1];
-  __shared__ double sPart[4][kTile];
-  __shared__ int sCol[kBlock];
-  const DensityItem it = items[blockIdx.x];
-  const DevGroup g = groups[it.group];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int lrow = lane & 15, lk = lane >> 4;
-  const int* fl = fnl + g.fn_off;
-  const int pbase = g.pt_off + it.p0;
-  if (tid < 4 * kTile) (&sPart[0][0])[tid] = 0.0;
-  for (int c0 = 0; c0 < g.m_pad; c0 += kBlock) {
-    __syncthreads();
-    if (tid < kBlock) sCol[tid] = (c0 + tid < g.m_pad) ? fl[c0 + tid] : -1;
-    const bool active = (c0 + 16 * wave) < g.m_pad;
-    d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.m_pad; k0 += kChunk) {
-      __syncthreads();
-      {
-        const int pp = tid >> 4, kk = tid & 15;
-        const int f = fl[k0 + kk];
-        double v = 0.0;
-        if (f >= 0) {
-          const int q = pbase + pp;
-          eval_basis<1>(bas[f], px[q], py[q], pz[q], &v);
-        }
-        sA[pp][kk] = v;
-      }
-      for (int e = tid; e < kChunk * kBlock; e += 256) {
-        const int kk = e / kBlock, cc = e % kBlock;
-        const int fr = fl[k0 + kk], fc = sCol[cc];
-        sB[kk][cc] = (fr >= 0 && fc >= 0) ? P[(long long)fr * M + fc] : 0.0;
-      }
-      __syncthreads();
-      if (active) {
-        for (int ks = 0; ks < kChunk / 4; ++ks) {
-          const double b = sB[4 * ks + lk][16 * wave + lrow];
-          const double a = sA[lrow][4 * ks + lk];
-          mfma64(a, b, acc);
-        }
-      }
-    }
-    if (active) {
-      const int cl = 16 * wave + lrow;
-      const int col = c0 + cl;
-      const int f = sCol[cl];
-      for (int j = 0; j < 4; ++j) {
-        const int pp = lk + 4 * j;  // D row lk + 4 j
-        const int q = pbase + pp;
-        double v = 0.0;
-        if (f >= 0) eval_basis<1>(bas[f], px[q], py[q], pz[q], &v);
-        phi[g.phi_off + (long long)(it.p0 + pp) * g.m_pad + col] = v;
-        double x = v * acc[j];
-        x += __shfl_xor(x, 1);
-        x += __shfl_xor(x, 2);
-        x += __shfl_xor(x, 4);
-        x += __shfl_xor(x, 8);
-        if (lrow == 0) sPart[wave][pp] += x;
-      }
-    }
-  }
-  __syncthreads();
-  if (tid < 64) {
-    double e = 0.0, n = 0.0;
-    if (tid < kTile) {
-      const double dens = sPart[0][tid] + sPart[1][tid] + sPart[2][tid] + sPart[3][tid];
-      double ex, ec, y2a;
-      xc::lda_point(dens, iexch, ex, ec, y2a);
-      const int q = pbase + tid;
-      const double w = pw[q];
-      factor[q] = w * y2a;
-      e = (dens * w) * (ex + ec);
-      n = dens * w;
-    }
-    for (int o = 8; o >= 1; o >>= 1) {
-      e += __shfl_xor(e, o);
-      n += __shfl_xor(n, o);
-    }
-    if (tid == 0) {
-      epart[2 * blockIdx.x] = e;
-      epart[2 * blockIdx.x + 1] = n;
-    }
-  }
-}
-
-// Fock tile (ti, tj) of one group over one slice of points:
-// out[i][j] = sum_p phi(p, i) factor_p phi(p, j), 64 x 64, FP64 MFMA.
-__global__ __launch_bounds__(256) void k_fock(const FockItem* __restrict__ items,
-                                              const DevGroup* __restrict__ groups,
-                                              const double* __restrict__ phi,
-                                              const double* __restrict__ factor,
-                                              double* __restrict__ fpart) {
-  __shared__ double sI[kTile][kBlock + 1];
-  __shared__ double sJ[kTile][kBlock + 1];
+// ---------------------------------------------------------------------------
+// Fock: one wave per (group, tile pair, 128-point slice)
+// ---------------------------------------------------------------------------
+// out[i][j] = sum_p phi_i(p) factor_p phi_j(p) for i in tile ti, j in tile tj.
+// The K order is permuted so that the four K slots a lane feeds to four
+// consecutive MFMAs are four consecutive points: one 32-byte load per lane
+// and operand row instead of four scattered 8-byte loads.
+__global__ __launch_bounds__(64) void k_fock(const FockItem* __restrict__ items,
+                                             const DevGroup* __restrict__ groups,
+                                             const double* __restrict__ vals,
+                                             const double* __restrict__ factor,
+                                             double* __restrict__ fpart) {
   const FockItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int lrow = lane & 15, lk = lane >> 4;
-  const int i0 = kBlock * it.ti, j0 = kBlock * it.tj;
+  const int lane = threadIdx.x, lrow = lane & 15, lk = lane >> 4;
+  const int np = g.npts_pad, mp = g.m_pad;
+  const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
+  const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
   const int pb = it.slice * kSlice;
-  const int pe = min(pb + kSlice, g.npts_pad);
-  const double* ph = phi + g.phi_off;
-  d4 acc[4];
-  for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int p0 = pb; p0 < pe; p0 += kTile) {
-    __syncthreads();
-    for (int e = tid; e < kTile * kBlock; e += 256) {
-      const int pp = e / kBlock, cc = e % kBlock;
-      const long long row = (long long)(p0 + pp) * g.m_pad;
-      const double fq = factor[g.pt_off + p0 + pp];
-      sI[pp][cc] = (i0 + cc < g.m_pad) ? ph[row + i0 + cc] : 0.0;
-      sJ[pp][cc] = (j0 + cc < g.m_pad) ? fq * ph[row + j0 + cc] : 0.0;
+  const int pe = min(pb + kSlice, np);
+  const double* V = vals + g.val_off;
+  const double* fac = factor + g.pt_off;
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int p = pb + 4 * lk; p < pe; p += 16) {
+    const d4 f = *reinterpret_cast<const d4*>(fac + p);
+    d4 A[4], B[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      A[r] = (r < nri) ? *reinterpret_cast<const d4*>(V + (long long)(i0 + 16 * r + lrow) * np + p)
+                       : d4{0.0, 0.0, 0.0, 0.0};
+      B[r] = (r < nrj) ? *reinterpret_cast<const d4*>(V + (long long)(j0 + 16 * r + lrow) * np + p) * f
+                       : d4{0.0, 0.0, 0.0, 0.0};
     }
-    __syncthreads();
-    for (int ks = 0; ks < kTile / 4; ++ks) {
-      const double a = sI[4 * ks + lk][16 * wave + lrow];
-      for (int ct = 0; ct < 4; ++ct) {
-        const double b = sJ[4 * ks + lk][16 * ct + lrow];
-        mfma64(a, b, acc[ct]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r < nri) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if (c < nrj) mfma64(A[r][s], B[c][s], acc[r][c]);
+        }
       }
-    }
   }
-  double* out = fpart + (long long)blockIdx.x * kBlock * kBlock;
-  for (int ct = 0; ct < 4; ++ct)
-    for (int j = 0; j < 4; ++j) out[(16 * wave + lk + 4 * j) * kBlock + 16 * ct + lrow] = acc[ct][j];
+  double* out = fpart + (long long)blockIdx.x * kFnChunk * kFnChunk;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        out[(16 * r + lk + 4 * i) * kFnChunk + 16 * c + lrow] = acc[r][c][i];
 }
 
 // F_g tile (ti, tj) = sum over slices, in slice order.
@@ -422,7 +406,6 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
                                                     double* __restrict__ fg) {
   const int2 gt = tiles[blockIdx.x];  // x: group, y: tile-pair index
   const DevGroup g = groups[gt.x];
-  // tile pair index -> (ti, tj), tj >= ti, row-major over the upper triangle
   int ti = 0, rem = gt.y;
   while (rem >= g.ntile - ti) {
     rem -= g.ntile - ti;
@@ -430,13 +413,13 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
   }
   const int tj = ti + rem;
   const int base = g.fock_base + gt.y * g.nslice;
-  for (int e = threadIdx.x; e < kBlock * kBlock; e += 256) {
-    const int r = e / kBlock, c = e % kBlock;
-    const int i = kBlock * ti + r, j = kBlock * tj + c;
+  for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) {
+    const int r = e / kFnChunk, c = e % kFnChunk;
+    const int i = kFnChunk * ti + r, j = kFnChunk * tj + c;
     if (i >= g.m_pad || j >= g.m_pad) continue;
     double s = 0.0;
-    for (int sl = 0; sl < g.nslice; ++sl) s += fpart[(long long)(base + sl) * kBlock * kBlock + e];
-    fg[g.fg_off + (long long)i * g.m_pad + j] = s;
+    for (int sl = 0; sl < g.nslice; ++sl) s += fpart[(long long)(base + sl) * kFnChunk * kFnChunk + e];
+    fg[g.pg_off + (long long)i * g.m_pad + j] = s;
   }
 }
 
@@ -461,7 +444,7 @@ __global__ void k_fock_pack(const int* __restrict__ memb_off, const int2* __rest
     } else {
       const DevGroup g = groups[u.x];
       const int li = min(u.y, v.y), lj = max(u.y, v.y);
-      s += fg[g.fg_off + (long long)li * g.m_pad + lj];
+      s += fg[g.pg_off + (long long)li * g.m_pad + lj];
       ++a;
       ++b;
     }
@@ -469,7 +452,7 @@ __global__ void k_fock_pack(const int* __restrict__ memb_off, const int2* __rest
   F[packed_index(I, J, M)] = s;
 }
 
-// Fixed-order reduction of the per-tile (Exc, N) pairs.
+// Fixed-order reduction of the per-block (Exc, N) pairs.
 __global__ __launch_bounds__(256) void k_energy(const double* __restrict__ epart, int n,
                                                 double* __restrict__ out) {
   __shared__ double se[256], sn[256];
@@ -505,34 +488,33 @@ double wall() {
 // Device-resident molecule: every buffer lives in one arena allocation.
 // ---------------------------------------------------------------------------
 struct DeviceGrid {
-  int M = 0, natom = 0, iexch = 9;
+  int M = 0, natom = 0, iexch = 9, nval = 10;
   bool gga = true;
-  int ngroups = 0, ndensity = 0, nfock = 0, ngroup_tiles = 0;
-  long long npts_pad = 0, phi_elems = 0, fg_elems = 0;
-  long long npts_real = 0;
-  double flops = 0.0;  // 10 n m^2 over groups (algorithmic)
+  int ngroups = 0, neval = 0, ngather = 0, ndensity = 0, nfock = 0, ngroup_tiles = 0;
+  long long npts_pad = 0, val_elems = 0, pg_elems = 0;
   char* arena = nullptr;
   size_t arena_bytes = 0;
-  // device pointers into the arena
+  double eval_ms = 0.0;
   DevFn* bas = nullptr;
   DevGroup* groups = nullptr;
+  EvalItem* eitems = nullptr;
+  GatherItem* gitems = nullptr;
   DensityItem* ditems = nullptr;
   FockItem* fitems = nullptr;
   int2* gtiles = nullptr;
   int* fnl = nullptr;
   double *px = nullptr, *py = nullptr, *pz = nullptr, *pw = nullptr;
-  double* P = nullptr;
   double* rho = nullptr;
   double* F = nullptr;
-  double* phi = nullptr;
+  double* vals = nullptr;
+  double* pg = nullptr;  // P_g blocks (per solve)
+  double* fg = nullptr;  // F_g blocks (per solve)
   double* factor = nullptr;
   double* epart = nullptr;
   double* fpart = nullptr;
-  double* fg = nullptr;
   int* memb_off = nullptr;
   int2* memb = nullptr;
   double* eout = nullptr;
-  // pinned host staging
   double* h_rho = nullptr;
   double* h_F = nullptr;
   double* h_e = nullptr;
@@ -590,6 +572,7 @@ Engine::~Engine() {
 }
 
 size_t Engine::arena_bytes() const { return grid_ ? grid_->arena_bytes : 0; }
+double Engine::eval_ms() const { return grid_ ? grid_->eval_ms : 0.0; }
 
 void Engine::load(const Molecule& mol, const GridPartition& part) {
   XCQ_HIP(hipSetDevice(device_));
@@ -601,10 +584,10 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   dg->natom = mol.natom;
   dg->iexch = mol.iexch;
   dg->gga = mol.gga;
+  dg->nval = mol.gga ? 10 : 1;
   const int G = part.ngroups, M = mol.M;
   dg->ngroups = G;
 
-  // host images ------------------------------------------------------------
   std::vector<DevFn> bas(M);
   for (int f = 0; f < M; ++f) {
     const BasisFn& b = mol.fn[f];
@@ -622,22 +605,22 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   std::vector<DevGroup> groups(G);
   std::vector<int> fnl;
   std::vector<double> hx, hy, hz, hw;
-  long long pt = 0, phi = 0, fgo = 0;
+  long long pt = 0, vo = 0, po = 0;
   int fock_base = 0;
   std::vector<std::vector<int2>> membership(M);
   for (int g = 0; g < G; ++g) {
     DevGroup& d = groups[g];
     const int n = part.npts[g], m = part.m[g];
     d.npts = n;
-    d.npts_pad = (n + kTile - 1) / kTile * kTile;
+    d.npts_pad = (n + kPtAlign - 1) / kPtAlign * kPtAlign;
     d.m = m;
-    d.m_pad = (m + 15) / 16 * 16;
+    d.m_pad = (m + kFnAlign - 1) / kFnAlign * kFnAlign;
     d.pt_off = (int)pt;
     d.fn_off = (int)fnl.size();
-    d.ntile = (d.m_pad + kBlock - 1) / kBlock;
+    d.ntile = (d.m_pad + kFnChunk - 1) / kFnChunk;
     d.nslice = (d.npts_pad + kSlice - 1) / kSlice;
-    d.phi_off = phi;
-    d.fg_off = fgo;
+    d.val_off = vo;
+    d.pg_off = po;
     d.fock_base = fock_base;
     d.pad_ = 0;
     fock_base += d.ntile * (d.ntile + 1) / 2 * d.nslice;
@@ -654,25 +637,30 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
       hw.push_back(p < n ? part.w[src] : 0.0);
     }
     pt += d.npts_pad;
-    phi += (long long)d.npts_pad * d.m_pad;
-    fgo += (long long)d.m_pad * d.m_pad;
-    dg->flops += 10.0 * n * (double)m * m;
-    dg->npts_real += n;
+    vo += (long long)dg->nval * d.m_pad * d.npts_pad;
+    po += (long long)d.m_pad * d.m_pad;
   }
   dg->npts_pad = pt;
-  dg->phi_elems = phi;
-  dg->fg_elems = fgo;
+  dg->val_elems = vo;
+  dg->pg_elems = po;
   dg->nfock = fock_base;
 
-  // work lists: heavy groups first (longest-processing-time order)
+  // evaluation, gather and density work: heavy groups first
   std::vector<int> order(G);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return (long long)groups[a].m_pad * groups[a].m_pad > (long long)groups[b].m_pad * groups[b].m_pad;
-  });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return groups[a].m_pad > groups[b].m_pad; });
+  std::vector<EvalItem> eitems;
+  std::vector<GatherItem> gitems;
   std::vector<DensityItem> ditems;
-  for (int g : order)
-    for (int p0 = 0; p0 < groups[g].npts_pad; p0 += kTile) ditems.push_back(DensityItem{g, p0});
+  for (int g : order) {
+    const DevGroup& d = groups[g];
+    for (int k = 0; k < d.m_pad; ++k)
+      for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalPts) eitems.push_back(EvalItem{g, k, p0, 0});
+    for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
+    for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) ditems.push_back(DensityItem{g, p0});
+  }
+  // Fock partial k belongs to (group, tile pair, slice) in slot order
   std::vector<FockItem> fitems(fock_base);
   std::vector<int2> gtiles;
   for (int g = 0; g < G; ++g) {
@@ -685,6 +673,8 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
           fitems[d.fock_base + pair * d.nslice + s] = FockItem{g, ti, tj, s};
       }
   }
+  dg->neval = (int)eitems.size();
+  dg->ngather = (int)gitems.size();
   dg->ndensity = (int)ditems.size();
   dg->ngroup_tiles = (int)gtiles.size();
   std::vector<int> moff(M + 1, 0);
@@ -695,11 +685,12 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   }
   moff[M] = (int)memb.size();
 
-  // arena --------------------------------------------------------------------
   const size_t packed = (size_t)M * (M + 1) / 2;
   auto plan = [&](Carver& c) {
     dg->bas = c.take<DevFn>(M);
     dg->groups = c.take<DevGroup>(G);
+    dg->eitems = c.take<EvalItem>(eitems.size());
+    dg->gitems = c.take<GatherItem>(gitems.size());
     dg->ditems = c.take<DensityItem>(ditems.size());
     dg->fitems = c.take<FockItem>(fitems.size());
     dg->gtiles = c.take<int2>(gtiles.size());
@@ -708,14 +699,14 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
     dg->py = c.take<double>(pt);
     dg->pz = c.take<double>(pt);
     dg->pw = c.take<double>(pt);
-    dg->P = c.take<double>((size_t)M * M);
     dg->rho = c.take<double>(packed);
     dg->F = c.take<double>(packed);
-    dg->phi = c.take<double>(phi);
+    dg->vals = c.take<double>(vo);
+    dg->pg = c.take<double>(po);
+    dg->fg = c.take<double>(po);
     dg->factor = c.take<double>(pt);
     dg->epart = c.take<double>(2 * ditems.size());
-    dg->fpart = c.take<double>((size_t)fitems.size() * kBlock * kBlock);
-    dg->fg = c.take<double>(fgo);
+    dg->fpart = c.take<double>((size_t)fitems.size() * kFnChunk * kFnChunk);
     dg->memb_off = c.take<int>(M + 1);
     dg->memb = c.take<int2>(memb.size() + 1);
     dg->eout = c.take<double>(2);
@@ -723,6 +714,15 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   Carver sizing{nullptr};
   plan(sizing);
   dg->arena_bytes = sizing.used + 256;
+  size_t free_b = 0, total_b = 0;
+  XCQ_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (dg->arena_bytes > free_b) {
+    const size_t need = dg->arena_bytes;
+    delete dg;
+    throw std::runtime_error("g2g: grid needs " + std::to_string(need >> 20) +
+                             " MiB of device memory, " + std::to_string(free_b >> 20) +
+                             " MiB free");
+  }
   XCQ_HIP(hipMalloc(&dg->arena, dg->arena_bytes));
   Carver carve{dg->arena};
   plan(carve);
@@ -736,6 +736,8 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   };
   up(dg->bas, bas.data(), bas.size() * sizeof(DevFn));
   up(dg->groups, groups.data(), groups.size() * sizeof(DevGroup));
+  up(dg->eitems, eitems.data(), eitems.size() * sizeof(EvalItem));
+  up(dg->gitems, gitems.data(), gitems.size() * sizeof(GatherItem));
   up(dg->ditems, ditems.data(), ditems.size() * sizeof(DensityItem));
   up(dg->fitems, fitems.data(), fitems.size() * sizeof(FockItem));
   up(dg->gtiles, gtiles.data(), gtiles.size() * sizeof(int2));
@@ -746,7 +748,23 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   up(dg->pw, hw.data(), hw.size() * sizeof(double));
   up(dg->memb_off, moff.data(), moff.size() * sizeof(int));
   up(dg->memb, memb.data(), memb.size() * sizeof(int2));
+  // basis values of every group, kept for all solves on this grid
+  hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
+  XCQ_HIP(hipEventRecord(ev[6], s));
+  if (dg->neval > 0) {
+    if (dg->gga)
+      hipLaunchKernelGGL(k_eval_basis<10>, dim3(dg->neval), dim3(kEvalPts), 0, s, dg->eitems,
+                         dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->vals);
+    else
+      hipLaunchKernelGGL(k_eval_basis<1>, dim3(dg->neval), dim3(kEvalPts), 0, s, dg->eitems,
+                         dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->vals);
+    XCQ_HIP(hipGetLastError());
+  }
+  XCQ_HIP(hipEventRecord(ev[7], s));
   XCQ_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  XCQ_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
+  dg->eval_ms = ms;
   grid_ = dg;
 }
 
@@ -758,25 +776,22 @@ SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_
   SolveResult res;
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
   XCQ_HIP(hipEventRecord(ev[0], s));
-  {
-    const long long n = (long long)M * M;
-    hipLaunchKernelGGL(k_expand_density, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_rho,
-                       g.P, M);
-  }
+  if (g.ngather > 0)
+    hipLaunchKernelGGL(k_gather_density, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
+                       g.fnl, d_rho, M, g.pg);
   XCQ_HIP(hipEventRecord(ev[1], s));
   if (g.ndensity > 0) {
     if (g.gga)
-      hipLaunchKernelGGL(k_density_gga, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
-                         g.fnl, g.bas, g.px, g.py, g.pz, g.pw, g.P, M, g.phi, g.factor, g.epart);
+      hipLaunchKernelGGL(k_density<4>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups, g.pg,
+                         g.vals, g.pw, g.factor, g.epart, g.iexch);
     else
-      hipLaunchKernelGGL(k_density_lda, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
-                         g.fnl, g.bas, g.px, g.py, g.pz, g.pw, g.P, M, g.phi, g.factor, g.epart,
-                         g.iexch);
+      hipLaunchKernelGGL(k_density<1>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups, g.pg,
+                         g.vals, g.pw, g.factor, g.epart, g.iexch);
   }
   XCQ_HIP(hipGetLastError());
   XCQ_HIP(hipEventRecord(ev[2], s));
-  if (g.nfock > 0) {
-    hipLaunchKernelGGL(k_fock, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.phi,
+  if (d_fock && g.nfock > 0) {
+    hipLaunchKernelGGL(k_fock, dim3(g.nfock), dim3(64), 0, s, g.fitems, g.groups, g.vals,
                        g.factor, g.fpart);
     hipLaunchKernelGGL(k_fock_group, dim3(g.ngroup_tiles), dim3(256), 0, s, g.groups, g.gtiles,
                        g.fpart, g.fg);

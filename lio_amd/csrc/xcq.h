@@ -141,6 +141,8 @@ class Engine {
 
   int device() const { return device_; }
   size_t arena_bytes() const;
+  // Duration (ms, HIP events) of the basis evaluation at the last load.
+  double eval_ms() const;
   void* stream() const { return stream_; }
   const DeviceGrid* grid() const { return grid_; }
 

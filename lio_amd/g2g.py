@@ -48,7 +48,7 @@ class XcqGridInfo(C.Structure):
     _fields_ = [("ngroups", C.c_int), ("npoints", C.c_longlong), ("npoints_padded", C.c_longlong),
                 ("flops", C.c_double), ("bytes_values", C.c_double),
                 ("grid_seconds", C.c_double), ("upload_seconds", C.c_double),
-                ("arena_bytes", C.c_ulonglong), ("density_tiles", C.c_int),
+                ("eval_ms", C.c_double), ("arena_bytes", C.c_ulonglong), ("density_tiles", C.c_int),
                 ("fock_items", C.c_int)]
 
 
