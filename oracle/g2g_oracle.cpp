@@ -889,4 +889,46 @@ int oracle_basis_values(const XcInput* in, int npts, const double* pts, int M, d
   }
 }
 
+// Vectorised point functionals for tests: arrays of n points; grad, h1 (xx,
+// yy, zz) and h2 (xy, xz, yz) are n x 3 row-major.
+void oracle_gga_points(int n, const double* rho, const double* grad, const double* h1,
+                       const double* h2, double* ex, double* ec, double* y2a) {
+  for (int i = 0; i < n; ++i) gga_point(rho[i], grad + 3 * i, h1 + 3 * i, h2 + 3 * i, ex[i], ec[i], y2a[i]);
+}
+
+void oracle_lda_points(int n, const double* rho, int iexch, double* ex, double* ec, double* y2a) {
+  for (int i = 0; i < n; ++i) lda_point(rho[i], iexch, ex[i], ec[i], y2a[i]);
+}
+
+void oracle_pw92(double srs, double* e, double* de_drs) { pw92(srs, *e, *de_drs); }
+
+// Grid export for tests: total point count, then points (x, y, z, w) and the
+// group index of each point, groups in cost-sorted order.
+long long oracle_grid_points(void* hp, double* xyzw, int* group) {
+  auto* h = static_cast<Handle*>(hp);
+  long long k = 0;
+  for (size_t g = 0; g < h->groups.size(); ++g) {
+    const Group& G = h->groups[g];
+    for (size_t p = 0; p < G.pts.size(); ++p, ++k) {
+      if (xyzw) {
+        xyzw[4 * k + 0] = G.pts[p].x;
+        xyzw[4 * k + 1] = G.pts[p].y;
+        xyzw[4 * k + 2] = G.pts[p].z;
+        xyzw[4 * k + 3] = G.w[p];
+      }
+      if (group) group[k] = (int)g;
+    }
+  }
+  return k;
+}
+
+// Expanded function list of group g (global indices); returns its length.
+int oracle_group_functions(void* hp, int g, int* out) {
+  auto* h = static_cast<Handle*>(hp);
+  const Group& G = h->groups[g];
+  if (out)
+    for (int i = 0; i < G.m(); ++i) out[i] = G.fglobal[i];
+  return G.m();
+}
+
 }  // extern "C"

@@ -162,6 +162,16 @@ class Oracle(_Checker):
         L.oracle_lda_point.restype = None
         L.oracle_basis_values.argtypes = [C.POINTER(XcInput), C.c_int, _dp, C.c_int, _dp]
         L.oracle_basis_values.restype = C.c_int
+        L.oracle_gga_points.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_gga_points.restype = None
+        L.oracle_lda_points.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp]
+        L.oracle_lda_points.restype = None
+        L.oracle_pw92.argtypes = [C.c_double, _dp, _dp]
+        L.oracle_pw92.restype = None
+        L.oracle_grid_points.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
+        L.oracle_grid_points.restype = C.c_longlong
+        L.oracle_group_functions.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.oracle_group_functions.restype = C.c_int
         self.h = None
 
     def setup(self, system, **options):
@@ -225,6 +235,45 @@ class Oracle(_Checker):
                                   _ptr(h2, C.c_double), iexch, C.byref(ex), C.byref(ec),
                                   C.byref(v))
         return ex.value, ec.value, v.value
+
+    def gga_points(self, rho, grad, hess1, hess2):
+        rho = np.ascontiguousarray(rho, np.float64)
+        n = len(rho)
+        g = np.ascontiguousarray(grad, np.float64).reshape(n, 3)
+        h1 = np.ascontiguousarray(hess1, np.float64).reshape(n, 3)
+        h2 = np.ascontiguousarray(hess2, np.float64).reshape(n, 3)
+        ex, ec, v = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.lib.oracle_gga_points(n, _ptr(rho, C.c_double), _ptr(g, C.c_double),
+                                   _ptr(h1, C.c_double), _ptr(h2, C.c_double),
+                                   _ptr(ex, C.c_double), _ptr(ec, C.c_double), _ptr(v, C.c_double))
+        return ex, ec, v
+
+    def lda_points(self, rho, iexch):
+        rho = np.ascontiguousarray(rho, np.float64)
+        n = len(rho)
+        ex, ec, v = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.lib.oracle_lda_points(n, _ptr(rho, C.c_double), iexch, _ptr(ex, C.c_double),
+                                   _ptr(ec, C.c_double), _ptr(v, C.c_double))
+        return ex, ec, v
+
+    def pw92(self, srs):
+        e, d = C.c_double(), C.c_double()
+        self.lib.oracle_pw92(srs, C.byref(e), C.byref(d))
+        return e.value, d.value
+
+    def grid(self):
+        """(points x,y,z,w as (n,4), group index per point, function lists per group)."""
+        n = self.lib.oracle_grid_points(self.h, None, None)
+        xyzw = np.zeros((n, 4))
+        grp = np.zeros(n, np.int32)
+        self.lib.oracle_grid_points(self.h, _ptr(xyzw, C.c_double), grp.ctypes.data_as(C.POINTER(C.c_int)))
+        fl = []
+        for g in range(int(grp.max()) + 1 if n else 0):
+            m = self.lib.oracle_group_functions(self.h, g, None)
+            a = np.zeros(m, np.int32)
+            self.lib.oracle_group_functions(self.h, g, a.ctypes.data_as(C.POINTER(C.c_int)))
+            fl.append(a)
+        return xyzw, grp, fl
 
     def lda_point(self, rho, iexch):
         ex, ec, v = C.c_double(), C.c_double(), C.c_double()

@@ -1,0 +1,108 @@
+"""The CPU oracle against the golden vectors of LIO's own CPU g2g path, and
+(where LIO's sources were built into oracle/_ref) against that path directly
+on more inputs and option settings."""
+import os
+
+import numpy as np
+import pytest
+
+from golden_io import check_inputs, load, same_partition, system_for
+from lio_amd import systems
+from pyxc import REF_SO, Oracle, RefG2G
+
+# The restatement follows the reference step by step: results agree to
+# rounding (observed ~1e-15 relative); 1e-10 leaves room for summation order.
+REL = 1e-10
+
+
+def _compare(E, F, N, gold):
+    assert abs(E - gold["energy"]) <= REL * abs(gold["energy"])
+    assert abs(N - gold["electrons"]) <= REL * abs(gold["electrons"])
+    assert np.max(np.abs(F - gold["fock"])) <= REL * max(1.0, np.max(np.abs(gold["fock"])))
+
+
+@pytest.mark.parametrize("name", ["c0_pbe", "c0_lda1", "c0_lda2", "c0_lda3", "mol3_s101_pbe",
+                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe"])
+def test_oracle_matches_golden(name):
+    gold = load(name)
+    s = system_for(name)
+    check_inputs(s, gold)
+    o = Oracle()
+    o.setup(s)
+    assert same_partition(o.groups(), gold["groups"])
+    E, F, _ = o.solve()
+    _compare(E, F, o.electrons(), gold)
+
+
+def test_oracle_matches_golden_water_cluster():
+    """(H2O)32, M = 800: about 15 s on 8 cores."""
+    gold = load("c1_pbe")
+    s = system_for("c1_pbe")
+    check_inputs(s, gold)
+    o = Oracle()
+    o.setup(s)
+    assert same_partition(o.groups(), gold["groups"])
+    E, F, _ = o.solve()
+    _compare(E, F, o.electrons(), gold)
+
+
+needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO),
+                               reason="LIO's CPU path is built only where its sources exist")
+
+
+@needs_ref
+@pytest.mark.parametrize("case", [
+    dict(n=6, seed=21, opts={}),
+    dict(n=14, seed=22, opts=dict(sphere_radius=0.0)),
+    dict(n=9, seed=23, opts=dict(little_cube_size=5.5)),
+    dict(n=5, seed=24, opts=dict(assign_all_functions=1)),
+    dict(n=11, seed=25, opts=dict(max_function_exponent=8, min_points_per_cube=20)),
+    dict(n=7, seed=26, opts=dict(sphere_radius=0.4, remove_zero_weights=0)),
+])
+def test_oracle_matches_reference_options(case):
+    s = systems.random_molecule(case["n"], case["seed"])
+    ref = RefG2G()
+    ref.setup(s, **case["opts"])
+    E_r, F_r, _ = ref.solve()
+    N_r = ref.electrons()
+    g_r = ref.groups()
+    ref.teardown()
+    o = Oracle()
+    o.setup(s, **case["opts"])
+    E, F, _ = o.solve()
+    assert same_partition(o.groups(), g_r)
+    assert abs(E - E_r) <= REL * abs(E_r)
+    assert abs(o.electrons() - N_r) <= REL * N_r
+    assert np.max(np.abs(F - F_r)) <= REL * max(1.0, np.max(np.abs(F_r)))
+
+
+@needs_ref
+@pytest.mark.parametrize("grid_type,iexch", [(0, 9), (1, 9), (2, 2)])
+def test_oracle_matches_reference_grids(grid_type, iexch):
+    s = systems.random_molecule(8, 31)
+    s.grid_type = grid_type
+    s.iexch = iexch
+    ref = RefG2G()
+    ref.setup(s)
+    E_r, F_r, _ = ref.solve()
+    ref.teardown()
+    o = Oracle()
+    o.setup(s)
+    E, F, _ = o.solve()
+    assert abs(E - E_r) <= REL * abs(E_r)
+    assert np.max(np.abs(F - F_r)) <= REL * max(1.0, np.max(np.abs(F_r)))
+
+
+@needs_ref
+def test_oracle_rereads_density():
+    s = systems.random_molecule(6, 41)
+    ref = RefG2G()
+    ref.setup(s)
+    rho2 = s.rho_packed * 0.7
+    E_r, F_r, _ = ref.solve(rho2)
+    ref.teardown()
+    o = Oracle()
+    o.setup(s)
+    E, F, _ = o.solve(rho2)
+    assert abs(E - E_r) <= REL * abs(E_r)
+    assert np.max(np.abs(F - F_r)) <= REL * max(1.0, np.max(np.abs(F_r)))

@@ -1,0 +1,223 @@
+"""Pins the CPU oracle with what the mathematics fixes (no reference needed).
+
+* Lebedev rules integrate low-order spherical polynomials exactly.
+* Basis gradients and Hessians equal finite differences of the values; the
+  analytic overlap equals a numerical quadrature of the same functions.
+* PBE at zero gradient reduces to Slater exchange (with the reference's
+  single-precision constant) plus PW92, and PW92's d/drs matches a finite
+  difference.
+* The GGA potential y2a is the functional derivative of E = int rho eps: the
+  derivative of E along a density perturbation equals int y2a * delta rho.
+* LDA potentials equal d(rho eps)/d rho.
+* Energy, Fock and electron count of a whole solve equal a brute-force numpy
+  contraction over the oracle's own grid, and N approaches tr(P S) = 2 n_occ.
+Each check is chosen so that a dropped term, a wrong sign or a transposed
+operand in the oracle fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from lio_amd import systems
+from pyxc import Oracle
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Oracle()
+
+
+@pytest.mark.parametrize("n", [50, 116, 194])
+def test_lebedev_integrates_polynomials(n):
+    e, w = systems.lebedev(n)
+    x, y, z = e.T
+    four_pi = 4 * math.pi
+    assert abs(w.sum() / four_pi - 1) < 1e-11
+    assert np.allclose(np.linalg.norm(e, axis=1), 1.0, atol=1e-11)
+    # <x^2> = 1/3, <x^4> = 1/5, <x^2 y^2> = 1/15, <x^6> = 1/7, odd moments 0
+    for f, exact in ((x ** 2, 1 / 3), (x ** 4, 1 / 5), (x ** 2 * y ** 2, 1 / 15),
+                     (x ** 6, 1 / 7), (x * y, 0.0), (x ** 3 * z, 0.0)):
+        assert abs((w * f).sum() / four_pi - exact) < 1e-10
+
+
+def test_basis_derivatives_match_finite_differences(oracle):
+    s = systems.config0()
+    rng = np.random.default_rng(5)
+    pts = rng.normal(scale=1.2, size=(6, 3)) + s.pos[0]
+    h = 1e-5
+    v = oracle.basis_values(s, pts)
+    for ax in range(3):
+        dp = pts.copy()
+        dm = pts.copy()
+        dp[:, ax] += h
+        dm[:, ax] -= h
+        vp = oracle.basis_values(s, dp)
+        vm = oracle.basis_values(s, dm)
+        grad_fd = (vp[:, :, 0] - vm[:, :, 0]) / (2 * h)
+        assert np.allclose(v[:, :, 1 + ax], grad_fd, rtol=1e-6, atol=1e-8)
+        # Hessian column ax: d/d ax of the gradient components
+        hfd = (vp[:, :, 1:4] - vm[:, :, 1:4]) / (2 * h)
+        diag = {0: 4, 1: 5, 2: 6}
+        assert np.allclose(v[:, :, diag[ax]], hfd[:, :, ax], rtol=1e-6, atol=1e-7)
+    # mixed: xy = d/dy of d/dx, xz = d/dz of d/dx, yz = d/dz of d/dy
+    for (a, b), col in (((0, 1), 7), ((0, 2), 8), ((1, 2), 9)):
+        dp = pts.copy()
+        dm = pts.copy()
+        dp[:, b] += h
+        dm[:, b] -= h
+        hfd = (oracle.basis_values(s, dp)[:, :, 1 + a] - oracle.basis_values(s, dm)[:, :, 1 + a]) / (2 * h)
+        assert np.allclose(v[:, :, col], hfd, rtol=1e-6, atol=1e-7)
+
+
+def test_overlap_equals_quadrature_of_basis_values(oracle):
+    """Analytic S (Obara-Saika) of a single atom's functions vs a fine
+    one-centre quadrature of the oracle's basis values."""
+    Z, p = np.array([8]), np.zeros((1, 3))
+    s = systems.make_system("O", np.array([8, 1]), np.array([[0, 0, 0], [0, 0, 30.0]]), nocc=5)
+    # restrict to the oxygen functions (the H is 30 A away, outside any overlap)
+    o_funcs = np.where(s.basis.nuc == 1)[0]
+    e, w = systems.lebedev(194)
+    nr = 120
+    i = np.arange(1, nr + 1)
+    x = np.cos(i * math.pi / (nr + 1))
+    rm = 0.6
+    r = rm * (1 + x) / (1 - x)
+    wr = (math.pi / (nr + 1)) * np.sin(i * math.pi / (nr + 1)) * r ** 2 * 2 * rm / (1 - x) ** 2
+    pts = (r[:, None, None] * e[None, :, :]).reshape(-1, 3)
+    wt = (wr[:, None] * w[None, :]).reshape(-1)
+    vals = oracle.basis_values(s, pts)[:, o_funcs, 0]
+    Sq = (vals * wt[:, None]).T @ vals
+    assert np.allclose(Sq, s.S[np.ix_(o_funcs, o_funcs)], atol=2e-7)
+
+
+def test_pbe_zero_gradient_is_slater_plus_pw92(oracle):
+    ax_f = float(np.float32(-0.738558766382022405884230032680836))
+    for rho in (1e-3, 0.05, 1.0, 20.0):
+        ex, ec, v = oracle.gga_point(rho, [0, 0, 0], [0, 0, 0], [0, 0, 0])
+        assert abs(ex - ax_f * rho ** (1 / 3)) < 1e-13 * abs(ex)
+        rs = (3 / (4 * math.pi * rho)) ** (1 / 3)
+        e_pw, _ = oracle.pw92(math.sqrt(rs))
+        assert abs(ec - e_pw) < 1e-14 + 1e-12 * abs(e_pw)
+
+
+def test_pw92_derivative(oracle):
+    for rs in (0.1, 1.0, 3.0, 10.0):
+        h = 1e-6 * rs
+        ep, _ = oracle.pw92(math.sqrt(rs + h))
+        em, _ = oracle.pw92(math.sqrt(rs - h))
+        _, d = oracle.pw92(math.sqrt(rs))
+        assert abs(d - (ep - em) / (2 * h)) < 1e-7 * max(1.0, abs(d))
+
+
+def _gaussians(pts, centers, alphas, coefs):
+    """Sum of c exp(-a |r-R|^2): value, gradient (n,3), Hessian diag and mixed (n,3)."""
+    rho = np.zeros(len(pts))
+    g = np.zeros((len(pts), 3))
+    h1 = np.zeros((len(pts), 3))
+    h2 = np.zeros((len(pts), 3))
+    for R, a, c in zip(centers, alphas, coefs):
+        d = pts - R
+        f = c * np.exp(-a * (d ** 2).sum(1))
+        rho += f
+        g += (-2 * a * d) * f[:, None]
+        h1 += (4 * a * a * d ** 2 - 2 * a) * f[:, None]
+        h2 += np.stack([4 * a * a * d[:, 0] * d[:, 1], 4 * a * a * d[:, 0] * d[:, 2],
+                        4 * a * a * d[:, 1] * d[:, 2]], 1) * f[:, None]
+    return rho, g, h1, h2
+
+
+def _sphere_grid(nr=150, rm=1.0):
+    e, w = systems.lebedev(194)
+    i = np.arange(1, nr + 1)
+    x = np.cos(i * math.pi / (nr + 1))
+    r = rm * (1 + x) / (1 - x)
+    wr = (math.pi / (nr + 1)) * np.sin(i * math.pi / (nr + 1)) * r ** 2 * 2 * rm / (1 - x) ** 2
+    pts = (r[:, None, None] * e[None, :, :]).reshape(-1, 3)
+    wt = (wr[:, None] * w[None, :]).reshape(-1)
+    return pts, wt
+
+
+def test_gga_potential_is_functional_derivative(oracle):
+    pts, wt = _sphere_grid()
+    c0 = dict(centers=[np.array([0, 0, 0.35]), np.array([0, 0, -0.35])], alphas=[1.1, 0.8],
+              coefs=[0.9, 1.3])
+    dr = dict(centers=[np.array([0.2, -0.1, 0.15])], alphas=[0.9], coefs=[0.3])
+
+    def energy(lam):
+        r0, g0, a0, b0 = _gaussians(pts, **c0)
+        r1, g1, a1, b1 = _gaussians(pts, **dr)
+        rho, g, h1, h2 = r0 + lam * r1, g0 + lam * g1, a0 + lam * a1, b0 + lam * b1
+        ex, ec, v = oracle.gga_points(rho, g, h1, h2)
+        return (wt * rho * (ex + ec)).sum(), v, r1
+
+    lam = 1e-4
+    ep, _, _ = energy(lam)
+    em, _, _ = energy(-lam)
+    _, v, d = energy(0.0)
+    fd = (ep - em) / (2 * lam)
+    analytic = (wt * v * d).sum()
+    assert abs(fd - analytic) < 1e-6 * abs(analytic)
+
+
+@pytest.mark.parametrize("iexch", [1, 2, 3])
+def test_lda_potential_is_derivative(oracle, iexch):
+    rho = np.array([1e-4, 1e-2, 0.3, 2.0, 50.0])
+    h = 1e-6 * rho
+    exp_, ecp, _ = oracle.lda_points(rho + h, iexch)
+    exm, ecm, _ = oracle.lda_points(rho - h, iexch)
+    _, _, v = oracle.lda_points(rho, iexch)
+    fd = ((rho + h) * (exp_ + ecp) - (rho - h) * (exm + ecm)) / (2 * h)
+    if iexch == 3:
+        # LIO's VWN potential (g2g/pointxc/calc_ldaCS.h:92-97, constant
+        # POT_VOSKO_B1X0) carries 1 - (x0/X0)(b - 2 x0) where the derivative of
+        # its own energy has 1 - (x0/X0)(b + 2 x0); the oracle keeps that for
+        # parity, so the potential is the derivative plus exactly this term.
+        A, b, c, x0, Q = 0.03109205, 3.72744, 12.9352, -0.10498, 6.15199066246304849
+        X0 = x0 * x0 + b * x0 + c
+        x = np.sqrt(0.620350490899400087 / np.cbrt(rho))
+        t1 = 2 * x + b
+        fd = fd + (A / 6) * x * (4 * b / (t1 * t1 + Q * Q)) * (x0 / X0) * (4 * x0)
+    assert np.allclose(v, fd, rtol=1e-7)
+
+
+@pytest.mark.parametrize("iexch", [9, 1])
+def test_solve_equals_brute_force_contraction(oracle, iexch):
+    s = systems.config0()
+    s.iexch = iexch
+    oracle.setup(s)
+    E, Fp, _ = oracle.solve()
+    N = oracle.electrons()
+    xyzw, grp, flists = oracle.grid()
+    vals = oracle.basis_values(s, xyzw[:, :3])  # all functions at all points
+    P = s.P
+    E_bf, N_bf = 0.0, 0.0
+    F = np.zeros((s.M, s.M))
+    for g, fl in enumerate(flists):
+        sel = grp == g
+        V = vals[sel][:, fl, :]          # (n, m, 10)
+        Pg = P[np.ix_(fl, fl)]
+        phi = V[:, :, 0]
+        Y = phi @ Pg
+        rho = (phi * Y).sum(1)
+        w = xyzw[sel, 3]
+        if iexch == 9:
+            grad = 2 * np.einsum("pia,pi->pa", V[:, :, 1:4], Y)
+            Yd = np.einsum("pka,kj->pja", V[:, :, 1:4], Pg)  # (dPhi_a P)[p][j]
+            h1 = 2 * (np.einsum("pia,pi->pa", V[:, :, 4:7], Y) +
+                      np.einsum("pia,pia->pa", V[:, :, 1:4], Yd))
+            h2 = 2 * np.stack([
+                (V[:, :, 7] * Y).sum(1) + (V[:, :, 1] * Yd[:, :, 1]).sum(1),
+                (V[:, :, 8] * Y).sum(1) + (V[:, :, 1] * Yd[:, :, 2]).sum(1),
+                (V[:, :, 9] * Y).sum(1) + (V[:, :, 2] * Yd[:, :, 2]).sum(1)], 1)
+            ex, ec, v = oracle.gga_points(rho, grad, h1, h2)
+        else:
+            ex, ec, v = oracle.lda_points(rho, iexch)
+        E_bf += (rho * w * (ex + ec)).sum()
+        N_bf += (rho * w).sum()
+        F[np.ix_(fl, fl)] += (phi * (w * v)[:, None]).T @ phi
+    F_packed = np.concatenate([F[i:, i] for i in range(s.M)])
+    assert abs(E - E_bf) < 1e-10 * abs(E)
+    assert abs(N - N_bf) < 1e-10 * N
+    assert np.max(np.abs(Fp - F_packed)) < 1e-10
+    assert abs(N - 2 * s.nocc) < 1e-3  # quadrature of tr(P S)
