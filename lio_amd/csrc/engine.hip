@@ -12,7 +12,7 @@
 //                      FP64 MFMA (16x16x4), 64 points per block; row-dot
 //                      epilogue -> rho, grad rho, Hessian of rho -> PBE/LDA
 //                      -> factor_p = w_p v_p and per-block (Exc, N)
-//   k_fock             F_g tile (64 x 64) over a 128-point slice,
+//   k_fock             F_g tile (64 x 64) over a 512-point slice,
 //                      phi^T diag(factor) phi on FP64 MFMA, one partial/slice
 //   k_fock_group       sums the slices of each tile in fixed order
 //   k_fock_pack        packed F_IJ = sum over groups holding I and J, in
@@ -55,7 +55,7 @@ constexpr int kPtAlign = 16;     // point padding (one MFMA column tile)
 constexpr int kFnAlign = 16;     // function padding (one MFMA row tile)
 constexpr int kDensPts = 64;     // points per density block (16 per wave)
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
-constexpr int kSlice = 128;      // points per Fock slice (split-K)
+constexpr int kSlice = 512;      // points per Fock slice (split-K over blocks)
 constexpr int kEvalPts = 256;    // points per evaluation block
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __rest
 // j = 16 rt + l/16 + 4 i (i = item) of all NT derivatives, and contracts them
 // with the values of those functions at that point (10 row-dot partials).
 template <int NT>
-__global__ __launch_bounds__(256) void k_density(const DensityItem* __restrict__ items,
+__global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
                                                  const double* __restrict__ pg,
                                                  const double* __restrict__ vals,
@@ -342,25 +342,28 @@ __global__ __launch_bounds__(256) void k_density(const DensityItem* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Fock: one wave per (group, tile pair, 128-point slice)
+// Fock: one block (4 waves) per (group, tile pair, 512-point slice)
 // ---------------------------------------------------------------------------
 // out[i][j] = sum_p phi_i(p) factor_p phi_j(p) for i in tile ti, j in tile tj.
+// Wave w takes the points pb + [128 w, 128 w + 128) of the slice; the four
+// partial tiles are added through LDS in wave order (fixed, reproducible).
 // The K order is permuted so that the four K slots a lane feeds to four
 // consecutive MFMAs are four consecutive points: one 32-byte load per lane
 // and operand row instead of four scattered 8-byte loads.
-__global__ __launch_bounds__(64) void k_fock(const FockItem* __restrict__ items,
-                                             const DevGroup* __restrict__ groups,
-                                             const double* __restrict__ vals,
-                                             const double* __restrict__ factor,
-                                             double* __restrict__ fpart) {
+__global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ items,
+                                              const DevGroup* __restrict__ groups,
+                                              const double* __restrict__ vals,
+                                              const double* __restrict__ factor,
+                                              double* __restrict__ fpart) {
+  __shared__ double sF[kFnChunk * kFnChunk];
   const FockItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
-  const int lane = threadIdx.x, lrow = lane & 15, lk = lane >> 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lrow = lane & 15, lk = lane >> 4;
   const int np = g.npts_pad, mp = g.m_pad;
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
-  const int pb = it.slice * kSlice;
-  const int pe = min(pb + kSlice, np);
+  const int pb = it.slice * kSlice + wave * (kSlice / 4);
+  const int pe = min(pb + kSlice / 4, np);
   const double* V = vals + g.val_off;
   const double* fac = factor + g.pt_off;
   d4 acc[4][4];
@@ -389,22 +392,31 @@ __global__ __launch_bounds__(64) void k_fock(const FockItem* __restrict__ items,
         }
       }
   }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int e = (16 * r + lk + 4 * i) * kFnChunk + 16 * c + lrow;
+            sF[e] = (w == 0) ? acc[r][c][i] : sF[e] + acc[r][c][i];
+          }
+    }
+    __syncthreads();
+  }
   double* out = fpart + (long long)blockIdx.x * kFnChunk * kFnChunk;
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        out[(16 * r + lk + 4 * i) * kFnChunk + 16 * c + lrow] = acc[r][c][i];
+  for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) out[e] = sF[e];
 }
 
-// F_g tile (ti, tj) = sum over slices, in slice order.
+// F_g tile (ti, tj) = sum over slices, in slice order.  Block b of a tile
+// pair handles elements [256 (b % 16), +256) of the 64 x 64 tile.
 __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__ groups,
                                                     const int2* __restrict__ tiles,
                                                     const double* __restrict__ fpart,
                                                     double* __restrict__ fg) {
-  const int2 gt = tiles[blockIdx.x];  // x: group, y: tile-pair index
+  const int2 gt = tiles[blockIdx.x >> 4];  // x: group, y: tile-pair index
   const DevGroup g = groups[gt.x];
   int ti = 0, rem = gt.y;
   while (rem >= g.ntile - ti) {
@@ -412,15 +424,15 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
     ++ti;
   }
   const int tj = ti + rem;
-  const int base = g.fock_base + gt.y * g.nslice;
-  for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) {
-    const int r = e / kFnChunk, c = e % kFnChunk;
-    const int i = kFnChunk * ti + r, j = kFnChunk * tj + c;
-    if (i >= g.m_pad || j >= g.m_pad) continue;
-    double s = 0.0;
-    for (int sl = 0; sl < g.nslice; ++sl) s += fpart[(long long)(base + sl) * kFnChunk * kFnChunk + e];
-    fg[g.pg_off + (long long)i * g.m_pad + j] = s;
-  }
+  const int e = ((blockIdx.x & 15) << 8) + threadIdx.x;
+  const int r = e / kFnChunk, c = e % kFnChunk;
+  const int i = kFnChunk * ti + r, j = kFnChunk * tj + c;
+  if (i >= g.m_pad || j >= g.m_pad) return;
+  const double* src = fpart + (long long)(g.fock_base + gt.y * g.nslice) * kFnChunk * kFnChunk + e;
+  double s = 0.0;
+#pragma unroll 8
+  for (int sl = 0; sl < g.nslice; ++sl) s += src[(long long)sl * kFnChunk * kFnChunk];
+  fg[g.pg_off + (long long)i * g.m_pad + j] = s;
 }
 
 // Packed F: entry (I <= J) sums F_g(li, lj) over the groups holding both
@@ -453,18 +465,18 @@ __global__ void k_fock_pack(const int* __restrict__ memb_off, const int2* __rest
 }
 
 // Fixed-order reduction of the per-block (Exc, N) pairs.
-__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ epart, int n,
-                                                double* __restrict__ out) {
-  __shared__ double se[256], sn[256];
+__global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epart, int n,
+                                                 double* __restrict__ out) {
+  __shared__ double se[1024], sn[1024];
   double e = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = threadIdx.x; i < n; i += 1024) {
     e += epart[2 * i];
     c += epart[2 * i + 1];
   }
   se[threadIdx.x] = e;
   sn[threadIdx.x] = c;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = 512; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       se[threadIdx.x] += se[threadIdx.x + s];
       sn[threadIdx.x] += sn[threadIdx.x + s];
@@ -791,17 +803,17 @@ SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_
   XCQ_HIP(hipGetLastError());
   XCQ_HIP(hipEventRecord(ev[2], s));
   if (d_fock && g.nfock > 0) {
-    hipLaunchKernelGGL(k_fock, dim3(g.nfock), dim3(64), 0, s, g.fitems, g.groups, g.vals,
+    hipLaunchKernelGGL(k_fock, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.vals,
                        g.factor, g.fpart);
-    hipLaunchKernelGGL(k_fock_group, dim3(g.ngroup_tiles), dim3(256), 0, s, g.groups, g.gtiles,
-                       g.fpart, g.fg);
+    hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
+                       g.gtiles, g.fpart, g.fg);
   }
   XCQ_HIP(hipGetLastError());
   XCQ_HIP(hipEventRecord(ev[3], s));
   if (d_fock)
     hipLaunchKernelGGL(k_fock_pack, dim3((M + 127) / 128, M), dim3(128), 0, s, g.memb_off, g.memb,
                        g.groups, g.fg, d_fock, M);
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, g.epart, g.ndensity, g.eout);
+  hipLaunchKernelGGL(k_energy, dim3(1), dim3(1024), 0, s, g.epart, g.ndensity, g.eout);
   XCQ_HIP(hipGetLastError());
   XCQ_HIP(hipEventRecord(ev[4], s));
   (void)want_energy;
