@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB read + 256 MiB write elementwise kernel "
+                         "(calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the XC Fock build on a single "
@@ -144,6 +147,17 @@ def main():
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
     }
+    if args.copy_probe:
+        # 1 GiB read + 1 GiB write by one elementwise kernel; a 2 GiB write in
+        # between evicts the source from the 256 MB last-level cache first
+        a = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device=dev)
+        flush = torch.ones(256 * 1024 * 1024, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        result["copy_probe_bytes"] = {"read": a.numel() * 8, "write": b.numel() * 8}
+        del a, b, flush
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(s)
