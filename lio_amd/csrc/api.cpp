@@ -32,6 +32,7 @@ struct xcq_context {
   long long npts_pad = 0;
   double flops = 0.0;
   int density_tiles = 0, fock_items = 0;
+  xcq::Precision prec = xcq::Precision::FP64;
 };
 
 namespace {
@@ -119,7 +120,7 @@ void build(xcq_context* ctx, unsigned grid_type) {
   ctx->mol = xcq::capture(ctx->pv, grid_type);
   ctx->part = xcq::build_partition(ctx->mol, ctx->opt);
   const double t0 = wall();
-  ctx->engine->load(ctx->mol, ctx->part);
+  ctx->engine->load(ctx->mol, ctx->part, ctx->prec);
   ctx->upload_seconds = wall() - t0;
   ctx->grid_type = (int)grid_type;
 }
@@ -152,6 +153,19 @@ int xcq_set_problem(xcq_context* ctx, const xcq_problem* p) {
     view_from_problem(p, ctx->pv, ctx->opt);
     ctx->have_problem = true;
     ctx->grid_type = -1;
+  });
+}
+
+int xcq_set_precision(xcq_context* ctx, int precision) {
+  return guarded([&] {
+    if (precision != XCQ_FP64 && precision != XCQ_FP32)
+      throw std::runtime_error("g2g: unknown precision mode");
+    ctx->prec = static_cast<xcq::Precision>(precision);
+    if (ctx->grid_type >= 0) {
+      const double t0 = wall();
+      ctx->engine->load(ctx->mol, ctx->part, ctx->prec);
+      ctx->upload_seconds = wall() - t0;
+    }
   });
 }
 
@@ -264,6 +278,17 @@ void abi(const char* where, F&& f) {
   }
 }
 
+// LIO_XC_PRECISION = fp64 (default) | fp32 selects the contraction
+// arithmetic of the Fortran-facing singleton.
+xcq::Precision precision_from_env() {
+  const char* v = std::getenv("LIO_XC_PRECISION");
+  if (!v || !*v) return xcq::Precision::FP64;
+  const std::string s(v);
+  if (s == "fp64") return xcq::Precision::FP64;
+  if (s == "fp32") return xcq::Precision::FP32;
+  throw std::runtime_error("LIO_XC_PRECISION must be fp64 or fp32, got '" + s + "'");
+}
+
 xcq_context* ctx_or_die(const char* where) {
   if (!g_ctx) die(where, "g2g_init_ was not called");
   return g_ctx;
@@ -279,6 +304,7 @@ void g2g_init_(void) {
       g_ctx = new xcq_context;
       g_ctx->engine.reset(new xcq::Engine(-1));
     }
+    g_ctx->prec = precision_from_env();
     g_ctx->opt = g_opt;
     if (g_opt.verbose > 2) std::printf("  Using HIP device %d for the XC quadrature.\n", g_ctx->engine->device());
   });

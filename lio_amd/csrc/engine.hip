@@ -144,11 +144,33 @@ __device__ inline void eval_basis(const DevFn& b, double px, double py, double p
   o[9] = Pyz * t + Py * gz + Pz * gy + P * (4.0 * vy * vz * th);
 }
 
-// FP64 16x16x4 MFMA.  Operands: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the result D[r][c] sits in lane (r%4)*16 + c, item r/4.
-__device__ inline void mfma64(double a, double b, d4& c) {
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+// 16x16x4 MFMA in FP64 or FP32.  Operands: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  Results (measured on the MI355X): the FP64 form leaves
+// D[r][c] in lane (r%4)*16 + c, item r/4 (lane l, item i holds row
+// l/16 + 4 i); the FP32 form leaves it in lane (r/4)*16 + c, item r%4 (row
+// 4 (l/16) + i).
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+struct Mma;
+
+template <>
+struct Mma<double> {
+  typedef d4 acc;
+  __device__ static inline void op(double a, double b, d4& c) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static constexpr int row(int lk, int i) { return lk + 4 * i; }
+};
+
+template <>
+struct Mma<float> {
+  typedef f4 acc;
+  __device__ static inline void op(float a, float b, f4& c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static constexpr int row(int lk, int i) { return 4 * lk + i; }
+};
 
 // Packed index of (i, j), i <= j, column-major lower triangle (partition.cpp:132-134).
 __device__ __host__ inline long long packed_index(long long i, long long j, long long M) {
@@ -158,7 +180,7 @@ __device__ __host__ inline long long packed_index(long long i, long long j, long
 // ---------------------------------------------------------------------------
 // basis evaluation: one function x 256 points per block
 // ---------------------------------------------------------------------------
-template <int NV>
+template <int NV, class T>
 __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__ items,
                                                     const DevGroup* __restrict__ groups,
                                                     const int* __restrict__ fnl,
@@ -166,7 +188,7 @@ __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__
                                                     const double* __restrict__ px,
                                                     const double* __restrict__ py,
                                                     const double* __restrict__ pz,
-                                                    double* __restrict__ vals) {
+                                                    T* __restrict__ vals) {
   const EvalItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int p = it.p0 + threadIdx.x;
@@ -179,19 +201,20 @@ __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__
   } else {
     for (int k = 0; k < NV; ++k) v[k] = 0.0;
   }
-  double* out = vals + g.val_off + (long long)it.k * g.npts_pad + p;
+  T* out = vals + g.val_off + (long long)it.k * g.npts_pad + p;
   const long long plane = (long long)g.m_pad * g.npts_pad;
-  for (int k = 0; k < NV; ++k) out[k * plane] = v[k];
+  for (int k = 0; k < NV; ++k) out[k * plane] = (T)v[k];
 }
 
 // ---------------------------------------------------------------------------
 // density blocks P_g[k][j] = P(f_k, f_j) with the true symmetric P
 // ---------------------------------------------------------------------------
+template <class T>
 __global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __restrict__ items,
                                                         const DevGroup* __restrict__ groups,
                                                         const int* __restrict__ fnl,
                                                         const double* __restrict__ rho, int M,
-                                                        double* __restrict__ pg) {
+                                                        T* __restrict__ pg) {
   const GatherItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int e = it.start + threadIdx.x;
@@ -204,7 +227,7 @@ __global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __rest
     v = rho[packed_index(lo, hi, M)];
     if (lo != hi) v *= 0.5;
   }
-  pg[g.pg_off + e] = v;
+  pg[g.pg_off + e] = (T)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -216,11 +239,11 @@ __global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __rest
 // lane l then holds, for point p0 + l%16, the four functions
 // j = 16 rt + l/16 + 4 i (i = item) of all NT derivatives, and contracts them
 // with the values of those functions at that point (10 row-dot partials).
-template <int NT>
+template <int NT, class T>
 __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
-                                                 const double* __restrict__ pg,
-                                                 const double* __restrict__ vals,
+                                                 const T* __restrict__ pg,
+                                                 const T* __restrict__ vals,
                                                  const double* __restrict__ pw,
                                                  double* __restrict__ factor,
                                                  double* __restrict__ epart, int iexch) {
@@ -235,39 +258,39 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
   if (p0 < g.npts_pad) {
     const int np = g.npts_pad, mp = g.m_pad;
     const long long plane = (long long)mp * np;
-    const double* V = vals + g.val_off;
-    const double* Pm = pg + g.pg_off;
+    const T* V = vals + g.val_off;
+    const T* Pm = pg + g.pg_off;
     double s[NS];
     for (int k = 0; k < NS; ++k) s[k] = 0.0;
     for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
       const int nrt = min(4, (mp - c0) / 16);
-      d4 acc[4][NT];
+      typename Mma<T>::acc acc[4][NT];
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[rt][t] = d4{0.0, 0.0, 0.0, 0.0};
-      const double* Bk = V + (long long)lk * np + p0 + lrow;   // Phi_t[k][p]
-      const double* Ak = Pm + (long long)lk * mp + c0 + lrow;  // P[k][j] = P[j][k]
+        for (int t = 0; t < NT; ++t) acc[rt][t] = typename Mma<T>::acc{0, 0, 0, 0};
+      const T* Bk = V + (long long)lk * np + p0 + lrow;   // Phi_t[k][p]
+      const T* Ak = Pm + (long long)lk * mp + c0 + lrow;  // P[k][j] = P[j][k]
       // operands of step k0 in (a, b); the next step's are loaded before the
       // MFMAs of this one so the loads overlap the matrix pipe
-      double a[4], b[NT];
+      T a[4], b[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) b[t] = Bk[t * plane];
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt) a[rt] = (rt < nrt) ? Ak[16 * rt] : 0.0;
+      for (int rt = 0; rt < 4; ++rt) a[rt] = (rt < nrt) ? Ak[16 * rt] : T(0);
       for (int k0 = 0; k0 < mp; k0 += 4) {
-        double an[4], bn[NT];
+        T an[4], bn[NT];
         const bool more = k0 + 4 < mp;
         const long long kb = (long long)(more ? k0 + 4 : k0);
 #pragma unroll
         for (int t = 0; t < NT; ++t) bn[t] = Bk[t * plane + kb * np];
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) an[rt] = (rt < nrt) ? Ak[kb * mp + 16 * rt] : 0.0;
+        for (int rt = 0; rt < 4; ++rt) an[rt] = (rt < nrt) ? Ak[kb * mp + 16 * rt] : T(0);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
           if (rt < nrt) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t) mfma64(a[rt], b[t], acc[rt][t]);
+            for (int t = 0; t < NT; ++t) Mma<T>::op(a[rt], b[t], acc[rt][t]);
           }
         }
 #pragma unroll
@@ -280,12 +303,12 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
         if (rt >= nrt) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int j = c0 + 16 * rt + lk + 4 * i;
-          const double* vj = V + (long long)j * np + p0 + lrow;
+          const int j = c0 + 16 * rt + Mma<T>::row(lk, i);
+          const T* vj = V + (long long)j * np + p0 + lrow;
           if constexpr (NT == 4) {
             double v[10];
 #pragma unroll
-            for (int q = 0; q < 10; ++q) v[q] = vj[q * plane];
+            for (int q = 0; q < 10; ++q) v[q] = (double)vj[q * plane];
             const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
                          yz = acc[rt][3][i];
             s[0] += v[0] * y0;
@@ -299,7 +322,7 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
             s[8] += v[8] * y0 + v[1] * yz;
             s[9] += v[9] * y0 + v[2] * yz;
           } else {
-            s[0] += vj[0] * acc[rt][0][i];
+            s[0] += (double)vj[0] * (double)acc[rt][0][i];
           }
         }
       }
@@ -350,11 +373,17 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
 // The K order is permuted so that the four K slots a lane feeds to four
 // consecutive MFMAs are four consecutive points: one 32-byte load per lane
 // and operand row instead of four scattered 8-byte loads.
+template <class T>
 __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ items,
-                                              const DevGroup* __restrict__ groups,
-                                              const double* __restrict__ vals,
-                                              const double* __restrict__ factor,
-                                              double* __restrict__ fpart) {
+                                                 const DevGroup* __restrict__ groups,
+                                                 const T* __restrict__ vals,
+                                                 const double* __restrict__ factor,
+                                                 double* __restrict__ fpart) {
+  typedef typename Mma<T>::acc A4;
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  // FP32 products are summed over at most 32 points in the matrix unit and
+  // then added into FP64; FP64 accumulates in place.
+  constexpr bool kFlush = sizeof(T) == 4;
   __shared__ double sF[kFnChunk * kFnChunk];
   const FockItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
@@ -364,22 +393,28 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
   const int pb = it.slice * kSlice + wave * (kSlice / 4);
   const int pe = min(pb + kSlice / 4, np);
-  const double* V = vals + g.val_off;
+  const T* V = vals + g.val_off;
   const double* fac = factor + g.pt_off;
-  d4 acc[4][4];
+  A4 acc[4][4];
+  d4 dacc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int p = pb + 4 * lk; p < pe; p += 16) {
-    const d4 f = *reinterpret_cast<const d4*>(fac + p);
-    d4 A[4], B[4];
+    for (int b = 0; b < 4; ++b) {
+      acc[a][b] = A4{0, 0, 0, 0};
+      dacc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  int step = 0;
+  for (int p = pb + 4 * lk; p < pe; p += 16, ++step) {
+    const d4 fd = *reinterpret_cast<const d4*>(fac + p);
+    const V4 f = V4{(T)fd[0], (T)fd[1], (T)fd[2], (T)fd[3]};
+    V4 A[4], B[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      A[r] = (r < nri) ? *reinterpret_cast<const d4*>(V + (long long)(i0 + 16 * r + lrow) * np + p)
-                       : d4{0.0, 0.0, 0.0, 0.0};
-      B[r] = (r < nrj) ? *reinterpret_cast<const d4*>(V + (long long)(j0 + 16 * r + lrow) * np + p) * f
-                       : d4{0.0, 0.0, 0.0, 0.0};
+      A[r] = (r < nri) ? *reinterpret_cast<const V4*>(V + (long long)(i0 + 16 * r + lrow) * np + p)
+                       : V4{0, 0, 0, 0};
+      B[r] = (r < nrj) ? *reinterpret_cast<const V4*>(V + (long long)(j0 + 16 * r + lrow) * np + p) * f
+                       : V4{0, 0, 0, 0};
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s)
@@ -388,9 +423,22 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
         if (r < nri) {
 #pragma unroll
           for (int c = 0; c < 4; ++c)
-            if (c < nrj) mfma64(A[r][s], B[c][s], acc[r][c]);
+            if (c < nrj) Mma<T>::op(A[r][s], B[c][s], acc[r][c]);
         }
       }
+    if constexpr (kFlush) {
+      if ((step & 1) == 1 || p + 16 >= pe) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              dacc[r][c][i] += (double)acc[r][c][i];
+              acc[r][c][i] = 0;
+            }
+      }
+    }
   }
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
@@ -400,8 +448,9 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
         for (int c = 0; c < 4; ++c)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int e = (16 * r + lk + 4 * i) * kFnChunk + 16 * c + lrow;
-            sF[e] = (w == 0) ? acc[r][c][i] : sF[e] + acc[r][c][i];
+            const int e = (16 * r + Mma<T>::row(lk, i)) * kFnChunk + 16 * c + lrow;
+            const double x = kFlush ? dacc[r][c][i] : (double)acc[r][c][i];
+            sF[e] = (w == 0) ? x : sF[e] + x;
           }
     }
     __syncthreads();
@@ -518,8 +567,9 @@ struct DeviceGrid {
   double *px = nullptr, *py = nullptr, *pz = nullptr, *pw = nullptr;
   double* rho = nullptr;
   double* F = nullptr;
-  double* vals = nullptr;
-  double* pg = nullptr;  // P_g blocks (per solve)
+  void* vals = nullptr;   // value planes, FP64 or FP32 by precision
+  void* pg = nullptr;     // P_g blocks (per solve), same type
+  Precision prec = Precision::FP64;
   double* fg = nullptr;  // F_g blocks (per solve)
   double* factor = nullptr;
   double* epart = nullptr;
@@ -586,7 +636,7 @@ Engine::~Engine() {
 size_t Engine::arena_bytes() const { return grid_ ? grid_->arena_bytes : 0; }
 double Engine::eval_ms() const { return grid_ ? grid_->eval_ms : 0.0; }
 
-void Engine::load(const Molecule& mol, const GridPartition& part) {
+void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec) {
   XCQ_HIP(hipSetDevice(device_));
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
   release(grid_);
@@ -597,6 +647,8 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   dg->iexch = mol.iexch;
   dg->gga = mol.gga;
   dg->nval = mol.gga ? 10 : 1;
+  dg->prec = prec;
+  const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const int G = part.ngroups, M = mol.M;
   dg->ngroups = G;
 
@@ -713,8 +765,8 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
     dg->pw = c.take<double>(pt);
     dg->rho = c.take<double>(packed);
     dg->F = c.take<double>(packed);
-    dg->vals = c.take<double>(vo);
-    dg->pg = c.take<double>(po);
+    dg->vals = c.take<char>(vo * vbytes);
+    dg->pg = c.take<char>(po * vbytes);
     dg->fg = c.take<double>(po);
     dg->factor = c.take<double>(pt);
     dg->epart = c.take<double>(2 * ditems.size());
@@ -764,12 +816,24 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
   XCQ_HIP(hipEventRecord(ev[6], s));
   if (dg->neval > 0) {
-    if (dg->gga)
-      hipLaunchKernelGGL(k_eval_basis<10>, dim3(dg->neval), dim3(kEvalPts), 0, s, dg->eitems,
-                         dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->vals);
-    else
-      hipLaunchKernelGGL(k_eval_basis<1>, dim3(dg->neval), dim3(kEvalPts), 0, s, dg->eitems,
-                         dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->vals);
+    const dim3 grid(dg->neval), block(kEvalPts);
+    if (prec == Precision::FP64) {
+      double* v = static_cast<double*>(dg->vals);
+      if (dg->gga)
+        hipLaunchKernelGGL((k_eval_basis<10, double>), grid, block, 0, s, dg->eitems, dg->groups,
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+      else
+        hipLaunchKernelGGL((k_eval_basis<1, double>), grid, block, 0, s, dg->eitems, dg->groups,
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+    } else {
+      float* v = static_cast<float*>(dg->vals);
+      if (dg->gga)
+        hipLaunchKernelGGL((k_eval_basis<10, float>), grid, block, 0, s, dg->eitems, dg->groups,
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+      else
+        hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+    }
     XCQ_HIP(hipGetLastError());
   }
   XCQ_HIP(hipEventRecord(ev[7], s));
@@ -780,6 +844,34 @@ void Engine::load(const Molecule& mol, const GridPartition& part) {
   grid_ = dg;
 }
 
+template <class T>
+static void launch_solve(DeviceGrid& g, hipStream_t s, const double* d_rho, bool fock,
+                         hipEvent_t* ev) {
+  T* pg = static_cast<T*>(g.pg);
+  const T* vals = static_cast<const T*>(g.vals);
+  if (g.ngather > 0)
+    hipLaunchKernelGGL(k_gather_density<T>, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
+                       g.fnl, d_rho, g.M, pg);
+  XCQ_HIP(hipEventRecord(ev[1], s));
+  if (g.ndensity > 0) {
+    if (g.gga)
+      hipLaunchKernelGGL((k_density<4, T>), dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
+                         pg, vals, g.pw, g.factor, g.epart, g.iexch);
+    else
+      hipLaunchKernelGGL((k_density<1, T>), dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
+                         pg, vals, g.pw, g.factor, g.epart, g.iexch);
+  }
+  XCQ_HIP(hipGetLastError());
+  XCQ_HIP(hipEventRecord(ev[2], s));
+  if (fock && g.nfock > 0) {
+    hipLaunchKernelGGL(k_fock<T>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, vals,
+                       g.factor, g.fpart);
+    hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
+                       g.gtiles, g.fpart, g.fg);
+  }
+  XCQ_HIP(hipGetLastError());
+}
+
 SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
   DeviceGrid& g = *grid_;
@@ -788,27 +880,10 @@ SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_
   SolveResult res;
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
   XCQ_HIP(hipEventRecord(ev[0], s));
-  if (g.ngather > 0)
-    hipLaunchKernelGGL(k_gather_density, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
-                       g.fnl, d_rho, M, g.pg);
-  XCQ_HIP(hipEventRecord(ev[1], s));
-  if (g.ndensity > 0) {
-    if (g.gga)
-      hipLaunchKernelGGL(k_density<4>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups, g.pg,
-                         g.vals, g.pw, g.factor, g.epart, g.iexch);
-    else
-      hipLaunchKernelGGL(k_density<1>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups, g.pg,
-                         g.vals, g.pw, g.factor, g.epart, g.iexch);
-  }
-  XCQ_HIP(hipGetLastError());
-  XCQ_HIP(hipEventRecord(ev[2], s));
-  if (d_fock && g.nfock > 0) {
-    hipLaunchKernelGGL(k_fock, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.vals,
-                       g.factor, g.fpart);
-    hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
-                       g.gtiles, g.fpart, g.fg);
-  }
-  XCQ_HIP(hipGetLastError());
+  if (g.prec == Precision::FP64)
+    launch_solve<double>(g, s, d_rho, d_fock != nullptr, ev);
+  else
+    launch_solve<float>(g, s, d_rho, d_fock != nullptr, ev);
   XCQ_HIP(hipEventRecord(ev[3], s));
   if (d_fock)
     hipLaunchKernelGGL(k_fock_pack, dim3((M + 127) / 128, M), dim3(128), 0, s, g.memb_off, g.memb,

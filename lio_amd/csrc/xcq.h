@@ -106,7 +106,13 @@ GridPartition build_partition(const Molecule& mol, const Options& opt);
 // ---------------------------------------------------------------------------
 // Device engine
 // ---------------------------------------------------------------------------
-enum class Precision : int { FP64 = 0 };
+// Arithmetic of the two contractions (density and Fock GEMMs).  The
+// functional, the row-dot epilogue, the Exc/N reductions and the Fock
+// accumulation across slices and groups are FP64 in every mode.
+//   FP64: FP64 values, FP64 MFMA.
+//   FP32: FP32 values (half the bytes), FP32 MFMA, Fock products summed in
+//         the matrix unit over at most 32 points before FP64 accumulation.
+enum class Precision : int { FP64 = 0, FP32 = 1 };
 
 struct SolveResult {
   double energy = 0.0;
@@ -128,7 +134,7 @@ class Engine {
   Engine& operator=(const Engine&) = delete;
 
   // Uploads basis, grid and work lists; replaces any previous grid.
-  void load(const Molecule& mol, const GridPartition& part);
+  void load(const Molecule& mol, const GridPartition& part, Precision prec = Precision::FP64);
   // One XC Fock build from packed rho (host memory).  fock_accum (host) gets
   // += F_xc; pass nullptr to skip the copy-back.
   SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy);

@@ -75,6 +75,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_destroy.argtypes = [vp]
     L.xcq_set_problem.argtypes = [vp, C.POINTER(XcqProblem)]
     L.xcq_build_grid.argtypes = [vp, C.c_uint]
+    L.xcq_set_precision.argtypes = [vp, C.c_int]
     L.xcq_query_grid.argtypes = [vp, C.POINTER(XcqGridInfo)]
     L.xcq_groups.argtypes = [vp, C.POINTER(XcqGroupInfo), C.c_int, C.POINTER(C.c_int)]
     L.xcq_solve.argtypes = [vp, _dp, _dp, C.POINTER(XcqSolveInfo)]
@@ -97,6 +98,8 @@ def _check(rc: int, what: str):
 def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
+
+PRECISION = {"fp64": 0, "fp32": 1}
 
 DEFAULT_OPTIONS = dict(little_cube_size=8.0, sphere_radius=0.6, assign_all_functions=0,
                        remove_zero_weights=1, min_points_per_cube=1, max_function_exponent=10)
@@ -154,14 +157,20 @@ def host_partition(system, grid_type=None, **options):
 class Context:
     """One molecule on the device through the handle API."""
 
-    def __init__(self, system=None, device: int = -1, **options):
+    def __init__(self, system=None, device: int = -1, precision: str = "fp64", **options):
         self.L = load_library()
         h = C.c_void_p()
         _check(self.L.xcq_create(device, C.byref(h)), "xcq_create")
         self.h = h
         self.M = 0
+        self.set_precision(precision)
         if system is not None:
             self.set_system(system, **options)
+
+    def set_precision(self, precision: str):
+        """'fp64' or 'fp32' arithmetic for the two contractions."""
+        _check(self.L.xcq_set_precision(self.h, PRECISION[precision]), "xcq_set_precision")
+        self.precision = precision
 
     def set_system(self, system, **options):
         self._p, self._keep = make_problem(system, **options)
