@@ -1,16 +1,25 @@
-"""XC Fock build benchmark on one MI355X.
+"""XC Fock build benchmark on one MI355X: mixed precision vs FP64.
 
 Workload: the 32-water cluster ((H2O)32, 96 atoms), 6-31G** Cartesian basis
-(M = 800), 40 radial x 194 Lebedev points per atom, PBE, FP64, seeded
+(M = 800), 40 radial x 194 Lebedev points per atom, PBE, seeded
 Loewdin-orthonormal density with 160 occupied orbitals -- LIO's XC Fock build
 behind g2g_solve_groups_.  One step = one full XC Fock build (density,
 functional, Fock matrix, packed reduction, energy) with the density already
 resident in HBM and the packed Fock left in HBM.
 
+The same build runs twice: with FP64 contractions (FP64 values, FP64 MFMA)
+and with split-BF16 contractions (FP32 values split exactly into three BF16
+pieces, six BF16 MFMA products per tile, FP32 accumulation; functional,
+Exc/N reductions and Fock accumulation in FP64).  The mixed result is checked
+against the FP64 one in the same run (|dExc|, max |dF_ij|, dN).
+
 Prints ONE JSON line:
-  value          Fock builds/s (whole job, one GPU)
-  roofline       the density kernel (dominant) against FP64 MFMA peak, from
-                 HIP events recorded on the engine's stream
+  value          mixed-precision Fock builds/s (whole job, one GPU)
+  fp64           the FP64 build: builds/s, phases, grid points/s
+  mixed          speedup over FP64 and the errors against it
+  latency_ms     p50/p99 of single synchronised mixed builds
+  roofline       the dominant mixed kernel (k_density_bf<4>) from HIP events
+                 recorded on the engine's stream
   cpu_baseline   LIO's own CPU g2g path (oracle/_ref, when built) or this
                  repository's FP64 CPU restatement, timed on this host
 """
@@ -25,8 +34,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix
+PEAK_FP64_TFLOPS = 78.6    # MI355X dense FP64 matrix
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 PEAK_HBM_GBS = 8000.0
+MIXED = "bf16x6"
+BF16_PRODUCTS = 6          # BF16 MFMA products per split-BF16 tile product
 
 
 def log(*a):
@@ -57,14 +69,71 @@ def cpu_baseline(system, max_seconds: float = 25.0):
                       "(median %.3f s per build, OMP_NUM_THREADS=%d)" % (len(times), t_grid, med, threads)}
 
 
+def run_mode(s, precision, steps, warmup, latency_samples):
+    """Builds the grid in one precision mode and times it; returns a dict."""
+    import numpy as np
+    import torch
+    from lio_amd import g2g
+    dev = torch.device("cuda:0")
+    ctx = g2g.Context(s, precision=precision)
+    info = ctx.build_grid()
+    log("%s grid: %d groups, %d points, %.3f algorithmic TFLOP/build, host partition %.2f s, "
+        "upload+eval %.2f s, eval kernel %.3f ms" % (precision, info["ngroups"], info["npoints"],
+                                                      info["flops"] / 1e12, info["grid_seconds"],
+                                                      info["upload_seconds"], info["eval_ms"]))
+    rho = torch.from_numpy(s.rho_packed.copy()).to(dev)
+    F = torch.zeros_like(rho)
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream(), device=dev)
+    for _ in range(warmup):
+        ctx.solve_device(rho.data_ptr(), F.data_ptr())
+    ctx.sync_energy()
+
+    # throughput: K back-to-back builds on the engine stream
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    t_wall = time.perf_counter()
+    for _ in range(steps):
+        ctx.solve_device(rho.data_ptr(), F.data_ptr())
+    end.record(stream)
+    energy, electrons = ctx.sync_energy()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms_step = max(start.elapsed_time(end), 1e3 * t_wall) / steps
+    fock = F.cpu().numpy().copy()
+
+    # per-phase durations from the engine's HIP events (same stream)
+    phases = np.zeros(4)
+    for _ in range(steps):
+        ctx.solve_device(rho.data_ptr(), F.data_ptr())
+        phases += ctx.kernel_times()
+    phases /= steps
+
+    # latency of single builds, each waited for (queue -> energy on the host)
+    lat = []
+    for _ in range(latency_samples):
+        t0 = time.perf_counter()
+        ctx.solve_device(rho.data_ptr(), F.data_ptr())
+        ctx.sync_energy()
+        lat.append(1e3 * (time.perf_counter() - t0))
+    ctx.close()
+    lat = np.sort(np.array(lat))
+    log("%s: %.3f ms/build, phases ms: expand %.4f density %.4f fock %.4f pack %.4f"
+        % ((precision, ms_step) + tuple(phases)))
+    return dict(info=info, ms=ms_step, energy=energy, electrons=electrons, fock=fock,
+                phases=phases, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--latency-samples", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="also run a plain 256 MiB read + 256 MiB write elementwise kernel "
+                    help="also run a plain 1 GiB read + 1 GiB write elementwise kernel "
                          "(calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -75,7 +144,7 @@ def main():
 
     import numpy as np
     import torch
-    from lio_amd import g2g, systems
+    from lio_amd import systems
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
@@ -83,69 +152,60 @@ def main():
     dev = torch.device("cuda:0")
     log("building (H2O)32 system ...")
     s = systems.config1()
-    ctx = g2g.Context(s)
-    t0 = time.time()
-    info = ctx.build_grid()
-    log("grid: %d groups, %d points, %.3f TFLOP/build, host partition %.2f s, upload+eval %.2f s, eval kernel %.3f ms"
-        % (info["ngroups"], info["npoints"], info["flops"] / 1e12, info["grid_seconds"],
-           info["upload_seconds"], info["eval_ms"]))
-    rho = torch.from_numpy(s.rho_packed.copy()).to(dev)
-    F = torch.zeros_like(rho)
-    torch.cuda.synchronize()
-    stream = torch.cuda.ExternalStream(ctx.stream(), device=dev)
+    r64 = run_mode(s, "fp64", args.steps, args.warmup, args.latency_samples)
+    rmx = run_mode(s, MIXED, args.steps, args.warmup, args.latency_samples)
+    info = rmx["info"]
+    npts = info["npoints"]
+    dE = rmx["energy"] - r64["energy"]
+    dF = float(np.max(np.abs(rmx["fock"] - r64["fock"])))
+    dN = rmx["electrons"] - r64["electrons"]
 
-    for _ in range(args.warmup):
-        ctx.solve_device(rho.data_ptr(), F.data_ptr())
-    ctx.sync_energy()
-
-    # throughput: K back-to-back builds on the engine stream
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    t_wall = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.solve_device(rho.data_ptr(), F.data_ptr())
-    end.record(stream)
-    energy, electrons = ctx.sync_energy()
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    ms_total = start.elapsed_time(end)
-    ms_step = max(ms_total, 1e3 * t_wall) / args.steps
-
-    # per-phase durations from the engine's HIP events (same stream)
-    phases = np.zeros(4)
-    for _ in range(args.steps):
-        ctx.solve_device(rho.data_ptr(), F.data_ptr())
-        phases += ctx.kernel_times()
-    phases /= args.steps
-    log("phases ms: expand %.4f density %.4f fock %.4f pack %.4f" % tuple(phases))
+    # dominant kernel of the mixed build: the density/functional launch
     density_flops = 0.8 * info["flops"]  # 8 n m^2 of the 10 n m^2 per group
-    achieved = density_flops / (phases[1] * 1e-3) / 1e12
+    t_dens = rmx["phases"][1] * 1e-3
+    achieved = density_flops / t_dens / 1e12
+    peak_alg = PEAK_BF16_TFLOPS / BF16_PRODUCTS
+    fp64_density = 0.8 * r64["info"]["flops"] / (r64["phases"][1] * 1e-3) / 1e12
 
     result = {
-        "metric": "XC Fock builds/s (FP64 GPU, drop-in g2g_solve_groups_)",
-        "value": 1000.0 / ms_step,
+        "metric": "XC Fock builds/s, mixed (split-BF16) vs FP64 GPU at |dExc|<=1e-6 Ha "
+                  "(drop-in g2g_solve_groups_)",
+        "value": 1000.0 / rmx["ms"],
         "unit": "Fock builds/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": rmx["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x6 MFMA (fp32 accumulate) + f64 functional/reductions",
         "data": "synthetic",
-        "config": {"workload": "(H2O)32 water cluster, 6-31G** (M=800), 40x194 grid, PBE, FP64",
-                   "natoms": int(s.natom), "M": int(s.M), "grid_points": int(info["npoints"]),
+        "config": {"workload": "(H2O)32 water cluster, 6-31G** (M=800), 40x194 grid, PBE, "
+                               "split-BF16 contractions with FP64 functional and accumulation",
+                   "natoms": int(s.natom), "M": int(s.M), "grid_points": int(npts),
                    "groups": int(info["ngroups"]), "parallelism": "single GPU"},
-        "grid_points_per_s": info["npoints"] / (ms_step * 1e-3),
-        "energy": energy,
-        "electrons": electrons,
-        "phases_ms": {"expand": phases[0], "density": phases[1], "fock": phases[2],
-                      "pack": phases[3]},
-        "roofline": {"kernel": "k_density_gga", "bound": "mfma", "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None},
+        "grid_points_per_s": npts / (rmx["ms"] * 1e-3),
+        "latency_ms": {"p50": rmx["p50"], "p99": rmx["p99"],
+                       "samples": args.latency_samples, "what": "single mixed build, queue to "
+                       "energy on the host"},
+        "fp64": {"value": 1000.0 / r64["ms"], "ms_per_step": r64["ms"],
+                 "grid_points_per_s": npts / (r64["ms"] * 1e-3),
+                 "latency_ms": {"p50": r64["p50"], "p99": r64["p99"]},
+                 "phases_ms": dict(zip(["expand", "density", "fock", "pack"], r64["phases"].tolist())),
+                 "density_tflops": fp64_density, "density_frac_of_fp64_peak": fp64_density / PEAK_FP64_TFLOPS},
+        "mixed": {"precision": MIXED, "speedup_vs_fp64": r64["ms"] / rmx["ms"],
+                  "dExc_vs_fp64": dE, "max_dF_vs_fp64": dF, "dN_vs_fp64": dN,
+                  "within_1e-6": bool(abs(dE) <= 1e-6 and dF <= 1e-6 and abs(dN) <= 1e-6),
+                  "phases_ms": dict(zip(["expand", "density", "fock", "pack"], rmx["phases"].tolist()))},
+        "energy": rmx["energy"],
+        "electrons": rmx["electrons"],
+        "energy_fp64": r64["energy"],
+        "roofline": {"kernel": "k_density_bf<4>", "bound": "mfma", "achieved": achieved,
+                     "peak": peak_alg, "unit": "TFLOP/s", "frac": achieved / peak_alg,
+                     "traffic": None,
+                     "peak_note": "algorithmic FLOP (8 n m^2 per group) against the BF16 dense "
+                                  "peak / 6 products per split-BF16 product"},
     }
     if args.copy_probe:
         # 1 GiB read + 1 GiB write by one elementwise kernel; a 2 GiB write in
@@ -163,7 +223,6 @@ def main():
         result["cpu_baseline"] = cpu_baseline(s)
     else:
         result["cpu_baseline"] = None
-    ctx.close()
     print(json.dumps(result))
 
 

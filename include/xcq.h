@@ -85,11 +85,14 @@ int xcq_set_problem(xcq_context* ctx, const xcq_problem* problem);
 /* Arithmetic of the density and Fock contractions (the functional, the
  * energy/electron reductions and the Fock accumulation stay FP64):
  *   XCQ_FP64 (default) FP64 values and FP64 MFMA;
- *   XCQ_FP32           FP32 values and FP32 MFMA.
+ *   XCQ_FP32           FP32 values and FP32 MFMA;
+ *   XCQ_BF16X6         FP32 values, each split exactly into three BF16
+ *                      pieces; six BF16 MFMA products per tile (the pieces
+ *                      of combined weight above 2^-24), FP32 accumulation.
  * Takes effect at the next xcq_build_grid, or at once (re-uploading and
  * re-evaluating the basis) when a grid is already built.  No reference
  * counterpart: LIO's GPU path picks precision at compile time. */
-enum { XCQ_FP64 = 0, XCQ_FP32 = 1 };
+enum { XCQ_FP64 = 0, XCQ_FP32 = 1, XCQ_BF16X6 = 2 };
 int xcq_set_precision(xcq_context* ctx, int precision);
 
 /* Builds grid and partition for grid_type (0/1/2) and uploads them. */

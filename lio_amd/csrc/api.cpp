@@ -158,7 +158,7 @@ int xcq_set_problem(xcq_context* ctx, const xcq_problem* p) {
 
 int xcq_set_precision(xcq_context* ctx, int precision) {
   return guarded([&] {
-    if (precision != XCQ_FP64 && precision != XCQ_FP32)
+    if (precision != XCQ_FP64 && precision != XCQ_FP32 && precision != XCQ_BF16X6)
       throw std::runtime_error("g2g: unknown precision mode");
     ctx->prec = static_cast<xcq::Precision>(precision);
     if (ctx->grid_type >= 0) {
@@ -278,7 +278,7 @@ void abi(const char* where, F&& f) {
   }
 }
 
-// LIO_XC_PRECISION = fp64 (default) | fp32 selects the contraction
+// LIO_XC_PRECISION = fp64 (default) | fp32 | bf16x6 selects the contraction
 // arithmetic of the Fortran-facing singleton.
 xcq::Precision precision_from_env() {
   const char* v = std::getenv("LIO_XC_PRECISION");
@@ -286,7 +286,8 @@ xcq::Precision precision_from_env() {
   const std::string s(v);
   if (s == "fp64") return xcq::Precision::FP64;
   if (s == "fp32") return xcq::Precision::FP32;
-  throw std::runtime_error("LIO_XC_PRECISION must be fp64 or fp32, got '" + s + "'");
+  if (s == "bf16x6") return xcq::Precision::BF16X6;
+  throw std::runtime_error("LIO_XC_PRECISION must be fp64, fp32 or bf16x6, got '" + s + "'");
 }
 
 xcq_context* ctx_or_die(const char* where) {

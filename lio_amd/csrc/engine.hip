@@ -53,7 +53,7 @@ namespace {
 
 constexpr int kPtAlign = 16;     // point padding (one MFMA column tile)
 constexpr int kFnAlign = 16;     // function padding (one MFMA row tile)
-constexpr int kDensPts = 64;     // points per density block (16 per wave)
+constexpr int kDensPts = 64;     // points per k_density block (16 per wave)
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
 constexpr int kSlice = 512;      // points per Fock slice (split-K over blocks)
 constexpr int kEvalPts = 256;    // points per evaluation block
@@ -80,6 +80,7 @@ struct DevGroup {
   long long pg_off;    // P_g / F_g block: m_pad x m_pad
   int fock_base;       // first Fock item
   int pad_;
+  long long phi_off;   // function-major phi plane (split-BF16 mode): m_pad x npts_pad
 };
 
 struct EvalItem {
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__
                                                     const double* __restrict__ px,
                                                     const double* __restrict__ py,
                                                     const double* __restrict__ pz,
-                                                    T* __restrict__ vals) {
+                                                    T* __restrict__ vals, int to_phi) {
   const EvalItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int p = it.p0 + threadIdx.x;
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__
   } else {
     for (int k = 0; k < NV; ++k) v[k] = 0.0;
   }
-  T* out = vals + g.val_off + (long long)it.k * g.npts_pad + p;
+  // to_phi: the function-major phi plane of the split-BF16 path
+  T* out = vals + (to_phi ? g.phi_off : g.val_off) + (long long)it.k * g.npts_pad + p;
   const long long plane = (long long)g.m_pad * g.npts_pad;
   for (int k = 0; k < NV; ++k) out[k * plane] = (T)v[k];
 }
@@ -364,6 +366,12 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
   }
 }
 
+// Item i of an accumulator with a runtime index (selects, no scratch).
+template <class A>
+__device__ inline double pick(const A& v, int i) {
+  return (double)(i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3])));
+}
+
 // ---------------------------------------------------------------------------
 // Fock: one block (4 waves) per (group, tile pair, 512-point slice)
 // ---------------------------------------------------------------------------
@@ -450,6 +458,340 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
           for (int i = 0; i < 4; ++i) {
             const int e = (16 * r + Mma<T>::row(lk, i)) * kFnChunk + 16 * c + lrow;
             const double x = kFlush ? dacc[r][c][i] : (double)acc[r][c][i];
+            sF[e] = (w == 0) ? x : sF[e] + x;
+          }
+    }
+    __syncthreads();
+  }
+  double* out = fpart + (long long)blockIdx.x * kFnChunk * kFnChunk;
+  for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) out[e] = sF[e];
+}
+
+// ---------------------------------------------------------------------------
+// split-BF16 (BF16X6) contractions
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+typedef float fl4 __attribute__((ext_vector_type(4)));
+
+// Exact three-piece split of eight FP32 values: x = hi + mid + lo, each
+// piece what is left rounded to nearest BF16 (8 significant bits).  A FP32
+// significand has 24 bits, so the three pieces hold x exactly and every
+// subtraction is exact (Sterbenz).  Rounding to nearest (rather than
+// truncating) gives the mid and lo pieces either sign, so the truncation the
+// BF16 matrix unit applies when it aligns addends does not pile up in one
+// direction.
+__device__ inline void split3(const float (&x)[8], bf8& h, bf8& m, bf8& l) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const __bf16 bh = (__bf16)x[q];
+    const float r = x[q] - (float)bh;
+    const __bf16 bm = (__bf16)r;
+    const float r2 = r - (float)bm;
+    h[q] = bh;
+    m[q] = bm;
+    l[q] = (__bf16)r2;
+  }
+}
+
+// 16x16x32 BF16 MFMA (measured on the MI355X): lane l supplies
+// A[l%16][8 (l/16) + q] and B[8 (l/16) + q][l%16], q = 0..7; the result
+// D[4 (l/16) + i][l%16] is item i of lane l.  The unit adds the 32 products
+// and C aligned to the largest of them and truncates what falls below that
+// window (measured), so a long chain onto a large C loses the low bits of
+// every small addend: callers start each K step from zero and add the step
+// into their running sum on the VALU.  The six piece products of weight
+// >= 2^-16 relative, small ones first; returns the step's tile.
+__device__ inline f4 mma6(const bf8& ah, const bf8& am, const bf8& al, const bf8& bh,
+                          const bf8& bm, const bf8& bl) {
+  f4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, f4{0, 0, 0, 0}, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  return c;
+}
+
+__device__ inline void load8(const float* p, float (&x)[8]) {
+  const fl4 a = *reinterpret_cast<const fl4*>(p);
+  const fl4 b = *reinterpret_cast<const fl4*>(p + 4);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[q] = a[q];
+    x[4 + q] = b[q];
+  }
+}
+
+// Point-major FP32 basis planes [value][point][function] (functions
+// contiguous) for the split-BF16 path: one (point, function) per thread.
+template <int NV>
+__global__ __launch_bounds__(256) void k_eval_pm(const GatherItem* __restrict__ items,
+                                                 const DevGroup* __restrict__ groups,
+                                                 const int* __restrict__ fnl,
+                                                 const DevFn* __restrict__ bas,
+                                                 const double* __restrict__ px,
+                                                 const double* __restrict__ py,
+                                                 const double* __restrict__ pz,
+                                                 float* __restrict__ vals) {
+  const GatherItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int e = it.start + threadIdx.x;
+  if (e >= g.npts_pad * g.m_pad) return;
+  const int p = e / g.m_pad, k = e % g.m_pad;
+  const int f = fnl[g.fn_off + k];
+  double v[NV];
+  if (f >= 0) {
+    const int q = g.pt_off + p;
+    eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
+  } else {
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  }
+  float* out = vals + g.val_off + e;
+  const long long plane = (long long)g.m_pad * g.npts_pad;
+  for (int q = 0; q < NV; ++q) out[q * plane] = (float)v[q];
+}
+
+// Density block P_g as three BF16 piece planes [piece][k][j] (true symmetric
+// P rounded to FP32, then split exactly as split3 does).
+__global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __restrict__ items,
+                                                           const DevGroup* __restrict__ groups,
+                                                           const int* __restrict__ fnl,
+                                                           const double* __restrict__ rho, int M,
+                                                           unsigned short* __restrict__ pg3) {
+  const GatherItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int e = it.start + threadIdx.x;
+  const long long psz = (long long)g.m_pad * g.m_pad;
+  if (e >= psz) return;
+  const int k = e / g.m_pad, j = e % g.m_pad;
+  const int fk = fnl[g.fn_off + k], fj = fnl[g.fn_off + j];
+  double v = 0.0;
+  if (fk >= 0 && fj >= 0) {
+    const int lo = min(fk, fj), hi = max(fk, fj);
+    v = rho[packed_index(lo, hi, M)];
+    if (lo != hi) v *= 0.5;
+  }
+  const float x = (float)v;
+  const __bf16 bh = (__bf16)x;
+  const float r = x - (float)bh;
+  const __bf16 bm = (__bf16)r;
+  const __bf16 bl = (__bf16)(r - (float)bm);
+  __bf16* out = reinterpret_cast<__bf16*>(pg3) + 3 * g.pg_off + e;
+  out[0] = bh;
+  out[psz] = bm;
+  out[2 * psz] = bl;
+}
+
+// Density + functional on split-BF16 MFMA: same blocking as k_density (64
+// points per block, 16 per wave, 64-function row chunks) with K steps of 32
+// functions.  The B operand of a lane is 8 consecutive functions of its point
+// (32 contiguous bytes of the point-major planes), split in registers; the A
+// operand is 8 consecutive columns of a P_g row, read as pre-split pieces.
+// The epilogue reads the four functions a lane holds per row tile as one
+// 16-byte load per plane and runs in FP64.
+template <int NT>
+__global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __restrict__ items,
+                                                    const DevGroup* __restrict__ groups,
+                                                    const unsigned short* __restrict__ pg3,
+                                                    const float* __restrict__ vals,
+                                                    const double* __restrict__ pw,
+                                                    double* __restrict__ factor,
+                                                    double* __restrict__ epart, int iexch) {
+  __shared__ double sE[4][2];
+  constexpr int NS = (NT == 4) ? 10 : 1;
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  double e_w = 0.0, n_w = 0.0;
+  if (p0 < g.npts_pad) {
+    const int np = g.npts_pad, mp = g.m_pad;
+    const long long plane = (long long)mp * np;
+    const long long psz = (long long)mp * mp;
+    const float* Vp = vals + g.val_off + (long long)(p0 + lrow) * mp;  // this lane's point
+    const unsigned short* Pp = pg3 + 3 * g.pg_off;
+    double s[NS];
+    for (int k = 0; k < NS; ++k) s[k] = 0.0;
+    for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
+      const int nrt = min(4, (mp - c0) / 16);
+      f4 acc[4][NT];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[rt][t] = f4{0, 0, 0, 0};
+      for (int k0 = 0; k0 < mp; k0 += 32) {
+        const int kk = k0 + 8 * lk;
+        bf8 bh[NT], bm[NT], bl[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          float x[8];
+          load8(Vp + t * plane + kk, x);
+          split3(x, bh[t], bm[t], bl[t]);
+        }
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          if (rt < nrt) {
+            const unsigned short* pa = Pp + (long long)(c0 + 16 * rt + lrow) * mp + kk;
+            const bf8 ah = *reinterpret_cast<const bf8*>(pa);
+            const bf8 am = *reinterpret_cast<const bf8*>(pa + psz);
+            const bf8 al = *reinterpret_cast<const bf8*>(pa + 2 * psz);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[rt][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
+          }
+        }
+      }
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        if (rt >= nrt) continue;
+        const float* vj = Vp + c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
+        if constexpr (NT == 4) {
+          fl4 v[10];
+#pragma unroll
+          for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
+                         yz = acc[rt][3][i];
+            const double v1 = v[1][i], v2 = v[2][i], v3 = v[3][i];
+            s[0] += (double)v[0][i] * y0;
+            s[1] += v1 * y0;
+            s[2] += v2 * y0;
+            s[3] += v3 * y0;
+            s[4] += (double)v[4][i] * y0 + v1 * yx;
+            s[5] += (double)v[5][i] * y0 + v2 * yy;
+            s[6] += (double)v[6][i] * y0 + v3 * yz;
+            s[7] += (double)v[7][i] * y0 + v1 * yy;
+            s[8] += (double)v[8][i] * y0 + v1 * yz;
+            s[9] += (double)v[9][i] * y0 + v2 * yz;
+          }
+        } else {
+          const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) s[0] += (double)v0[i] * (double)acc[rt][0][i];
+        }
+      }
+    }
+    for (int k = 0; k < NS; ++k) {  // the 4 lane groups holding the same point
+      s[k] += __shfl_xor(s[k], 16);
+      s[k] += __shfl_xor(s[k], 32);
+    }
+    if (lk == 0) {
+      const int q = g.pt_off + p0 + lrow;
+      const double dens = s[0];
+      double ex, ec, y2a;
+      if constexpr (NT == 4) {
+        const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
+        const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
+        const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
+        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+      } else {
+        xc::lda_point(dens, iexch, ex, ec, y2a);
+      }
+      const double w = pw[q];
+      factor[q] = w * y2a;
+      e_w = (dens * w) * (ex + ec);
+      n_w = dens * w;
+    }
+    for (int o = 8; o >= 1; o >>= 1) {
+      e_w += __shfl_xor(e_w, o);
+      n_w += __shfl_xor(n_w, o);
+    }
+  }
+  if (lane == 0) {
+    sE[wave][0] = e_w;
+    sE[wave][1] = n_w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    epart[2 * blockIdx.x] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
+    epart[2 * blockIdx.x + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
+  }
+}
+
+// Fock on split-BF16 MFMA, same blocking as k_fock (4 waves x 128 points of a
+// 512-point slice, 64 x 64 tile).  K steps of 32 points: a lane's operand is
+// 8 consecutive points of one function row of the function-major phi plane,
+// times the factor for the B side (rounded to FP32 from the FP64 product),
+// split in registers.  Each wave sums its 128 points in FP32 in the matrix
+// unit; the four partial tiles are added in FP64 in wave order.
+__global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__ items,
+                                                  const DevGroup* __restrict__ groups,
+                                                  const float* __restrict__ phiT,
+                                                  const double* __restrict__ factor,
+                                                  double* __restrict__ fpart) {
+  __shared__ double sF[kFnChunk * kFnChunk];
+  const FockItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lrow = lane & 15, lk = lane >> 4;
+  const int np = g.npts_pad, mp = g.m_pad;
+  const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
+  const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
+  const int pb = it.slice * kSlice + wave * (kSlice / 4);
+  const int pe = min(pb + kSlice / 4, np);
+  const float* Phi = phiT + g.phi_off;
+  const double* fac = factor + g.pt_off;
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+  for (int p = pb + 8 * lk; p - 8 * lk < pe; p += 32) {
+    const bool ok = p < pe;
+    double f[8];
+    if (ok) {
+      const d4 f0 = *reinterpret_cast<const d4*>(fac + p);
+      const d4 f1 = *reinterpret_cast<const d4*>(fac + p + 4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        f[q] = f0[q];
+        f[4 + q] = f1[q];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) f[q] = 0.0;
+    }
+    bf8 ah[4], am[4], al[4], bh[4], bm[4], bl[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float x[8];
+      if (ok && r < nri) {
+        load8(Phi + (long long)(i0 + 16 * r + lrow) * np + p, x);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = 0.f;
+      }
+      split3(x, ah[r], am[r], al[r]);
+      if (ok && r < nrj) {
+        load8(Phi + (long long)(j0 + 16 * r + lrow) * np + p, x);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = (float)((double)x[q] * f[q]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = 0.f;
+      }
+      split3(x, bh[r], bm[r], bl[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r < nri) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (c < nrj) acc[r][c] += mma6(ah[r], am[r], al[r], bh[c], bm[c], bl[c]);
+      }
+    }
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int e = (16 * r + 4 * lk + i) * kFnChunk + 16 * c + lrow;
+            const double x = (double)acc[r][c][i];
             sF[e] = (w == 0) ? x : sF[e] + x;
           }
     }
@@ -560,6 +902,9 @@ struct DeviceGrid {
   DevGroup* groups = nullptr;
   EvalItem* eitems = nullptr;
   GatherItem* gitems = nullptr;
+  GatherItem* pmitems = nullptr;  // point-major evaluation blocks (split-BF16)
+  int npm = 0;
+  float* phiT = nullptr;          // function-major phi (split-BF16 Fock operand)
   DensityItem* ditems = nullptr;
   FockItem* fitems = nullptr;
   int2* gtiles = nullptr;
@@ -649,6 +994,10 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   dg->nval = mol.gga ? 10 : 1;
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
+  const bool bf = prec == Precision::BF16X6;
+  // split-BF16 K steps are 32 functions; its P_g holds three 2-byte pieces
+  const int fn_align = bf ? 32 : kFnAlign;
+  const size_t pbytes = bf ? 3 * sizeof(unsigned short) : vbytes;
   const int G = part.ngroups, M = mol.M;
   dg->ngroups = G;
 
@@ -669,7 +1018,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   std::vector<DevGroup> groups(G);
   std::vector<int> fnl;
   std::vector<double> hx, hy, hz, hw;
-  long long pt = 0, vo = 0, po = 0;
+  long long pt = 0, vo = 0, po = 0, fo = 0;
   int fock_base = 0;
   std::vector<std::vector<int2>> membership(M);
   for (int g = 0; g < G; ++g) {
@@ -678,7 +1027,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     d.npts = n;
     d.npts_pad = (n + kPtAlign - 1) / kPtAlign * kPtAlign;
     d.m = m;
-    d.m_pad = (m + kFnAlign - 1) / kFnAlign * kFnAlign;
+    d.m_pad = (m + fn_align - 1) / fn_align * fn_align;
     d.pt_off = (int)pt;
     d.fn_off = (int)fnl.size();
     d.ntile = (d.m_pad + kFnChunk - 1) / kFnChunk;
@@ -687,6 +1036,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     d.pg_off = po;
     d.fock_base = fock_base;
     d.pad_ = 0;
+    d.phi_off = fo;
     fock_base += d.ntile * (d.ntile + 1) / 2 * d.nslice;
     for (int i = 0; i < d.m_pad; ++i) {
       const int f = i < m ? part.fn[part.fn_off[g] + i] : -1;
@@ -703,6 +1053,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     pt += d.npts_pad;
     vo += (long long)dg->nval * d.m_pad * d.npts_pad;
     po += (long long)d.m_pad * d.m_pad;
+    if (bf) fo += (long long)d.m_pad * d.npts_pad;
   }
   dg->npts_pad = pt;
   dg->val_elems = vo;
@@ -715,13 +1066,15 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return groups[a].m_pad > groups[b].m_pad; });
   std::vector<EvalItem> eitems;
-  std::vector<GatherItem> gitems;
+  std::vector<GatherItem> gitems, pmitems;
   std::vector<DensityItem> ditems;
   for (int g : order) {
     const DevGroup& d = groups[g];
     for (int k = 0; k < d.m_pad; ++k)
       for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalPts) eitems.push_back(EvalItem{g, k, p0, 0});
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
+    if (bf)
+      for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += 256) pmitems.push_back(GatherItem{g, e0});
     for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) ditems.push_back(DensityItem{g, p0});
   }
   // Fock partial k belongs to (group, tile pair, slice) in slot order
@@ -739,6 +1092,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   }
   dg->neval = (int)eitems.size();
   dg->ngather = (int)gitems.size();
+  dg->npm = (int)pmitems.size();
   dg->ndensity = (int)ditems.size();
   dg->ngroup_tiles = (int)gtiles.size();
   std::vector<int> moff(M + 1, 0);
@@ -755,6 +1109,8 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     dg->groups = c.take<DevGroup>(G);
     dg->eitems = c.take<EvalItem>(eitems.size());
     dg->gitems = c.take<GatherItem>(gitems.size());
+    dg->pmitems = c.take<GatherItem>(pmitems.size());
+    dg->phiT = c.take<float>(fo);
     dg->ditems = c.take<DensityItem>(ditems.size());
     dg->fitems = c.take<FockItem>(fitems.size());
     dg->gtiles = c.take<int2>(gtiles.size());
@@ -766,7 +1122,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     dg->rho = c.take<double>(packed);
     dg->F = c.take<double>(packed);
     dg->vals = c.take<char>(vo * vbytes);
-    dg->pg = c.take<char>(po * vbytes);
+    dg->pg = c.take<char>(po * pbytes);
     dg->fg = c.take<double>(po);
     dg->factor = c.take<double>(pt);
     dg->epart = c.take<double>(2 * ditems.size());
@@ -802,6 +1158,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   up(dg->groups, groups.data(), groups.size() * sizeof(DevGroup));
   up(dg->eitems, eitems.data(), eitems.size() * sizeof(EvalItem));
   up(dg->gitems, gitems.data(), gitems.size() * sizeof(GatherItem));
+  up(dg->pmitems, pmitems.data(), pmitems.size() * sizeof(GatherItem));
   up(dg->ditems, ditems.data(), ditems.size() * sizeof(DensityItem));
   up(dg->fitems, fitems.data(), fitems.size() * sizeof(FockItem));
   up(dg->gtiles, gtiles.data(), gtiles.size() * sizeof(int2));
@@ -821,18 +1178,30 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
       double* v = static_cast<double*>(dg->vals);
       if (dg->gga)
         hipLaunchKernelGGL((k_eval_basis<10, double>), grid, block, 0, s, dg->eitems, dg->groups,
-                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
       else
         hipLaunchKernelGGL((k_eval_basis<1, double>), grid, block, 0, s, dg->eitems, dg->groups,
-                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
-    } else {
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
+    } else if (prec == Precision::FP32) {
       float* v = static_cast<float*>(dg->vals);
       if (dg->gga)
         hipLaunchKernelGGL((k_eval_basis<10, float>), grid, block, 0, s, dg->eitems, dg->groups,
-                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
       else
         hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
-                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+                           dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
+    } else {
+      float* v = static_cast<float*>(dg->vals);
+      if (dg->npm > 0) {
+        if (dg->gga)
+          hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
+                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+        else
+          hipLaunchKernelGGL(k_eval_pm<1>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
+                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+      }
+      hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
+                         dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->phiT, 1);
     }
     XCQ_HIP(hipGetLastError());
   }
@@ -872,6 +1241,34 @@ static void launch_solve(DeviceGrid& g, hipStream_t s, const double* d_rho, bool
   XCQ_HIP(hipGetLastError());
 }
 
+// Split-BF16 solve: pieces of P_g, BF16 MFMA density and Fock.
+static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, bool fock,
+                            hipEvent_t* ev) {
+  unsigned short* pg3 = static_cast<unsigned short*>(g.pg);
+  const float* vals = static_cast<const float*>(g.vals);
+  if (g.ngather > 0)
+    hipLaunchKernelGGL(k_gather_density_bf, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
+                       g.fnl, d_rho, g.M, pg3);
+  XCQ_HIP(hipEventRecord(ev[1], s));
+  if (g.ndensity > 0) {
+    if (g.gga)
+      hipLaunchKernelGGL(k_density_bf<4>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
+                         pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+    else
+      hipLaunchKernelGGL(k_density_bf<1>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
+                         pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+  }
+  XCQ_HIP(hipGetLastError());
+  XCQ_HIP(hipEventRecord(ev[2], s));
+  if (fock && g.nfock > 0) {
+    hipLaunchKernelGGL(k_fock_bf, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.phiT,
+                       g.factor, g.fpart);
+    hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
+                       g.gtiles, g.fpart, g.fg);
+  }
+  XCQ_HIP(hipGetLastError());
+}
+
 SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
   DeviceGrid& g = *grid_;
@@ -882,8 +1279,10 @@ SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_
   XCQ_HIP(hipEventRecord(ev[0], s));
   if (g.prec == Precision::FP64)
     launch_solve<double>(g, s, d_rho, d_fock != nullptr, ev);
-  else
+  else if (g.prec == Precision::FP32)
     launch_solve<float>(g, s, d_rho, d_fock != nullptr, ev);
+  else
+    launch_solve_bf(g, s, d_rho, d_fock != nullptr, ev);
   XCQ_HIP(hipEventRecord(ev[3], s));
   if (d_fock)
     hipLaunchKernelGGL(k_fock_pack, dim3((M + 127) / 128, M), dim3(128), 0, s, g.memb_off, g.memb,

@@ -112,7 +112,12 @@ GridPartition build_partition(const Molecule& mol, const Options& opt);
 //   FP64: FP64 values, FP64 MFMA.
 //   FP32: FP32 values (half the bytes), FP32 MFMA, Fock products summed in
 //         the matrix unit over at most 32 points before FP64 accumulation.
-enum class Precision : int { FP64 = 0, FP32 = 1 };
+//   BF16X6: FP32 values stored point-major; every operand is split exactly
+//         into three BF16 pieces (8 significant bits each) and a tile
+//         product is the six piece products of weight >= 2^-16 relative
+//         (hi*hi, hi*mid, mid*hi, hi*lo, mid*mid, lo*hi) on BF16 MFMA with
+//         FP32 accumulation.
+enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 
 struct SolveResult {
   double energy = 0.0;

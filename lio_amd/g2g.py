@@ -99,7 +99,7 @@ def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
-PRECISION = {"fp64": 0, "fp32": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x6": 2}
 
 DEFAULT_OPTIONS = dict(little_cube_size=8.0, sphere_radius=0.6, assign_all_functions=0,
                        remove_zero_weights=1, min_points_per_cube=1, max_function_exponent=10)
