@@ -120,6 +120,38 @@ void* xcq_stream(xcq_context* ctx);
 int xcq_host_partition(const xcq_problem* problem, unsigned grid_type,
                        xcq_group_info* out, int max, int* ngroups);
 
+/* ---- Batches: several molecules resident on one device --------------------
+ * No reference counterpart (LIO solves one molecule per process); for callers
+ * that build many independent XC Fock matrices at once.  Each solve phase is
+ * one launch over the groups of all molecules (heavy groups first), so
+ * uneven molecules share the machine; the per-molecule results are bitwise
+ * those of the molecule solved alone.  The molecules' packed rho and F are
+ * concatenated in molecule order; offsets[k] is the first packed entry of
+ * molecule k and offsets[n] the total (n + 1 entries).  All molecules of a
+ * batch use the same functional. */
+typedef struct xcq_batch xcq_batch;
+int xcq_batch_create(int device, int n, const xcq_problem* problems,
+                     const unsigned* grid_types, int precision, xcq_batch** out);
+int xcq_batch_destroy(xcq_batch* b);
+int xcq_batch_offsets(xcq_batch* b, long long* offsets);
+/* Queues one build of every molecule on the batch stream (device buffers). */
+int xcq_batch_solve_device(xcq_batch* b, const double* d_rho, double* d_fock);
+/* Records one build on these buffers as a hipGraph (replayed by
+ * xcq_batch_replay, one launch call per build). */
+int xcq_batch_capture(xcq_batch* b, const double* d_rho, double* d_fock);
+int xcq_batch_replay(xcq_batch* b);
+/* Waits; Exc and N of every molecule (n entries each). */
+int xcq_batch_energies(xcq_batch* b, double* energy, double* electrons);
+int xcq_batch_kernel_times(xcq_batch* b, double ms[4]);
+void* xcq_batch_stream(xcq_batch* b);
+long long xcq_batch_arena_bytes(xcq_batch* b);
+/* Autotuner: times `reps` builds for every candidate Fock slice length in
+ * slices[0 .. ncand) on these buffers (rebuilding the work lists for each),
+ * keeps the fastest and returns the per-candidate ms in ms_out; *best is its
+ * index.  A captured graph is dropped. */
+int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps,
+                   const int* slices, int ncand, double* ms_out, int* best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,6 +250,136 @@ int xcq_host_partition(const xcq_problem* p, unsigned grid_type, xcq_group_info*
   });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+struct xcq_batch {
+  std::unique_ptr<xcq::Engine> engine;
+  std::vector<xcq::Molecule> mols;
+  std::vector<xcq::GridPartition> parts;
+  xcq::Precision prec = xcq::Precision::FP64;
+};
+
+namespace {
+
+void load_batch(xcq_batch* b) {
+  std::vector<const xcq::Molecule*> m;
+  std::vector<const xcq::GridPartition*> p;
+  for (size_t k = 0; k < b->mols.size(); ++k) {
+    m.push_back(&b->mols[k]);
+    p.push_back(&b->parts[k]);
+  }
+  b->engine->load_batch(m, p, b->prec);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xcq_batch_create(int device, int n, const xcq_problem* problems, const unsigned* grid_types,
+                     int precision, xcq_batch** out) {
+  return guarded([&] {
+    if (n <= 0 || !problems || !grid_types) throw std::runtime_error("g2g: empty batch");
+    if (precision != XCQ_FP64 && precision != XCQ_FP32 && precision != XCQ_BF16X6)
+      throw std::runtime_error("g2g: unknown precision mode");
+    std::unique_ptr<xcq_batch> b(new xcq_batch);
+    b->prec = static_cast<xcq::Precision>(precision);
+    b->engine.reset(new xcq::Engine(device));
+    b->mols.resize(n);
+    b->parts.resize(n);
+    for (int k = 0; k < n; ++k) {
+      xcq::ProblemView pv;
+      xcq::Options opt;
+      view_from_problem(&problems[k], pv, opt);
+      check_problem(pv);
+      b->mols[k] = xcq::capture(pv, grid_types[k]);
+      b->parts[k] = xcq::build_partition(b->mols[k], opt);
+    }
+    load_batch(b.get());
+    *out = b.release();
+  });
+}
+
+int xcq_batch_destroy(xcq_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int xcq_batch_offsets(xcq_batch* b, long long* offsets) {
+  return guarded([&] {
+    const int n = b->engine->molecules();
+    for (int k = 0; k <= n; ++k) offsets[k] = b->engine->packed_offset(k);
+  });
+}
+
+int xcq_batch_solve_device(xcq_batch* b, const double* d_rho, double* d_fock) {
+  return guarded([&] { b->engine->solve_device(d_rho, d_fock, true); });
+}
+
+int xcq_batch_capture(xcq_batch* b, const double* d_rho, double* d_fock) {
+  return guarded([&] { b->engine->capture_graph(d_rho, d_fock); });
+}
+
+int xcq_batch_replay(xcq_batch* b) {
+  return guarded([&] { b->engine->replay_graph(); });
+}
+
+int xcq_batch_energies(xcq_batch* b, double* energy, double* electrons) {
+  return guarded([&] {
+    const int n = b->engine->molecules();
+    std::vector<double> en(2 * n);
+    b->engine->read_energies(en.data());
+    for (int k = 0; k < n; ++k) {
+      if (energy) energy[k] = en[2 * k];
+      if (electrons) electrons[k] = en[2 * k + 1];
+    }
+  });
+}
+
+int xcq_batch_kernel_times(xcq_batch* b, double ms[4]) {
+  return guarded([&] { b->engine->kernel_times(ms); });
+}
+
+void* xcq_batch_stream(xcq_batch* b) { return b ? b->engine->stream() : nullptr; }
+
+long long xcq_batch_arena_bytes(xcq_batch* b) {
+  return b ? (long long)b->engine->arena_bytes() : -1;
+}
+
+int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, const int* slices,
+                   int ncand, double* ms_out, int* best) {
+  return guarded([&] {
+    if (ncand <= 0 || reps <= 0) throw std::runtime_error("g2g: nothing to tune");
+    int ibest = -1;
+    double tbest = 0.0;
+    for (int c = 0; c < ncand; ++c) {
+      xcq::Tuning t = b->engine->tuning();
+      t.fock_slice = slices[c];
+      b->engine->set_tuning(t);
+      load_batch(b);
+      b->engine->solve_device(d_rho, d_fock, true);  // warm-up
+      double e[2];
+      b->engine->read_energy(&e[0], &e[1]);
+      const double t0 = wall();
+      for (int r = 0; r < reps; ++r) b->engine->solve_device(d_rho, d_fock, true);
+      b->engine->read_energy(&e[0], &e[1]);
+      const double ms = 1e3 * (wall() - t0) / reps;
+      if (ms_out) ms_out[c] = ms;
+      if (ibest < 0 || ms < tbest) {
+        ibest = c;
+        tbest = ms;
+      }
+    }
+    xcq::Tuning t = b->engine->tuning();
+    t.fock_slice = slices[ibest];
+    b->engine->set_tuning(t);
+    load_batch(b);
+    if (best) *best = ibest;
+  });
+}
+
 // ---------------------------------------------------------------------------
 // g2g Fortran ABI on the default context
 // ---------------------------------------------------------------------------

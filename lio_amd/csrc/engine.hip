@@ -55,7 +55,7 @@ constexpr int kPtAlign = 16;     // point padding (one MFMA column tile)
 constexpr int kFnAlign = 16;     // function padding (one MFMA row tile)
 constexpr int kDensPts = 64;     // points per k_density block (16 per wave)
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
-constexpr int kSlice = 512;      // points per Fock slice (split-K over blocks)
+constexpr int kSliceDefault = 512;  // points per Fock slice (split-K over blocks)
 constexpr int kEvalPts = 256;    // points per evaluation block
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -81,6 +81,17 @@ struct DevGroup {
   int fock_base;       // first Fock item
   int pad_;
   long long phi_off;   // function-major phi plane (split-BF16 mode): m_pad x npts_pad
+  int mol;             // molecule of the batch
+  int pad2_;
+};
+
+// One molecule of a batch: its functions are [fn_base, fn_base + M) of the
+// batch numbering, its packed rho/F start at rho_off of the concatenated
+// buffers, and its density blocks are eidx[eb .. ee) (ascending).
+struct DevMol {
+  int M, fn_base;
+  long long rho_off;
+  int eb, ee;
 };
 
 struct EvalItem {
@@ -211,24 +222,29 @@ __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__
 // ---------------------------------------------------------------------------
 // density blocks P_g[k][j] = P(f_k, f_j) with the true symmetric P
 // ---------------------------------------------------------------------------
+// Entry (fk, fj) of the true symmetric density (packed off-diagonals
+// halved); fk, fj are batch function indices of molecule mol, or -1.
+__device__ inline double density_entry(const DevMol& mol, const double* __restrict__ rho, int fk,
+                                       int fj) {
+  if (fk < 0 || fj < 0) return 0.0;
+  const int lo = min(fk, fj) - mol.fn_base, hi = max(fk, fj) - mol.fn_base;
+  const double v = rho[mol.rho_off + packed_index(lo, hi, mol.M)];
+  return lo != hi ? 0.5 * v : v;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __restrict__ items,
                                                         const DevGroup* __restrict__ groups,
+                                                        const DevMol* __restrict__ mols,
                                                         const int* __restrict__ fnl,
-                                                        const double* __restrict__ rho, int M,
+                                                        const double* __restrict__ rho,
                                                         T* __restrict__ pg) {
   const GatherItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int e = it.start + threadIdx.x;
   if (e >= g.m_pad * g.m_pad) return;
   const int k = e / g.m_pad, j = e % g.m_pad;
-  const int fk = fnl[g.fn_off + k], fj = fnl[g.fn_off + j];
-  double v = 0.0;
-  if (fk >= 0 && fj >= 0) {
-    const int lo = min(fk, fj), hi = max(fk, fj);
-    v = rho[packed_index(lo, hi, M)];
-    if (lo != hi) v *= 0.5;
-  }
+  const double v = density_entry(mols[g.mol], rho, fnl[g.fn_off + k], fnl[g.fn_off + j]);
   pg[g.pg_off + e] = (T)v;
 }
 
@@ -386,7 +402,7 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
                                                  const DevGroup* __restrict__ groups,
                                                  const T* __restrict__ vals,
                                                  const double* __restrict__ factor,
-                                                 double* __restrict__ fpart) {
+                                                 double* __restrict__ fpart, int slen) {
   typedef typename Mma<T>::acc A4;
   typedef T V4 __attribute__((ext_vector_type(4)));
   // FP32 products are summed over at most 32 points in the matrix unit and
@@ -399,8 +415,8 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
   const int np = g.npts_pad, mp = g.m_pad;
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
-  const int pb = it.slice * kSlice + wave * (kSlice / 4);
-  const int pe = min(pb + kSlice / 4, np);
+  const int pb = it.slice * slen + wave * (slen / 4);
+  const int pe = min(pb + slen / 4, np);
   const T* V = vals + g.val_off;
   const double* fac = factor + g.pt_off;
   A4 acc[4][4];
@@ -474,42 +490,55 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
 
-// Exact three-piece split of eight FP32 values: x = hi + mid + lo, each
-// piece what is left rounded to nearest BF16 (8 significant bits).  A FP32
-// significand has 24 bits, so the three pieces hold x exactly and every
-// subtraction is exact (Sterbenz).  Rounding to nearest (rather than
-// truncating) gives the mid and lo pieces either sign, so the truncation the
-// BF16 matrix unit applies when it aligns addends does not pile up in one
-// direction.
+// Block fixed-point split of the 32 values of one K step (8 per lane, in the
+// 4 lanes l, l^16, l^32, l^48 that hold the same row or point): they share
+// one exponent e (2^(e-1) <= max |x| < 2^e) and each value becomes integers
+// a0 + a1 2^-8 + a2 2^-16 (|a0| <= 256, |a1|, |a2| <= 128, round to nearest)
+// in units of 2^(e-8), kept as the BF16 pieces a_s 2^(e-8-8s) (exact: at most
+// 9 significant bits).  The error is below 2^-24 of the block maximum.  With
+// both operands split this way every product of one MFMA output is an
+// integer multiple of one unit and their sum (below 2^21 units) is exact in
+// the matrix unit, whose adder otherwise truncates products that lie far
+// below the largest one (measured) -- a bias on quadratic forms such as rho.
+// All 64 lanes must call it (cross-lane maximum).
 __device__ inline void split3(const float (&x)[8], bf8& h, bf8& m, bf8& l) {
+  float mx = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) mx = fmaxf(mx, fabsf(x[q]));
+  mx = fmaxf(mx, __shfl_xor(mx, 16));
+  mx = fmaxf(mx, __shfl_xor(mx, 32));
+  int e;
+  (void)frexpf(mx, &e);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const __bf16 bh = (__bf16)x[q];
-    const float r = x[q] - (float)bh;
-    const __bf16 bm = (__bf16)r;
-    const float r2 = r - (float)bm;
-    h[q] = bh;
-    m[q] = bm;
-    l[q] = (__bf16)r2;
+    const float xs = ldexpf(x[q], 8 - e);  // |xs| < 256, exact
+    const float a0 = rintf(xs);
+    const float r1 = (xs - a0) * 256.f;    // exact, |r1| <= 128
+    const float a1 = rintf(r1);
+    const float a2 = rintf((r1 - a1) * 256.f);
+    h[q] = (__bf16)ldexpf(a0, e - 8);
+    m[q] = (__bf16)ldexpf(a1, e - 16);
+    l[q] = (__bf16)ldexpf(a2, e - 24);
   }
 }
 
 // 16x16x32 BF16 MFMA (measured on the MI355X): lane l supplies
 // A[l%16][8 (l/16) + q] and B[8 (l/16) + q][l%16], q = 0..7; the result
-// D[4 (l/16) + i][l%16] is item i of lane l.  The unit adds the 32 products
-// and C aligned to the largest of them and truncates what falls below that
-// window (measured), so a long chain onto a large C loses the low bits of
-// every small addend: callers start each K step from zero and add the step
-// into their running sum on the VALU.  The six piece products of weight
-// >= 2^-16 relative, small ones first; returns the step's tile.
+// D[4 (l/16) + i][l%16] is item i of lane l.  The unit sums the 32 products
+// exactly for block fixed-point pieces, and rounds C + sum to nearest when C
+// is the larger term; a C much smaller than the products is aligned to them
+// and loses its low bits.  So the six piece products of weight >= 2^-16
+// relative run largest first (hi*hi from C = 0), each later one adding
+// smaller products onto a larger C.  Returns the step's tile, which callers
+// add into their running sum on the VALU.
 __device__ inline f4 mma6(const bf8& ah, const bf8& am, const bf8& al, const bf8& bh,
                           const bf8& bm, const bf8& bl) {
-  f4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, f4{0, 0, 0, 0}, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c, 0, 0, 0);
+  f4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, f4{0, 0, 0, 0}, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
   return c;
 }
 
@@ -552,35 +581,39 @@ __global__ __launch_bounds__(256) void k_eval_pm(const GatherItem* __restrict__ 
   for (int q = 0; q < NV; ++q) out[q * plane] = (float)v[q];
 }
 
-// Density block P_g as three BF16 piece planes [piece][k][j] (true symmetric
-// P rounded to FP32, then split exactly as split3 does).
+// Density block P_g as three BF16 piece planes [piece][j][k] (true symmetric
+// P), split in blocks of 32 consecutive k of one row exactly as split3 does
+// (the 32 threads of a block are 32 consecutive lanes of one wave).
 __global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __restrict__ items,
                                                            const DevGroup* __restrict__ groups,
+                                                           const DevMol* __restrict__ mols,
                                                            const int* __restrict__ fnl,
-                                                           const double* __restrict__ rho, int M,
+                                                           const double* __restrict__ rho,
                                                            unsigned short* __restrict__ pg3) {
   const GatherItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int e = it.start + threadIdx.x;
   const long long psz = (long long)g.m_pad * g.m_pad;
-  if (e >= psz) return;
-  const int k = e / g.m_pad, j = e % g.m_pad;
-  const int fk = fnl[g.fn_off + k], fj = fnl[g.fn_off + j];
+  const bool in = e < psz;  // whole 32-lane blocks are in or out (m_pad % 32 == 0)
   double v = 0.0;
-  if (fk >= 0 && fj >= 0) {
-    const int lo = min(fk, fj), hi = max(fk, fj);
-    v = rho[packed_index(lo, hi, M)];
-    if (lo != hi) v *= 0.5;
+  if (in) {
+    const int k = e / g.m_pad, j = e % g.m_pad;
+    v = density_entry(mols[g.mol], rho, fnl[g.fn_off + k], fnl[g.fn_off + j]);
   }
-  const float x = (float)v;
-  const __bf16 bh = (__bf16)x;
-  const float r = x - (float)bh;
-  const __bf16 bm = (__bf16)r;
-  const __bf16 bl = (__bf16)(r - (float)bm);
+  double mx = fabs(v);
+  for (int o = 1; o < 32; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  int ex;
+  (void)frexp(mx, &ex);
+  const double xs = ldexp(v, 8 - ex);
+  const double a0 = rint(xs);
+  const double r1 = (xs - a0) * 256.0;
+  const double a1 = rint(r1);
+  const double a2 = rint((r1 - a1) * 256.0);
+  if (!in) return;
   __bf16* out = reinterpret_cast<__bf16*>(pg3) + 3 * g.pg_off + e;
-  out[0] = bh;
-  out[psz] = bm;
-  out[2 * psz] = bl;
+  out[0] = (__bf16)(float)ldexp(a0, ex - 8);
+  out[psz] = (__bf16)(float)ldexp(a1, ex - 16);
+  out[2 * psz] = (__bf16)(float)ldexp(a2, ex - 24);
 }
 
 // Density + functional on split-BF16 MFMA: same blocking as k_density (64
@@ -720,7 +753,7 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
                                                   const DevGroup* __restrict__ groups,
                                                   const float* __restrict__ phiT,
                                                   const double* __restrict__ factor,
-                                                  double* __restrict__ fpart) {
+                                                  double* __restrict__ fpart, int slen) {
   __shared__ double sF[kFnChunk * kFnChunk];
   const FockItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
@@ -728,8 +761,8 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
   const int np = g.npts_pad, mp = g.m_pad;
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
-  const int pb = it.slice * kSlice + wave * (kSlice / 4);
-  const int pe = min(pb + kSlice / 4, np);
+  const int pb = it.slice * slen + wave * (slen / 4);
+  const int pe = min(pb + slen / 4, np);
   const float* Phi = phiT + g.phi_off;
   const double* fac = factor + g.pt_off;
   f4 acc[4][4];
@@ -828,13 +861,16 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
 
 // Packed F: entry (I <= J) sums F_g(li, lj) over the groups holding both
 // functions, in ascending group order.  memb holds, per function, its
-// (group, local index) pairs sorted by group.
+// (group, local index) pairs sorted by group.  I (blockIdx.y) and J are
+// batch function indices; J runs over the rest of I's molecule.
 __global__ void k_fock_pack(const int* __restrict__ memb_off, const int2* __restrict__ memb,
-                            const DevGroup* __restrict__ groups, const double* __restrict__ fg,
-                            double* __restrict__ F, int M) {
-  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+                            const DevGroup* __restrict__ groups, const DevMol* __restrict__ mols,
+                            const int* __restrict__ fn_mol, const double* __restrict__ fg,
+                            double* __restrict__ F) {
   const int I = blockIdx.y;
-  if (J >= M || J < I) return;
+  const DevMol mol = mols[fn_mol[I]];
+  const int J = mol.fn_base + blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= mol.fn_base + mol.M || J < I) return;
   int a = memb_off[I], ae = memb_off[I + 1];
   int b = memb_off[J], be = memb_off[J + 1];
   double s = 0.0;
@@ -852,17 +888,21 @@ __global__ void k_fock_pack(const int* __restrict__ memb_off, const int2* __rest
       ++b;
     }
   }
-  F[packed_index(I, J, M)] = s;
+  F[mol.rho_off + packed_index(I - mol.fn_base, J - mol.fn_base, mol.M)] = s;
 }
 
-// Fixed-order reduction of the per-block (Exc, N) pairs.
-__global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epart, int n,
+// Fixed-order reduction of the per-block (Exc, N) pairs of each molecule
+// (block b: molecule b, its density blocks eidx[eb .. ee)).
+__global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epart,
+                                                 const DevMol* __restrict__ mols,
+                                                 const int* __restrict__ eidx,
                                                  double* __restrict__ out) {
   __shared__ double se[1024], sn[1024];
+  const DevMol mol = mols[blockIdx.x];
   double e = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    e += epart[2 * i];
-    c += epart[2 * i + 1];
+  for (int i = mol.eb + threadIdx.x; i < mol.ee; i += 1024) {
+    e += epart[2 * eidx[i]];
+    c += epart[2 * eidx[i] + 1];
   }
   se[threadIdx.x] = e;
   sn[threadIdx.x] = c;
@@ -875,8 +915,8 @@ __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epar
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = se[0];
-    out[1] = sn[0];
+    out[2 * blockIdx.x] = se[0];
+    out[2 * blockIdx.x + 1] = sn[0];
   }
 }
 
@@ -891,7 +931,14 @@ double wall() {
 // Device-resident molecule: every buffer lives in one arena allocation.
 // ---------------------------------------------------------------------------
 struct DeviceGrid {
-  int M = 0, natom = 0, iexch = 9, nval = 10;
+  int M = 0, natom = 0, iexch = 9, nval = 10;  // M, natom: whole batch
+  int nmol = 1, max_M = 0;
+  int fock_slice = kSliceDefault;
+  long long packed = 0;                 // packed entries of the whole batch
+  std::vector<long long> mol_packed;    // per molecule offset, then the total
+  DevMol* mols = nullptr;
+  int* fn_mol = nullptr;
+  int* eidx = nullptr;
   bool gga = true;
   int ngroups = 0, neval = 0, ngather = 0, ndensity = 0, nfock = 0, ngroup_tiles = 0;
   long long npts_pad = 0, val_elems = 0, pg_elems = 0;
@@ -972,6 +1019,7 @@ static void release(DeviceGrid* g) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
+  drop_graph();
   release(grid_);
   for (auto& e : events_)
     if (e) (void)hipEventDestroy((hipEvent_t)e);
@@ -982,38 +1030,77 @@ size_t Engine::arena_bytes() const { return grid_ ? grid_->arena_bytes : 0; }
 double Engine::eval_ms() const { return grid_ ? grid_->eval_ms : 0.0; }
 
 void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec) {
+  load_batch({&mol}, {&part}, prec);
+}
+
+void Engine::load_batch(const std::vector<const Molecule*>& mols,
+                        const std::vector<const GridPartition*>& parts, Precision prec) {
   XCQ_HIP(hipSetDevice(device_));
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
+  drop_graph();
   release(grid_);
   grid_ = nullptr;
+  const int nmol = (int)mols.size();
+  if (nmol == 0 || parts.size() != mols.size())
+    throw std::runtime_error("g2g: a batch needs one partition per molecule");
+  for (const Molecule* m : mols)
+    if (m->gga != mols[0]->gga || m->iexch != mols[0]->iexch)
+      throw std::runtime_error("g2g: the molecules of a batch must share the functional");
+  if (tuning_.fock_slice < 128 || tuning_.fock_slice % 128 != 0)
+    throw std::runtime_error("g2g: the Fock slice must be a positive multiple of 128 points");
   auto* dg = new DeviceGrid;
-  dg->M = mol.M;
-  dg->natom = mol.natom;
-  dg->iexch = mol.iexch;
-  dg->gga = mol.gga;
-  dg->nval = mol.gga ? 10 : 1;
+  dg->nmol = nmol;
+  dg->fock_slice = tuning_.fock_slice;
+  dg->iexch = mols[0]->iexch;
+  dg->gga = mols[0]->gga;
+  dg->nval = dg->gga ? 10 : 1;
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const bool bf = prec == Precision::BF16X6;
   // split-BF16 K steps are 32 functions; its P_g holds three 2-byte pieces
   const int fn_align = bf ? 32 : kFnAlign;
   const size_t pbytes = bf ? 3 * sizeof(unsigned short) : vbytes;
-  const int G = part.ngroups, M = mol.M;
+
+  // batch numbering: molecule k owns functions [fn_base, fn_base + M) and
+  // packed entries [rho_off, rho_off + M (M + 1) / 2)
+  std::vector<DevMol> hmol(nmol);
+  int M = 0, G = 0;
+  long long packed = 0;
+  for (int k = 0; k < nmol; ++k) {
+    hmol[k].M = mols[k]->M;
+    hmol[k].fn_base = M;
+    hmol[k].rho_off = packed;
+    M += mols[k]->M;
+    G += parts[k]->ngroups;
+    packed += (long long)mols[k]->M * (mols[k]->M + 1) / 2;
+    dg->max_M = std::max(dg->max_M, mols[k]->M);
+    dg->natom += mols[k]->natom;
+  }
+  dg->M = M;
+  dg->packed = packed;
   dg->ngroups = G;
+  dg->mol_packed.resize(nmol + 1);
+  for (int k = 0; k < nmol; ++k) dg->mol_packed[k] = hmol[k].rho_off;
+  dg->mol_packed[nmol] = packed;
 
   std::vector<DevFn> bas(M);
-  for (int f = 0; f < M; ++f) {
-    const BasisFn& b = mol.fn[f];
-    DevFn& d = bas[f];
-    std::memcpy(d.a, b.a, sizeof d.a);
-    std::memcpy(d.c, b.c, sizeof d.c);
-    d.cx = mol.x[b.atom];
-    d.cy = mol.y[b.atom];
-    d.cz = mol.z[b.atom];
-    d.ncont = b.ncont;
-    d.lx = b.lx;
-    d.ly = b.ly;
-    d.lz = b.lz;
+  std::vector<int> fn_mol(M);
+  for (int k = 0; k < nmol; ++k) {
+    const Molecule& mol = *mols[k];
+    for (int f = 0; f < mol.M; ++f) {
+      const BasisFn& b = mol.fn[f];
+      DevFn& d = bas[hmol[k].fn_base + f];
+      std::memcpy(d.a, b.a, sizeof d.a);
+      std::memcpy(d.c, b.c, sizeof d.c);
+      d.cx = mol.x[b.atom];
+      d.cy = mol.y[b.atom];
+      d.cz = mol.z[b.atom];
+      d.ncont = b.ncont;
+      d.lx = b.lx;
+      d.ly = b.ly;
+      d.lz = b.lz;
+      fn_mol[hmol[k].fn_base + f] = k;
+    }
   }
   std::vector<DevGroup> groups(G);
   std::vector<int> fnl;
@@ -1021,9 +1108,16 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   long long pt = 0, vo = 0, po = 0, fo = 0;
   int fock_base = 0;
   std::vector<std::vector<int2>> membership(M);
-  for (int g = 0; g < G; ++g) {
+  for (int g = 0, k = 0, gk = 0; g < G; ++g, ++gk) {
+    while (gk == parts[k]->ngroups) {
+      ++k;
+      gk = 0;
+    }
+    const GridPartition& part = *parts[k];
     DevGroup& d = groups[g];
-    const int n = part.npts[g], m = part.m[g];
+    d.mol = k;
+    d.pad2_ = 0;
+    const int n = part.npts[gk], m = part.m[gk];
     d.npts = n;
     d.npts_pad = (n + kPtAlign - 1) / kPtAlign * kPtAlign;
     d.m = m;
@@ -1031,7 +1125,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     d.pt_off = (int)pt;
     d.fn_off = (int)fnl.size();
     d.ntile = (d.m_pad + kFnChunk - 1) / kFnChunk;
-    d.nslice = (d.npts_pad + kSlice - 1) / kSlice;
+    d.nslice = (d.npts_pad + tuning_.fock_slice - 1) / tuning_.fock_slice;
     d.val_off = vo;
     d.pg_off = po;
     d.fock_base = fock_base;
@@ -1039,12 +1133,12 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     d.phi_off = fo;
     fock_base += d.ntile * (d.ntile + 1) / 2 * d.nslice;
     for (int i = 0; i < d.m_pad; ++i) {
-      const int f = i < m ? part.fn[part.fn_off[g] + i] : -1;
+      const int f = i < m ? hmol[k].fn_base + part.fn[part.fn_off[gk] + i] : -1;
       fnl.push_back(f);
       if (f >= 0) membership[f].push_back(make_int2(g, i));
     }
     for (int p = 0; p < d.npts_pad; ++p) {
-      const int src = part.pt_off[g] + (p < n ? p : 0);  // padding repeats point 0 with w = 0
+      const int src = part.pt_off[gk] + (p < n ? p : 0);  // padding repeats point 0 with w = 0
       hx.push_back(part.x[src]);
       hy.push_back(part.y[src]);
       hz.push_back(part.z[src]);
@@ -1077,6 +1171,15 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
       for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += 256) pmitems.push_back(GatherItem{g, e0});
     for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) ditems.push_back(DensityItem{g, p0});
   }
+  // per molecule, its density blocks in ascending block order (the energy
+  // reduction order, the same whether the molecule is alone or batched)
+  std::vector<int> eidx;
+  for (int k = 0; k < nmol; ++k) {
+    hmol[k].eb = (int)eidx.size();
+    for (int i = 0; i < (int)ditems.size(); ++i)
+      if (groups[ditems[i].group].mol == k) eidx.push_back(i);
+    hmol[k].ee = (int)eidx.size();
+  }
   // Fock partial k belongs to (group, tile pair, slice) in slot order
   std::vector<FockItem> fitems(fock_base);
   std::vector<int2> gtiles;
@@ -1103,7 +1206,6 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   }
   moff[M] = (int)memb.size();
 
-  const size_t packed = (size_t)M * (M + 1) / 2;
   auto plan = [&](Carver& c) {
     dg->bas = c.take<DevFn>(M);
     dg->groups = c.take<DevGroup>(G);
@@ -1129,7 +1231,10 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
     dg->fpart = c.take<double>((size_t)fitems.size() * kFnChunk * kFnChunk);
     dg->memb_off = c.take<int>(M + 1);
     dg->memb = c.take<int2>(memb.size() + 1);
-    dg->eout = c.take<double>(2);
+    dg->mols = c.take<DevMol>(nmol);
+    dg->fn_mol = c.take<int>(M);
+    dg->eidx = c.take<int>(eidx.size() + 1);
+    dg->eout = c.take<double>(2 * nmol);
   };
   Carver sizing{nullptr};
   plan(sizing);
@@ -1148,7 +1253,7 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   plan(carve);
   XCQ_HIP(hipHostMalloc(&dg->h_rho, packed * sizeof(double), hipHostMallocDefault));
   XCQ_HIP(hipHostMalloc(&dg->h_F, packed * sizeof(double), hipHostMallocDefault));
-  XCQ_HIP(hipHostMalloc(&dg->h_e, 2 * sizeof(double), hipHostMallocDefault));
+  XCQ_HIP(hipHostMalloc(&dg->h_e, 2 * nmol * sizeof(double), hipHostMallocDefault));
 
   hipStream_t s = (hipStream_t)stream_;
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -1169,6 +1274,9 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   up(dg->pw, hw.data(), hw.size() * sizeof(double));
   up(dg->memb_off, moff.data(), moff.size() * sizeof(int));
   up(dg->memb, memb.data(), memb.size() * sizeof(int2));
+  up(dg->mols, hmol.data(), hmol.size() * sizeof(DevMol));
+  up(dg->fn_mol, fn_mol.data(), fn_mol.size() * sizeof(int));
+  up(dg->eidx, eidx.data(), eidx.size() * sizeof(int));
   // basis values of every group, kept for all solves on this grid
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
   XCQ_HIP(hipEventRecord(ev[6], s));
@@ -1213,6 +1321,12 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
   grid_ = dg;
 }
 
+// Phase boundaries are marked with ev[k] when ev is given (not while a graph
+// is being captured).
+static void mark(hipEvent_t* ev, int k, hipStream_t s) {
+  if (ev) XCQ_HIP(hipEventRecord(ev[k], s));
+}
+
 template <class T>
 static void launch_solve(DeviceGrid& g, hipStream_t s, const double* d_rho, bool fock,
                          hipEvent_t* ev) {
@@ -1220,8 +1334,8 @@ static void launch_solve(DeviceGrid& g, hipStream_t s, const double* d_rho, bool
   const T* vals = static_cast<const T*>(g.vals);
   if (g.ngather > 0)
     hipLaunchKernelGGL(k_gather_density<T>, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
-                       g.fnl, d_rho, g.M, pg);
-  XCQ_HIP(hipEventRecord(ev[1], s));
+                       g.mols, g.fnl, d_rho, pg);
+  mark(ev, 1, s);
   if (g.ndensity > 0) {
     if (g.gga)
       hipLaunchKernelGGL((k_density<4, T>), dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
@@ -1231,10 +1345,10 @@ static void launch_solve(DeviceGrid& g, hipStream_t s, const double* d_rho, bool
                          pg, vals, g.pw, g.factor, g.epart, g.iexch);
   }
   XCQ_HIP(hipGetLastError());
-  XCQ_HIP(hipEventRecord(ev[2], s));
+  mark(ev, 2, s);
   if (fock && g.nfock > 0) {
     hipLaunchKernelGGL(k_fock<T>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, vals,
-                       g.factor, g.fpart);
+                       g.factor, g.fpart, g.fock_slice);
     hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
                        g.gtiles, g.fpart, g.fg);
   }
@@ -1248,8 +1362,8 @@ static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, b
   const float* vals = static_cast<const float*>(g.vals);
   if (g.ngather > 0)
     hipLaunchKernelGGL(k_gather_density_bf, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
-                       g.fnl, d_rho, g.M, pg3);
-  XCQ_HIP(hipEventRecord(ev[1], s));
+                       g.mols, g.fnl, d_rho, pg3);
+  mark(ev, 1, s);
   if (g.ndensity > 0) {
     if (g.gga)
       hipLaunchKernelGGL(k_density_bf<4>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
@@ -1259,48 +1373,97 @@ static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, b
                          pg3, vals, g.pw, g.factor, g.epart, g.iexch);
   }
   XCQ_HIP(hipGetLastError());
-  XCQ_HIP(hipEventRecord(ev[2], s));
+  mark(ev, 2, s);
   if (fock && g.nfock > 0) {
     hipLaunchKernelGGL(k_fock_bf, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.phiT,
-                       g.factor, g.fpart);
+                       g.factor, g.fpart, g.fock_slice);
     hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
                        g.gtiles, g.fpart, g.fg);
   }
   XCQ_HIP(hipGetLastError());
 }
 
-SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy) {
-  if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
-  DeviceGrid& g = *grid_;
-  hipStream_t s = (hipStream_t)stream_;
-  const int M = g.M;
-  SolveResult res;
-  hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
-  XCQ_HIP(hipEventRecord(ev[0], s));
+// The whole build: gather, density + functional, Fock, packed reduction,
+// per-molecule energies.  Every launch covers all molecules of the batch.
+static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double* d_fock,
+                       hipEvent_t* ev) {
+  mark(ev, 0, s);
   if (g.prec == Precision::FP64)
     launch_solve<double>(g, s, d_rho, d_fock != nullptr, ev);
   else if (g.prec == Precision::FP32)
     launch_solve<float>(g, s, d_rho, d_fock != nullptr, ev);
   else
     launch_solve_bf(g, s, d_rho, d_fock != nullptr, ev);
-  XCQ_HIP(hipEventRecord(ev[3], s));
+  mark(ev, 3, s);
   if (d_fock)
-    hipLaunchKernelGGL(k_fock_pack, dim3((M + 127) / 128, M), dim3(128), 0, s, g.memb_off, g.memb,
-                       g.groups, g.fg, d_fock, M);
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(1024), 0, s, g.epart, g.ndensity, g.eout);
+    hipLaunchKernelGGL(k_fock_pack, dim3((g.max_M + 127) / 128, g.M), dim3(128), 0, s, g.memb_off,
+                       g.memb, g.groups, g.mols, g.fn_mol, g.fg, d_fock);
+  hipLaunchKernelGGL(k_energy, dim3(g.nmol), dim3(1024), 0, s, g.epart, g.mols, g.eidx, g.eout);
   XCQ_HIP(hipGetLastError());
-  XCQ_HIP(hipEventRecord(ev[4], s));
+  mark(ev, 4, s);
+}
+
+SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy) {
+  if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  launch_all(*grid_, (hipStream_t)stream_, d_rho, d_fock, reinterpret_cast<hipEvent_t*>(events_));
   (void)want_energy;
-  return res;
+  return SolveResult{};
+}
+
+void Engine::capture_graph(const double* d_rho, double* d_fock) {
+  if (!grid_) throw std::runtime_error("g2g: graph capture before the grid was built");
+  XCQ_HIP(hipSetDevice(device_));
+  hipStream_t s = (hipStream_t)stream_;
+  XCQ_HIP(hipStreamSynchronize(s));
+  drop_graph();
+  hipGraph_t graph = nullptr;
+  XCQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    launch_all(*grid_, s, d_rho, d_fock, nullptr);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  XCQ_HIP(hipStreamEndCapture(s, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  XCQ_HIP(e);
+  graph_ = exec;
+}
+
+void Engine::replay_graph() {
+  if (!graph_) throw std::runtime_error("g2g: no captured graph");
+  XCQ_HIP(hipGraphLaunch((hipGraphExec_t)graph_, (hipStream_t)stream_));
+}
+
+void Engine::drop_graph() {
+  if (graph_) (void)hipGraphExecDestroy((hipGraphExec_t)graph_);
+  graph_ = nullptr;
 }
 
 void Engine::read_energy(double* energy, double* electrons) {
+  double e[2];
+  read_energies(e);
+  if (energy) *energy = e[0];
+  if (electrons) *electrons = e[1];
+}
+
+void Engine::read_energies(double* out) {
   if (!grid_) throw std::runtime_error("g2g: no grid");
   hipStream_t s = (hipStream_t)stream_;
-  XCQ_HIP(hipMemcpyAsync(grid_->h_e, grid_->eout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  const size_t bytes = 2 * grid_->nmol * sizeof(double);
+  XCQ_HIP(hipMemcpyAsync(grid_->h_e, grid_->eout, bytes, hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipStreamSynchronize(s));
-  if (energy) *energy = grid_->h_e[0];
-  if (electrons) *electrons = grid_->h_e[1];
+  std::memcpy(out, grid_->h_e, bytes);
+}
+
+int Engine::molecules() const { return grid_ ? grid_->nmol : 0; }
+
+long long Engine::packed_offset(int k) const {
+  if (!grid_ || k < 0 || k > grid_->nmol) throw std::runtime_error("g2g: molecule out of range");
+  return grid_->mol_packed[k];
 }
 
 void Engine::kernel_times(double ms[4]) {
@@ -1319,13 +1482,13 @@ SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool wan
   DeviceGrid& g = *grid_;
   hipStream_t s = (hipStream_t)stream_;
   XCQ_HIP(hipSetDevice(device_));
-  const size_t packed = (size_t)g.M * (g.M + 1) / 2;
+  const size_t packed = (size_t)g.packed;
   std::memcpy(g.h_rho, rho_packed, packed * sizeof(double));
   XCQ_HIP(hipMemcpyAsync(g.rho, g.h_rho, packed * sizeof(double), hipMemcpyHostToDevice, s));
   SolveResult res = solve_device(g.rho, fock_accum ? g.F : nullptr, want_energy);
   if (fock_accum)
     XCQ_HIP(hipMemcpyAsync(g.h_F, g.F, packed * sizeof(double), hipMemcpyDeviceToHost, s));
-  XCQ_HIP(hipMemcpyAsync(g.h_e, g.eout, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipMemcpyAsync(g.h_e, g.eout, 2 * g.nmol * sizeof(double), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipStreamSynchronize(s));
   if (fock_accum)
     for (size_t k = 0; k < packed; ++k) fock_accum[k] += g.h_F[k];

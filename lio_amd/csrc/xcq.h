@@ -119,6 +119,14 @@ GridPartition build_partition(const Molecule& mol, const Options& opt);
 //         FP32 accumulation.
 enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 
+// Launch schedule knobs, picked per grid by the autotuner (api.cpp).
+//   fock_slice: points per Fock split-K slice (multiple of 128): longer
+//               slices leave fewer partial tiles to add, shorter ones more
+//               blocks for small groups.
+struct Tuning {
+  int fock_slice = 512;
+};
+
 struct SolveResult {
   double energy = 0.0;
   double electrons = 0.0;
@@ -140,16 +148,33 @@ class Engine {
 
   // Uploads basis, grid and work lists; replaces any previous grid.
   void load(const Molecule& mol, const GridPartition& part, Precision prec = Precision::FP64);
+  // Several molecules resident together: every solve phase is one launch over
+  // all their groups; their packed rho/F are concatenated in molecule order
+  // (packed_offset).  Results are bitwise those of each molecule alone.
+  void load_batch(const std::vector<const Molecule*>& mols,
+                  const std::vector<const GridPartition*>& parts, Precision prec);
   // One XC Fock build from packed rho (host memory).  fock_accum (host) gets
   // += F_xc; pass nullptr to skip the copy-back.
   SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy);
   // Same with device-resident input/output (no PCIe copies), used for timing.
   SolveResult solve_device(const double* d_rho_packed, double* d_fock, bool want_energy);
-  // Waits for the stream and reads the last solve's energy and electron count.
+  // Waits for the stream and reads the last solve's energy and electron count
+  // (of the first molecule); read_energies: (Exc, N) of every molecule.
   void read_energy(double* energy, double* electrons);
+  void read_energies(double* e_n);
+  // Records one full solve on these device buffers as a hipGraph; replay
+  // launches it (same buffers) on the engine stream.  A new load drops it.
+  void capture_graph(const double* d_rho_packed, double* d_fock);
+  void replay_graph();
+  bool has_graph() const { return graph_ != nullptr; }
+  int molecules() const;
+  long long packed_offset(int k) const;  // k = molecules() gives the total
   // Waits for the stream; HIP-event durations (ms) of the last solve's phases.
   void kernel_times(double ms[4]);
 
+  // Takes effect at the next load / load_batch.
+  void set_tuning(const Tuning& t) { tuning_ = t; }
+  const Tuning& tuning() const { return tuning_; }
   int device() const { return device_; }
   size_t arena_bytes() const;
   // Duration (ms, HIP events) of the basis evaluation at the last load.
@@ -162,6 +187,9 @@ class Engine {
   void* stream_ = nullptr;
   DeviceGrid* grid_ = nullptr;
   void* events_[8] = {};
+  void* graph_ = nullptr;
+  Tuning tuning_;
+  void drop_graph();
 };
 
 }  // namespace xcq

@@ -8,6 +8,8 @@ Two ways in, mirroring how the library is used:
   owning every array, energy assigned and the packed Fock accumulated.
 * :class:`Context` uses the handle API (``xcq_*``) for several molecules at once
   and for timing with device-resident buffers.
+* :class:`Batch` keeps many molecules resident together and builds all their
+  XC Fock matrices with one launch per phase (optionally one hipGraph).
 
 The shared library must be built (``__graft_entry__.build()`` or
 ``make -C lio_amd/csrc``); there is no fallback implementation.
@@ -86,6 +88,21 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_stream.restype = vp
     L.xcq_host_partition.argtypes = [C.POINTER(XcqProblem), C.c_uint, C.POINTER(XcqGroupInfo),
                                      C.c_int, C.POINTER(C.c_int)]
+    L.xcq_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(XcqProblem), _up, C.c_int,
+                                   C.POINTER(vp)]
+    L.xcq_batch_destroy.argtypes = [vp]
+    L.xcq_batch_offsets.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.xcq_batch_solve_device.argtypes = [vp, vp, vp]
+    L.xcq_batch_capture.argtypes = [vp, vp, vp]
+    L.xcq_batch_replay.argtypes = [vp]
+    L.xcq_batch_energies.argtypes = [vp, _dp, _dp]
+    L.xcq_batch_kernel_times.argtypes = [vp, _dp]
+    L.xcq_batch_stream.argtypes = [vp]
+    L.xcq_batch_stream.restype = vp
+    L.xcq_batch_arena_bytes.argtypes = [vp]
+    L.xcq_batch_arena_bytes.restype = C.c_longlong
+    L.xcq_batch_tune.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.c_int, _dp,
+                                 C.POINTER(C.c_int)]
     _LIB = L
     return L
 
@@ -312,3 +329,94 @@ class G2G:
 
     def deinit(self):
         self.L.g2g_deinit_()
+
+
+class Batch:
+    """Several molecules resident on one device (``xcq_batch_*``).
+
+    Packed densities and Fock matrices are concatenated in molecule order;
+    ``offsets[k]`` is the first packed entry of molecule k and ``offsets[-1]``
+    the total.  Per-molecule results are bitwise those of the molecule alone.
+    """
+
+    def __init__(self, systems, device: int = -1, precision: str = "fp64", **options):
+        self.L = load_library()
+        self.systems = list(systems)
+        n = len(self.systems)
+        probs = (XcqProblem * n)()
+        self._keep = []
+        for k, s in enumerate(self.systems):
+            p, keep = make_problem(s, **options)
+            probs[k] = p
+            self._keep.append(keep)
+        gts = np.array([s.grid_type for s in self.systems], np.uint32)
+        h = C.c_void_p()
+        _check(self.L.xcq_batch_create(device, n, probs, _ptr(gts, C.c_uint), PRECISION[precision],
+                                       C.byref(h)), "xcq_batch_create")
+        self.h = h
+        self.n = n
+        self.precision = precision
+        off = np.zeros(n + 1, np.int64)
+        _check(self.L.xcq_batch_offsets(self.h, off.ctypes.data_as(C.POINTER(C.c_longlong))),
+               "xcq_batch_offsets")
+        self.offsets = off
+
+    def packed_density(self):
+        """The concatenated packed densities of the systems (host array)."""
+        return np.concatenate([s.rho_packed for s in self.systems])
+
+    def fock_of(self, F, k: int):
+        """Molecule k's packed Fock out of a concatenated array."""
+        return F[self.offsets[k]:self.offsets[k + 1]]
+
+    def solve_device(self, d_rho: int, d_fock: int):
+        _check(self.L.xcq_batch_solve_device(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),
+               "xcq_batch_solve_device")
+
+    def capture(self, d_rho: int, d_fock: int):
+        _check(self.L.xcq_batch_capture(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),
+               "xcq_batch_capture")
+
+    def replay(self):
+        _check(self.L.xcq_batch_replay(self.h), "xcq_batch_replay")
+
+    def energies(self):
+        """Waits; (Exc, N) arrays of all molecules."""
+        e, n = np.zeros(self.n), np.zeros(self.n)
+        _check(self.L.xcq_batch_energies(self.h, _ptr(e, C.c_double), _ptr(n, C.c_double)),
+               "xcq_batch_energies")
+        return e, n
+
+    def kernel_times(self):
+        out = np.zeros(4)
+        _check(self.L.xcq_batch_kernel_times(self.h, _ptr(out, C.c_double)),
+               "xcq_batch_kernel_times")
+        return out
+
+    def stream(self) -> int:
+        return self.L.xcq_batch_stream(self.h) or 0
+
+    def arena_bytes(self) -> int:
+        return int(self.L.xcq_batch_arena_bytes(self.h))
+
+    def tune(self, d_rho: int, d_fock: int, slices=(256, 512, 1024, 2048), reps: int = 5):
+        """Autotunes the Fock slice length on these buffers; returns
+        (best slice, {slice: ms per build})."""
+        sl = np.array(slices, np.int32)
+        ms = np.zeros(len(sl))
+        best = C.c_int(-1)
+        _check(self.L.xcq_batch_tune(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock), reps,
+                                     sl.ctypes.data_as(C.POINTER(C.c_int)), len(sl),
+                                     _ptr(ms, C.c_double), C.byref(best)), "xcq_batch_tune")
+        return int(sl[best.value]), dict(zip(sl.tolist(), ms.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xcq_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

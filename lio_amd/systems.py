@@ -522,3 +522,14 @@ def water_cluster(n: int, seed: int) -> System:
 def random_molecule(natoms: int, seed: int) -> System:
     Z, p = random_molecule_ang(natoms, seed)
     return make_system("mol%d_s%d" % (natoms, seed), Z, p, seed=seed)
+
+
+def molecule_batch(n: int = 64, seed: int = 3, atoms=(3, 60)) -> list[System]:
+    """n independent random molecules (C:H:N:O ~ 0.35:0.45:0.10:0.10, bonded
+    spacing 1.0-1.6 A, minimum separation 0.9 A, even electron count, 6-31G**,
+    40 x 194 grid, PBE), each with its own seeded density: the atom counts are
+    drawn uniformly from ``atoms`` by a generator seeded with ``seed`` and
+    molecule k is grown with seed 1000 * seed + k."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(atoms[0], atoms[1] + 1, size=n)
+    return [random_molecule(int(sizes[k]), 1000 * seed + k) for k in range(n)]
