@@ -97,6 +97,18 @@ int xcq_set_precision(xcq_context* ctx, int precision);
 
 /* Builds grid and partition for grid_type (0/1/2) and uploads them. */
 int xcq_build_grid(xcq_context* ctx, unsigned grid_type);
+/* Where the group-local Becke weights and the zero-weight removal run at the
+ * next xcq_build_grid: XCQ_GRID_HOST (default; OpenMP, as g2g/cpu/weight.cpp)
+ * or XCQ_GRID_DEVICE (HIP kernels; points generated, assigned to cubes and
+ * spheres and screened on the host, weighted and compacted on the GPU).  The
+ * groups and weights are the same.  No reference counterpart (LIO's GPU
+ * build weights points on the GPU unconditionally). */
+enum { XCQ_GRID_HOST = 0, XCQ_GRID_DEVICE = 1 };
+int xcq_set_grid_mode(xcq_context* ctx, int mode);
+/* Frees the context's device memory (grid, values, work buffers); the
+ * problem stays set and the next xcq_build_grid restores it.  No reference
+ * counterpart: lets a server hold more molecules than fit in HBM at once. */
+int xcq_release_device(xcq_context* ctx);
 int xcq_query_grid(xcq_context* ctx, xcq_grid_info* out);
 /* Group inventory of the built grid; returns the count through *ngroups. */
 int xcq_groups(xcq_context* ctx, xcq_group_info* out, int max, int* ngroups);
@@ -108,6 +120,15 @@ int xcq_solve(xcq_context* ctx, const double* rho_packed, double* fock_accum,
  * (xcq_stream); energy and electrons are read back by xcq_sync_energy. */
 int xcq_solve_device(xcq_context* ctx, const double* d_rho_packed, double* d_fock);
 int xcq_sync_energy(xcq_context* ctx, double* energy, double* electrons);
+/* Asynchronous build from a packed host density (copied at once): queued on
+ * the context stream with the copy-back of F, Exc and N.  xcq_query returns
+ * 1 when it has finished, 0 while it runs (negative on error); xcq_collect
+ * waits for it, adds F onto fock_accum (may be NULL) and fills info.  No
+ * reference counterpart: lets one host thread keep many molecules' builds in
+ * flight on their own streams. */
+int xcq_solve_async(xcq_context* ctx, const double* rho_packed);
+int xcq_query(xcq_context* ctx);
+int xcq_collect(xcq_context* ctx, double* fock_accum, xcq_solve_info* info);
 /* Waits for the context stream and returns the HIP-event durations (ms) of
  * the last solve's phases: [0] density expansion, [1] density + functional,
  * [2] Fock tiles + slice sums, [3] Fock packing + energy reduction. */
@@ -132,6 +153,10 @@ int xcq_host_partition(const xcq_problem* problem, unsigned grid_type,
 typedef struct xcq_batch xcq_batch;
 int xcq_batch_create(int device, int n, const xcq_problem* problems,
                      const unsigned* grid_types, int precision, xcq_batch** out);
+/* Same with the grid weighting on the host or the device (XCQ_GRID_*). */
+int xcq_batch_create_ex(int device, int n, const xcq_problem* problems,
+                        const unsigned* grid_types, int precision, int grid_mode,
+                        xcq_batch** out);
 int xcq_batch_destroy(xcq_batch* b);
 int xcq_batch_offsets(xcq_batch* b, long long* offsets);
 /* Queues one build of every molecule on the batch stream (device buffers). */

@@ -33,6 +33,8 @@ struct xcq_context {
   double flops = 0.0;
   int density_tiles = 0, fock_items = 0;
   xcq::Precision prec = xcq::Precision::FP64;
+  bool device_grid = false;  // Becke weights and zero-weight removal on the device
+  bool points_on_host = true;
 };
 
 namespace {
@@ -118,11 +120,44 @@ void build(xcq_context* ctx, unsigned grid_type) {
   if (!ctx->have_problem) throw std::runtime_error("g2g: grid requested before parameter init");
   check_problem(ctx->pv);
   ctx->mol = xcq::capture(ctx->pv, grid_type);
-  ctx->part = xcq::build_partition(ctx->mol, ctx->opt);
-  const double t0 = wall();
-  ctx->engine->load(ctx->mol, ctx->part, ctx->prec);
-  ctx->upload_seconds = wall() - t0;
+  if (ctx->device_grid) {
+    const xcq::StagedPartition st = xcq::stage_partition(ctx->mol, ctx->opt);
+    const double t0 = wall();
+    ctx->part = ctx->engine->load_staged({&ctx->mol}, {&st}, ctx->opt, ctx->prec)[0];
+    ctx->upload_seconds = wall() - t0;
+    ctx->points_on_host = false;
+  } else {
+    ctx->part = xcq::build_partition(ctx->mol, ctx->opt);
+    const double t0 = wall();
+    ctx->engine->load(ctx->mol, ctx->part, ctx->prec);
+    ctx->upload_seconds = wall() - t0;
+    ctx->points_on_host = true;
+  }
   ctx->grid_type = (int)grid_type;
+}
+
+// Device-built grids keep their points on the device: copy them into the
+// host partition (unpadded) when a caller asks for per-group sums.
+void ensure_points(xcq_context* ctx) {
+  if (ctx->points_on_host) return;
+  xcq::GridPartition& part = ctx->part;
+  const long long np = ctx->engine->padded_points();
+  std::vector<double> x(np), y(np), z(np), w(np);
+  ctx->engine->download_points(x.data(), y.data(), z.data(), w.data());
+  part.x.clear();
+  part.y.clear();
+  part.z.clear();
+  part.w.clear();
+  long long off = 0;
+  for (int g = 0; g < part.ngroups; ++g) {
+    const int n = part.npts[g];
+    part.x.insert(part.x.end(), x.begin() + off, x.begin() + off + n);
+    part.y.insert(part.y.end(), y.begin() + off, y.begin() + off + n);
+    part.z.insert(part.z.end(), z.begin() + off, z.begin() + off + n);
+    part.w.insert(part.w.end(), w.begin() + off, w.begin() + off + n);
+    off += (n + 15) / 16 * 16;
+  }
+  ctx->points_on_host = true;
 }
 
 }  // namespace
@@ -162,6 +197,10 @@ int xcq_set_precision(xcq_context* ctx, int precision) {
       throw std::runtime_error("g2g: unknown precision mode");
     ctx->prec = static_cast<xcq::Precision>(precision);
     if (ctx->grid_type >= 0) {
+      if (!ctx->points_on_host) {  // device-built grid: rebuild it in the new mode
+        build(ctx, (unsigned)ctx->grid_type);
+        return;
+      }
       const double t0 = wall();
       ctx->engine->load(ctx->mol, ctx->part, ctx->prec);
       ctx->upload_seconds = wall() - t0;
@@ -171,6 +210,21 @@ int xcq_set_precision(xcq_context* ctx, int precision) {
 
 int xcq_build_grid(xcq_context* ctx, unsigned grid_type) {
   return guarded([&] { build(ctx, grid_type); });
+}
+
+int xcq_release_device(xcq_context* ctx) {
+  return guarded([&] {
+    ctx->engine->unload();
+    ctx->grid_type = -1;
+  });
+}
+
+int xcq_set_grid_mode(xcq_context* ctx, int mode) {
+  return guarded([&] {
+    if (mode != XCQ_GRID_HOST && mode != XCQ_GRID_DEVICE)
+      throw std::runtime_error("g2g: unknown grid mode");
+    ctx->device_grid = mode == XCQ_GRID_DEVICE;
+  });
 }
 
 int xcq_query_grid(xcq_context* ctx, xcq_grid_info* o) {
@@ -201,6 +255,7 @@ int xcq_query_grid(xcq_context* ctx, xcq_grid_info* o) {
 int xcq_groups(xcq_context* ctx, xcq_group_info* out, int max, int* n) {
   return guarded([&] {
     if (ctx->grid_type < 0) throw std::runtime_error("g2g: no grid built");
+    if (out) ensure_points(ctx);
     if (out) fill_groups(ctx->part, out, max);
     if (n) *n = ctx->part.ngroups;
   });
@@ -223,6 +278,31 @@ int xcq_solve_device(xcq_context* ctx, const double* d_rho, double* d_fock) {
   return guarded([&] {
     if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
     ctx->engine->solve_device(d_rho, d_fock, true);
+  });
+}
+
+int xcq_solve_async(xcq_context* ctx, const double* rho) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
+    ctx->engine->solve_async(rho);
+  });
+}
+
+int xcq_query(xcq_context* ctx) {
+  int r = -1;
+  const int rc = guarded([&] { r = ctx->engine->idle() ? 1 : 0; });
+  return rc != 0 ? rc : r;
+}
+
+int xcq_collect(xcq_context* ctx, double* fock_accum, xcq_solve_info* info) {
+  return guarded([&] {
+    xcq::SolveResult r = ctx->engine->collect(fock_accum);
+    if (info) {
+      info->energy = r.energy;
+      info->electrons = r.electrons;
+      info->seconds = 0.0;
+      for (int k = 0; k < 4; ++k) info->kernel_ms[k] = 0.0;
+    }
   });
 }
 
@@ -260,6 +340,9 @@ struct xcq_batch {
   std::unique_ptr<xcq::Engine> engine;
   std::vector<xcq::Molecule> mols;
   std::vector<xcq::GridPartition> parts;
+  std::vector<xcq::StagedPartition> staged;  // device-grid mode
+  xcq::Options opt;                           // device-grid mode
+  bool device_grid = false;
   xcq::Precision prec = xcq::Precision::FP64;
 };
 
@@ -267,11 +350,15 @@ namespace {
 
 void load_batch(xcq_batch* b) {
   std::vector<const xcq::Molecule*> m;
-  std::vector<const xcq::GridPartition*> p;
-  for (size_t k = 0; k < b->mols.size(); ++k) {
-    m.push_back(&b->mols[k]);
-    p.push_back(&b->parts[k]);
+  for (size_t k = 0; k < b->mols.size(); ++k) m.push_back(&b->mols[k]);
+  if (b->device_grid) {
+    std::vector<const xcq::StagedPartition*> st;
+    for (size_t k = 0; k < b->staged.size(); ++k) st.push_back(&b->staged[k]);
+    b->parts = b->engine->load_staged(m, st, b->opt, b->prec);
+    return;
   }
+  std::vector<const xcq::GridPartition*> p;
+  for (size_t k = 0; k < b->parts.size(); ++k) p.push_back(&b->parts[k]);
   b->engine->load_batch(m, p, b->prec);
 }
 
@@ -281,7 +368,15 @@ extern "C" {
 
 int xcq_batch_create(int device, int n, const xcq_problem* problems, const unsigned* grid_types,
                      int precision, xcq_batch** out) {
+  return xcq_batch_create_ex(device, n, problems, grid_types, precision, XCQ_GRID_HOST, out);
+}
+
+int xcq_batch_create_ex(int device, int n, const xcq_problem* problems,
+                        const unsigned* grid_types, int precision, int grid_mode,
+                        xcq_batch** out) {
   return guarded([&] {
+    if (grid_mode != XCQ_GRID_HOST && grid_mode != XCQ_GRID_DEVICE)
+      throw std::runtime_error("g2g: unknown grid mode");
     if (n <= 0 || !problems || !grid_types) throw std::runtime_error("g2g: empty batch");
     if (precision != XCQ_FP64 && precision != XCQ_FP32 && precision != XCQ_BF16X6)
       throw std::runtime_error("g2g: unknown precision mode");
@@ -290,13 +385,25 @@ int xcq_batch_create(int device, int n, const xcq_problem* problems, const unsig
     b->engine.reset(new xcq::Engine(device));
     b->mols.resize(n);
     b->parts.resize(n);
+    b->device_grid = grid_mode == XCQ_GRID_DEVICE;
+    std::vector<xcq::StagedPartition>& staged = b->staged;
+    staged.resize(b->device_grid ? n : 0);
+    xcq::Options& opt0 = b->opt;
     for (int k = 0; k < n; ++k) {
       xcq::ProblemView pv;
       xcq::Options opt;
       view_from_problem(&problems[k], pv, opt);
       check_problem(pv);
+      if (k == 0) opt0 = opt;
       b->mols[k] = xcq::capture(pv, grid_types[k]);
-      b->parts[k] = xcq::build_partition(b->mols[k], opt);
+      if (grid_mode == XCQ_GRID_DEVICE) {
+        if (opt.remove_zero_weights != opt0.remove_zero_weights ||
+            opt.min_points_per_cube != opt0.min_points_per_cube)
+          throw std::runtime_error("g2g: a device-grid batch needs one weight-removal setting");
+        staged[k] = xcq::stage_partition(b->mols[k], opt);
+      } else {
+        b->parts[k] = xcq::build_partition(b->mols[k], opt);
+      }
     }
     load_batch(b.get());
     *out = b.release();
@@ -436,6 +543,7 @@ void g2g_init_(void) {
       g_ctx->engine.reset(new xcq::Engine(-1));
     }
     g_ctx->prec = precision_from_env();
+    if (const char* v = std::getenv("LIO_XC_DEVICE_GRID")) g_ctx->device_grid = v[0] == '1';
     g_ctx->opt = g_opt;
     if (g_opt.verbose > 2) std::printf("  Using HIP device %d for the XC quadrature.\n", g_ctx->engine->device());
   });

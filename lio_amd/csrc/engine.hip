@@ -920,6 +920,168 @@ __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epar
   }
 }
 
+// ---------------------------------------------------------------------------
+// device grid rebuild: group-local Becke weights and zero-weight removal
+// ---------------------------------------------------------------------------
+// A staged group (before weights) and its molecule's atom tables.
+struct StGroup {
+  int pt_off, npts;    // staged points
+  int atom_off, natoms;  // nuclei (molecule-local, ascending) in gatoms
+  int mol, pad_;
+};
+struct StMol {
+  int atom_base, natom;  // atoms in the batch-wide atom arrays
+  long long rij_off;     // natom x natom distances
+};
+
+// Becke cell function with the atom-size adjustment (g2g/cpu/weight.cpp:
+// 13-112, the same expression order as the host partition).
+__device__ inline double becke_cell(double dj, double dk, double rjk, double rmj, double rmk) {
+#pragma clang fp contract(off)
+  double u = (dj - dk) / rjk;
+  double chi = rmj / rmk;
+  chi = (chi - 1.0) / (chi + 1.0);
+  u += (chi / (chi * chi - 1.0)) * (1.0 - u * u);
+  u = 1.5 * u - 0.5 * (u * u * u);
+  u = 1.5 * u - 0.5 * (u * u * u);
+  u = 1.5 * u - 0.5 * (u * u * u);
+  return 0.5 * (1.0 - u);
+}
+
+__device__ inline double dist3(double x, double y, double z, double ax, double ay, double az) {
+#pragma clang fp contract(off)
+  const double dx = x - ax, dy = y - ay, dz = z - az;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// One staged point per thread: weight *= P_own / sum_j P_j over the group's
+// nuclei, the own atom's cell taken separately when it is not among them.
+// Distances are recomputed instead of stored per point (the host keeps them
+// in an array), with identical values.
+__global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ groups,
+                                               const int* __restrict__ pgroup,
+                                               const StMol* __restrict__ mols,
+                                               const int* __restrict__ gatoms,
+                                               const int* __restrict__ owner,
+                                               const double* __restrict__ ax,
+                                               const double* __restrict__ ay,
+                                               const double* __restrict__ az,
+                                               const double* __restrict__ arm,
+                                               const double* __restrict__ rij,
+                                               const double* __restrict__ sx,
+                                               const double* __restrict__ sy,
+                                               const double* __restrict__ sz,
+                                               double* __restrict__ sw, int npoints) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= npoints) return;
+  const StGroup G = groups[pgroup[p]];
+  const StMol m = mols[G.mol];
+  const int* at = gatoms + G.atom_off;
+  const int k = G.natoms, own = owner[p], na = m.natom;
+  const double* A = ax + m.atom_base;
+  const double* B = ay + m.atom_base;
+  const double* C = az + m.atom_base;
+  const double* R = arm + m.atom_base;
+  const double* D = rij + m.rij_off;
+  const double x = sx[p], y = sy[p], z = sz[p];
+  bool own_in = false;
+  double total = 0.0, mine = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const int aj = at[j];
+    own_in |= (aj == own);
+    const double dj = dist3(x, y, z, A[aj], B[aj], C[aj]);
+    double prod = 1.0;
+    for (int q = 0; q < k; ++q) {
+      const int ak = at[q];
+      if (ak == aj) continue;
+      prod *= becke_cell(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), D[(long long)aj * na + ak], R[aj],
+                         R[ak]);
+      if (prod == 0.0) break;
+    }
+    if (aj == own) {
+      mine = prod;
+      if (mine == 0.0) break;
+    }
+    total += prod;
+  }
+  if (!own_in) {  // (the j loop only breaks at the own atom, so own_in is complete)
+    const double down = dist3(x, y, z, A[own], B[own], C[own]);
+    mine = 1.0;
+    for (int q = 0; q < k; ++q) {
+      const int ak = at[q];
+      mine *= becke_cell(down, dist3(x, y, z, A[ak], B[ak], C[ak]), D[(long long)own * na + ak],
+                         R[own], R[ak]);
+      if (mine == 0.0) break;
+    }
+  }
+  sw[p] *= (total == 0.0 ? 0.0 : mine / total);
+}
+
+// Points a staged group keeps (all, or those of non-zero weight).
+__global__ __launch_bounds__(256) void k_group_count(const StGroup* __restrict__ groups,
+                                                     const double* __restrict__ sw, int drop_zero,
+                                                     int* __restrict__ count) {
+  __shared__ int sc[4];
+  const StGroup G = groups[blockIdx.x];
+  int c = 0;
+  for (int i = threadIdx.x; i < G.npts; i += 256) c += (!drop_zero || sw[G.pt_off + i] != 0.0);
+  for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) count[blockIdx.x] = sc[0] + sc[1] + sc[2] + sc[3];
+}
+
+// Final group f takes the kept points of staged group src[f] in staged
+// order into [pt_off, pt_off + n), then pads to npts_pad with copies of its
+// first point of zero weight (the host layout).
+struct CompactItem {
+  int src, pt_off, n, npts_pad;
+};
+__global__ __launch_bounds__(256) void k_compact(const CompactItem* __restrict__ items,
+                                                 const StGroup* __restrict__ groups,
+                                                 const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ sz,
+                                                 const double* __restrict__ sw, int drop_zero,
+                                                 double* __restrict__ px, double* __restrict__ py,
+                                                 double* __restrict__ pz, double* __restrict__ pw) {
+  __shared__ int swave[4];
+  __shared__ int sfirst;
+  const CompactItem it = items[blockIdx.x];
+  const StGroup G = groups[it.src];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) sfirst = -1;
+  int base = 0;
+  for (int c0 = 0; c0 < G.npts; c0 += 256) {
+    const int i = c0 + threadIdx.x;
+    const bool keep = i < G.npts && (!drop_zero || sw[G.pt_off + i] != 0.0);
+    const unsigned long long mask = __ballot(keep);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) swave[wave] = __popcll(mask);
+    __syncthreads();
+    int before = base;
+    for (int w = 0; w < wave; ++w) before += swave[w];
+    if (keep) {
+      const int q = it.pt_off + before + rank;
+      px[q] = sx[G.pt_off + i];
+      py[q] = sy[G.pt_off + i];
+      pz[q] = sz[G.pt_off + i];
+      pw[q] = sw[G.pt_off + i];
+      if (before + rank == 0) sfirst = i;
+    }
+    base += swave[0] + swave[1] + swave[2] + swave[3];
+    __syncthreads();
+  }
+  const int f = sfirst;
+  for (int q = it.n + threadIdx.x; q < it.npts_pad; q += 256) {
+    px[it.pt_off + q] = sx[G.pt_off + f];
+    py[it.pt_off + q] = sy[G.pt_off + f];
+    pz[it.pt_off + q] = sz[G.pt_off + f];
+    pw[it.pt_off + q] = 0.0;
+  }
+}
+
 double wall() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -1035,6 +1197,20 @@ void Engine::load(const Molecule& mol, const GridPartition& part, Precision prec
 
 void Engine::load_batch(const std::vector<const Molecule*>& mols,
                         const std::vector<const GridPartition*>& parts, Precision prec) {
+  load_batch_impl(mols, parts, prec, nullptr);
+}
+
+// Staged points on the device after the Becke weights (load_staged).
+struct DevicePoints {
+  const StGroup* groups;  // staged groups
+  const double *x, *y, *z, *w;
+  int drop_zero;
+  std::vector<int> src;   // staged group of each final group (batch order)
+};
+
+void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
+                             const std::vector<const GridPartition*>& parts, Precision prec,
+                             const DevicePoints* dpts) {
   XCQ_HIP(hipSetDevice(device_));
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
   drop_graph();
@@ -1137,7 +1313,7 @@ void Engine::load_batch(const std::vector<const Molecule*>& mols,
       fnl.push_back(f);
       if (f >= 0) membership[f].push_back(make_int2(g, i));
     }
-    for (int p = 0; p < d.npts_pad; ++p) {
+    for (int p = 0; !dpts && p < d.npts_pad; ++p) {
       const int src = part.pt_off[gk] + (p < n ? p : 0);  // padding repeats point 0 with w = 0
       hx.push_back(part.x[src]);
       hy.push_back(part.y[src]);
@@ -1277,6 +1453,20 @@ void Engine::load_batch(const std::vector<const Molecule*>& mols,
   up(dg->mols, hmol.data(), hmol.size() * sizeof(DevMol));
   up(dg->fn_mol, fn_mol.data(), fn_mol.size() * sizeof(int));
   up(dg->eidx, eidx.data(), eidx.size() * sizeof(int));
+  if (dpts) {  // points come from the device rebuild: compact them into place
+    if ((int)dpts->src.size() != G) throw std::runtime_error("g2g: staged group map mismatch");
+    std::vector<CompactItem> ci(G);
+    for (int g = 0; g < G; ++g)
+      ci[g] = CompactItem{dpts->src[g], groups[g].pt_off, groups[g].npts, groups[g].npts_pad};
+    CompactItem* dci = nullptr;
+    XCQ_HIP(hipMallocAsync((void**)&dci, std::max<size_t>(1, G) * sizeof(CompactItem), s));
+    up(dci, ci.data(), ci.size() * sizeof(CompactItem));
+    if (G > 0)
+      hipLaunchKernelGGL(k_compact, dim3(G), dim3(256), 0, s, dci, dpts->groups, dpts->x, dpts->y,
+                         dpts->z, dpts->w, dpts->drop_zero, dg->px, dg->py, dg->pz, dg->pw);
+    XCQ_HIP(hipGetLastError());
+    XCQ_HIP(hipFreeAsync(dci, s));
+  }
   // basis values of every group, kept for all solves on this grid
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
   XCQ_HIP(hipEventRecord(ev[6], s));
@@ -1320,6 +1510,173 @@ void Engine::load_batch(const std::vector<const Molecule*>& mols,
   dg->eval_ms = ms;
   grid_ = dg;
 }
+
+std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*>& mols,
+                                              const std::vector<const StagedPartition*>& staged,
+                                              const Options& opt, Precision prec) {
+  XCQ_HIP(hipSetDevice(device_));
+  const int nmol = (int)mols.size();
+  if (nmol == 0 || staged.size() != mols.size())
+    throw std::runtime_error("g2g: a staged batch needs one staging per molecule");
+  // batch-wide staging tables
+  std::vector<StGroup> sg;
+  std::vector<StMol> sm(nmol);
+  std::vector<int> pgroup, owner, gatoms;
+  std::vector<double> sx, sy, sz, sw, ax, ay, az, arm, rij;
+  for (int k = 0; k < nmol; ++k) {
+    const Molecule& mol = *mols[k];
+    const StagedPartition& st = *staged[k];
+    sm[k].atom_base = (int)ax.size();
+    sm[k].natom = mol.natom;
+    sm[k].rij_off = (long long)rij.size();
+    ax.insert(ax.end(), mol.x.begin(), mol.x.end());
+    ay.insert(ay.end(), mol.y.begin(), mol.y.end());
+    az.insert(az.end(), mol.z.begin(), mol.z.end());
+    arm.insert(arm.end(), mol.rm.begin(), mol.rm.end());
+    rij.insert(rij.end(), st.rij.begin(), st.rij.end());
+    const GridPartition& gp = st.groups;
+    for (int g = 0; g < gp.ngroups; ++g) {
+      StGroup G;
+      G.pt_off = (int)sx.size();
+      G.npts = gp.npts[g];
+      G.atom_off = (int)gatoms.size();
+      G.natoms = st.atom_off[g + 1] - st.atom_off[g];
+      G.mol = k;
+      G.pad_ = 0;
+      gatoms.insert(gatoms.end(), st.atoms.begin() + st.atom_off[g],
+                    st.atoms.begin() + st.atom_off[g + 1]);
+      const int b = gp.pt_off[g];
+      sx.insert(sx.end(), gp.x.begin() + b, gp.x.begin() + b + G.npts);
+      sy.insert(sy.end(), gp.y.begin() + b, gp.y.begin() + b + G.npts);
+      sz.insert(sz.end(), gp.z.begin() + b, gp.z.begin() + b + G.npts);
+      sw.insert(sw.end(), gp.w.begin() + b, gp.w.begin() + b + G.npts);
+      owner.insert(owner.end(), st.owner.begin() + b, st.owner.begin() + b + G.npts);
+      pgroup.insert(pgroup.end(), G.npts, (int)sg.size());
+      sg.push_back(G);
+    }
+  }
+  const int nst = (int)sg.size();
+  const long long npts = (long long)sx.size();
+  if (npts > (1LL << 31) - 256) throw std::runtime_error("g2g: too many staged points");
+  // one staging allocation, freed once the points are compacted into the arena
+  struct Stage {
+    StGroup* groups;
+    StMol* mols;
+    int *pgroup, *owner, *gatoms, *count;
+    double *ax, *ay, *az, *arm, *rij, *x, *y, *z, *w;
+  } d{};
+  Carver plan{nullptr};
+  auto carve = [&](Carver& c) {
+    d.groups = c.take<StGroup>(nst + 1);
+    d.mols = c.take<StMol>(nmol);
+    d.pgroup = c.take<int>(npts + 1);
+    d.owner = c.take<int>(npts + 1);
+    d.gatoms = c.take<int>(gatoms.size() + 1);
+    d.count = c.take<int>(nst + 1);
+    d.ax = c.take<double>(ax.size());
+    d.ay = c.take<double>(ay.size());
+    d.az = c.take<double>(az.size());
+    d.arm = c.take<double>(arm.size());
+    d.rij = c.take<double>(rij.size());
+    d.x = c.take<double>(npts);
+    d.y = c.take<double>(npts);
+    d.z = c.take<double>(npts);
+    d.w = c.take<double>(npts);
+  };
+  carve(plan);
+  char* base = nullptr;
+  XCQ_HIP(hipMalloc(&base, plan.used + 256));
+  struct Free {
+    char* p;
+    ~Free() {
+      if (p) (void)hipFree(p);
+    }
+  } guard{base};
+  Carver c{base};
+  carve(c);
+  hipStream_t s = (hipStream_t)stream_;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) XCQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  up(d.groups, sg.data(), sg.size() * sizeof(StGroup));
+  up(d.mols, sm.data(), sm.size() * sizeof(StMol));
+  up(d.pgroup, pgroup.data(), pgroup.size() * sizeof(int));
+  up(d.owner, owner.data(), owner.size() * sizeof(int));
+  up(d.gatoms, gatoms.data(), gatoms.size() * sizeof(int));
+  up(d.ax, ax.data(), ax.size() * sizeof(double));
+  up(d.ay, ay.data(), ay.size() * sizeof(double));
+  up(d.az, az.data(), az.size() * sizeof(double));
+  up(d.arm, arm.data(), arm.size() * sizeof(double));
+  up(d.rij, rij.data(), rij.size() * sizeof(double));
+  up(d.x, sx.data(), npts * sizeof(double));
+  up(d.y, sy.data(), npts * sizeof(double));
+  up(d.z, sz.data(), npts * sizeof(double));
+  up(d.w, sw.data(), npts * sizeof(double));
+  if (npts > 0)
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, d.groups,
+                       d.pgroup, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.arm, d.rij, d.x,
+                       d.y, d.z, d.w, (int)npts);
+  const int drop = opt.remove_zero_weights ? 1 : 0;
+  if (nst > 0)
+    hipLaunchKernelGGL(k_group_count, dim3(nst), dim3(256), 0, s, d.groups, d.w, drop, d.count);
+  XCQ_HIP(hipGetLastError());
+  std::vector<int> count(nst);
+  if (nst > 0) XCQ_HIP(hipMemcpyAsync(count.data(), d.count, nst * sizeof(int),
+                                      hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipStreamSynchronize(s));
+  // the final partitions: groups keeping at least min_points_per_cube points
+  std::vector<GridPartition> parts(nmol);
+  DevicePoints dp;
+  dp.groups = d.groups;
+  dp.x = d.x;
+  dp.y = d.y;
+  dp.z = d.z;
+  dp.w = d.w;
+  dp.drop_zero = drop;
+  for (int k = 0, gs = 0; k < nmol; ++k) {
+    const GridPartition& in = staged[k]->groups;
+    GridPartition& out = parts[k];
+    int pts = 0;
+    for (int g = 0; g < in.ngroups; ++g, ++gs) {
+      if ((unsigned)count[gs] < opt.min_points_per_cube) continue;
+      out.ngroups++;
+      out.kind.push_back(in.kind[g]);
+      out.npts.push_back(count[gs]);
+      out.pt_off.push_back(pts);
+      pts += count[gs];
+      out.fn_off.push_back((int)out.fn.size());
+      out.fn.insert(out.fn.end(), in.fn.begin() + in.fn_off[g],
+                    in.fn.begin() + in.fn_off[g] + in.m[g]);
+      out.m.push_back(in.m[g]);
+      out.nshells.push_back(in.nshells[g]);
+      out.ns.push_back(in.ns[g]);
+      out.np.push_back(in.np[g]);
+      out.nd.push_back(in.nd[g]);
+      out.nnuc.push_back(in.nnuc[g]);
+      out.fhash.push_back(in.fhash[g]);
+      dp.src.push_back(gs);
+    }
+    out.seconds = in.seconds;
+  }
+  std::vector<const GridPartition*> pp;
+  for (auto& p : parts) pp.push_back(&p);
+  load_batch_impl(mols, pp, prec, &dp);
+  return parts;
+}
+
+void Engine::download_points(double* x, double* y, double* z, double* w) {
+  if (!grid_) throw std::runtime_error("g2g: no grid");
+  const DeviceGrid& g = *grid_;
+  hipStream_t s = (hipStream_t)stream_;
+  const size_t b = (size_t)g.npts_pad * sizeof(double);
+  if (x) XCQ_HIP(hipMemcpyAsync(x, g.px, b, hipMemcpyDeviceToHost, s));
+  if (y) XCQ_HIP(hipMemcpyAsync(y, g.py, b, hipMemcpyDeviceToHost, s));
+  if (z) XCQ_HIP(hipMemcpyAsync(z, g.pz, b, hipMemcpyDeviceToHost, s));
+  if (w) XCQ_HIP(hipMemcpyAsync(w, g.pw, b, hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipStreamSynchronize(s));
+}
+
+long long Engine::padded_points() const { return grid_ ? grid_->npts_pad : 0; }
 
 // Phase boundaries are marked with ev[k] when ev is given (not while a graph
 // is being captured).
@@ -1433,6 +1790,14 @@ void Engine::capture_graph(const double* d_rho, double* d_fock) {
   graph_ = exec;
 }
 
+void Engine::unload() {
+  XCQ_HIP(hipSetDevice(device_));
+  XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
+  drop_graph();
+  release(grid_);
+  grid_ = nullptr;
+}
+
 void Engine::replay_graph() {
   if (!graph_) throw std::runtime_error("g2g: no captured graph");
   XCQ_HIP(hipGraphLaunch((hipGraphExec_t)graph_, (hipStream_t)stream_));
@@ -1444,8 +1809,8 @@ void Engine::drop_graph() {
 }
 
 void Engine::read_energy(double* energy, double* electrons) {
-  double e[2];
-  read_energies(e);
+  std::vector<double> e(2 * (size_t)std::max(1, molecules()));
+  read_energies(e.data());
   if (energy) *energy = e[0];
   if (electrons) *electrons = e[1];
 }
@@ -1474,6 +1839,38 @@ void Engine::kernel_times(double ms[4]) {
     XCQ_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
     ms[k] = t;
   }
+}
+
+void Engine::solve_async(const double* rho_packed) {
+  if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  DeviceGrid& g = *grid_;
+  hipStream_t s = (hipStream_t)stream_;
+  XCQ_HIP(hipSetDevice(device_));
+  const size_t packed = (size_t)g.packed;
+  std::memcpy(g.h_rho, rho_packed, packed * sizeof(double));
+  XCQ_HIP(hipMemcpyAsync(g.rho, g.h_rho, packed * sizeof(double), hipMemcpyHostToDevice, s));
+  solve_device(g.rho, g.F, true);
+  XCQ_HIP(hipMemcpyAsync(g.h_F, g.F, packed * sizeof(double), hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipMemcpyAsync(g.h_e, g.eout, 2 * g.nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+}
+
+bool Engine::idle() const {
+  const hipError_t e = hipStreamQuery((hipStream_t)stream_);
+  if (e == hipSuccess) return true;
+  if (e == hipErrorNotReady) return false;
+  throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e) + " (stream query)");
+}
+
+SolveResult Engine::collect(double* fock_accum) {
+  if (!grid_) throw std::runtime_error("g2g: no grid");
+  DeviceGrid& g = *grid_;
+  XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
+  if (fock_accum)
+    for (long long k = 0; k < g.packed; ++k) fock_accum[k] += g.h_F[k];
+  SolveResult res;
+  res.energy = g.h_e[0];
+  res.electrons = g.h_e[1];
+  return res;
 }
 
 SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool want_energy) {

@@ -102,7 +102,7 @@ inline double cell(double dj, double dk, double rjk, double rmj, double rmk) {
 void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& rij,
            const std::vector<int>& atoms, Staging& st) {
   const int n = (int)st.x.size(), k = (int)atoms.size(), na = mol.natom;
-#pragma omp parallel for schedule(static) if (n > 2048)
+#pragma omp parallel for schedule(dynamic, 64) if (n > 256)
   for (int p = 0; p < n; ++p) {
     const int own = st.atom[p];
     double dist[512];
@@ -255,7 +255,7 @@ Molecule capture(const ProblemView& pv, unsigned grid_type) {
   return mol;
 }
 
-GridPartition build_partition(const Molecule& mol, const Options& opt) {
+StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
   const double t_start = wall();
   const int na = mol.natom, M = mol.M;
   if (na == 0) throw std::runtime_error("g2g: no atoms");
@@ -345,29 +345,33 @@ GridPartition build_partition(const Molecule& mol, const Options& opt) {
     }
   }
 
-  GridPartition out;
+  StagedPartition out;
+  out.rij = rij;
   std::vector<double> d2(na);
   auto emit = [&](int kind, Staging& st, const Selected& s) {
-    const int g = out.ngroups++;
-    out.kind.push_back(kind);
-    out.npts.push_back((int)st.x.size());
-    out.pt_off.push_back((int)out.x.size());
-    out.x.insert(out.x.end(), st.x.begin(), st.x.end());
-    out.y.insert(out.y.end(), st.y.begin(), st.y.end());
-    out.z.insert(out.z.end(), st.z.begin(), st.z.end());
-    out.w.insert(out.w.end(), st.w.begin(), st.w.end());
-    out.fn_off.push_back((int)out.fn.size());
+    GridPartition& o = out.groups;
+    o.ngroups++;
+    o.kind.push_back(kind);
+    o.npts.push_back((int)st.x.size());
+    o.pt_off.push_back((int)o.x.size());
+    o.x.insert(o.x.end(), st.x.begin(), st.x.end());
+    o.y.insert(o.y.end(), st.y.begin(), st.y.end());
+    o.z.insert(o.z.end(), st.z.begin(), st.z.end());
+    o.w.insert(o.w.end(), st.w.begin(), st.w.end());
+    out.owner.insert(out.owner.end(), st.atom.begin(), st.atom.end());
+    o.fn_off.push_back((int)o.fn.size());
     int m = 0;
     for (size_t k = 0; k < s.first.size(); ++k)
-      for (int c = 0; c < s.len[k]; ++c, ++m) out.fn.push_back(s.first[k] + c);
-    out.m.push_back(m);
-    out.nshells.push_back((int)s.first.size());
-    out.ns.push_back(s.ns);
-    out.np.push_back(s.np);
-    out.nd.push_back(s.nd);
-    out.nnuc.push_back((int)s.atoms.size());
-    out.fhash.push_back(fnv1a(s.first));
-    (void)g;
+      for (int c = 0; c < s.len[k]; ++c, ++m) o.fn.push_back(s.first[k] + c);
+    o.m.push_back(m);
+    o.nshells.push_back((int)s.first.size());
+    o.ns.push_back(s.ns);
+    o.np.push_back(s.np);
+    o.nd.push_back(s.nd);
+    o.nnuc.push_back((int)s.atoms.size());
+    o.fhash.push_back(fnv1a(s.first));
+    out.atom_off.push_back((int)out.atoms.size());
+    out.atoms.insert(out.atoms.end(), s.atoms.begin(), s.atoms.end());
   };
   for (unsigned i = 0; i < dim[0]; ++i)
     for (unsigned j = 0; j < dim[1]; ++j)
@@ -387,8 +391,6 @@ GridPartition build_partition(const Molecule& mol, const Options& opt) {
         Selected s;
         select(mol, opt, d2, amin_f, s);
         if (s.first.empty() || st.x.size() < opt.min_points_per_cube) continue;
-        weigh(mol, opt, rij, s.atoms, st);
-        if (st.x.size() < opt.min_points_per_cube) continue;
         emit(0, st, s);
       }
   if (opt.sphere_radius > 0) {
@@ -406,13 +408,56 @@ GridPartition build_partition(const Molecule& mol, const Options& opt) {
       Selected s;
       select(mol, opt, d2, amin_f, s);
       if (st.x.size() < opt.min_points_per_cube) continue;
-      weigh(mol, opt, rij, s.atoms, st);
-      if (st.x.size() < opt.min_points_per_cube) continue;
       emit(1, st, s);
     }
   }
-  out.seconds = wall() - t_start;
+  out.atom_off.push_back((int)out.atoms.size());
+  out.groups.seconds = wall() - t_start;
   return out;
+}
+
+GridPartition finish_partition(const Molecule& mol, const Options& opt,
+                               const StagedPartition& sp) {
+  const double t_start = wall();
+  const GridPartition& in = sp.groups;
+  GridPartition out;
+  for (int g = 0; g < in.ngroups; ++g) {
+    Staging st;
+    const int b = in.pt_off[g], n = in.npts[g];
+    st.x.assign(in.x.begin() + b, in.x.begin() + b + n);
+    st.y.assign(in.y.begin() + b, in.y.begin() + b + n);
+    st.z.assign(in.z.begin() + b, in.z.begin() + b + n);
+    st.w.assign(in.w.begin() + b, in.w.begin() + b + n);
+    st.atom.assign(sp.owner.begin() + b, sp.owner.begin() + b + n);
+    const std::vector<int> atoms(sp.atoms.begin() + sp.atom_off[g],
+                                 sp.atoms.begin() + sp.atom_off[g + 1]);
+    weigh(mol, opt, sp.rij, atoms, st);
+    if (st.x.size() < opt.min_points_per_cube) continue;
+    out.ngroups++;
+    out.kind.push_back(in.kind[g]);
+    out.npts.push_back((int)st.x.size());
+    out.pt_off.push_back((int)out.x.size());
+    out.x.insert(out.x.end(), st.x.begin(), st.x.end());
+    out.y.insert(out.y.end(), st.y.begin(), st.y.end());
+    out.z.insert(out.z.end(), st.z.begin(), st.z.end());
+    out.w.insert(out.w.end(), st.w.begin(), st.w.end());
+    out.fn_off.push_back((int)out.fn.size());
+    out.fn.insert(out.fn.end(), in.fn.begin() + in.fn_off[g],
+                  in.fn.begin() + in.fn_off[g] + in.m[g]);
+    out.m.push_back(in.m[g]);
+    out.nshells.push_back(in.nshells[g]);
+    out.ns.push_back(in.ns[g]);
+    out.np.push_back(in.np[g]);
+    out.nd.push_back(in.nd[g]);
+    out.nnuc.push_back(in.nnuc[g]);
+    out.fhash.push_back(in.fhash[g]);
+  }
+  out.seconds = in.seconds + (wall() - t_start);
+  return out;
+}
+
+GridPartition build_partition(const Molecule& mol, const Options& opt) {
+  return finish_partition(mol, opt, stage_partition(mol, opt));
 }
 
 }  // namespace xcq

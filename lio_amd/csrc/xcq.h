@@ -103,6 +103,23 @@ struct GridPartition {
 
 GridPartition build_partition(const Molecule& mol, const Options& opt);
 
+// The partition before the Becke weights: the candidate groups (same order)
+// with their significant functions, staged points (radial x angular weight)
+// and, per group, its nuclei (atoms[atom_off[g] .. atom_off[g+1]), ascending)
+// and per point the atom that generated it.  finish_partition applies the
+// group-local Becke weights and zero-weight removal on the host
+// (build_partition = finish_partition(stage_partition)); the engine can do
+// the same on the device (Engine::load_staged).
+struct StagedPartition {
+  GridPartition groups;     // points unweighted by Becke
+  std::vector<int> owner;   // per staged point
+  std::vector<int> atom_off, atoms;
+  std::vector<double> rij;  // natom x natom distances
+};
+StagedPartition stage_partition(const Molecule& mol, const Options& opt);
+GridPartition finish_partition(const Molecule& mol, const Options& opt,
+                               const StagedPartition& staged);
+
 // ---------------------------------------------------------------------------
 // Device engine
 // ---------------------------------------------------------------------------
@@ -153,9 +170,28 @@ class Engine {
   // (packed_offset).  Results are bitwise those of each molecule alone.
   void load_batch(const std::vector<const Molecule*>& mols,
                   const std::vector<const GridPartition*>& parts, Precision prec);
+  // Device grid rebuild: uploads the staged partitions, applies the
+  // group-local Becke weights and zero-weight removal on the device, keeps
+  // the groups with at least min_points_per_cube points and loads the batch
+  // from there.  Returns the final partitions (point arrays left empty: the
+  // points live on the device, see download_points).
+  std::vector<GridPartition> load_staged(const std::vector<const Molecule*>& mols,
+                                         const std::vector<const StagedPartition*>& staged,
+                                         const Options& opt, Precision prec);
+  // Frees the device grid (arena, pinned buffers, graph); a load restores it.
+  void unload();
+  // Padded point arrays of the loaded grid (padded_points() entries each).
+  void download_points(double* x, double* y, double* z, double* w);
+  long long padded_points() const;
   // One XC Fock build from packed rho (host memory).  fock_accum (host) gets
   // += F_xc; pass nullptr to skip the copy-back.
   SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy);
+  // Queues a build from a packed host density (copied to pinned memory at
+  // once) with its copy-back; idle() tells when it has finished and
+  // collect() waits for it and adds F onto fock_accum (may be null).
+  void solve_async(const double* rho_packed);
+  bool idle() const;
+  SolveResult collect(double* fock_accum);
   // Same with device-resident input/output (no PCIe copies), used for timing.
   SolveResult solve_device(const double* d_rho_packed, double* d_fock, bool want_energy);
   // Waits for the stream and reads the last solve's energy and electron count
@@ -190,6 +226,9 @@ class Engine {
   void* graph_ = nullptr;
   Tuning tuning_;
   void drop_graph();
+  void load_batch_impl(const std::vector<const Molecule*>& mols,
+                       const std::vector<const GridPartition*>& parts, Precision prec,
+                       const struct DevicePoints* dpts);
 };
 
 }  // namespace xcq

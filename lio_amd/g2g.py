@@ -77,6 +77,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_destroy.argtypes = [vp]
     L.xcq_set_problem.argtypes = [vp, C.POINTER(XcqProblem)]
     L.xcq_build_grid.argtypes = [vp, C.c_uint]
+    L.xcq_set_grid_mode.argtypes = [vp, C.c_int]
+    L.xcq_release_device.argtypes = [vp]
+    L.xcq_solve_async.argtypes = [vp, _dp]
+    L.xcq_query.argtypes = [vp]
+    L.xcq_collect.argtypes = [vp, _dp, C.POINTER(XcqSolveInfo)]
+    L.xcq_batch_create_ex.argtypes = [C.c_int, C.c_int, C.POINTER(XcqProblem), _up, C.c_int,
+                                      C.c_int, C.POINTER(vp)]
     L.xcq_set_precision.argtypes = [vp, C.c_int]
     L.xcq_query_grid.argtypes = [vp, C.POINTER(XcqGridInfo)]
     L.xcq_groups.argtypes = [vp, C.POINTER(XcqGroupInfo), C.c_int, C.POINTER(C.c_int)]
@@ -117,6 +124,7 @@ def _ptr(a, t):
 
 
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x6": 2}
+GRID_MODE = {"host": 0, "device": 1}
 
 DEFAULT_OPTIONS = dict(little_cube_size=8.0, sphere_radius=0.6, assign_all_functions=0,
                        remove_zero_weights=1, min_points_per_cube=1, max_function_exponent=10)
@@ -174,20 +182,28 @@ def host_partition(system, grid_type=None, **options):
 class Context:
     """One molecule on the device through the handle API."""
 
-    def __init__(self, system=None, device: int = -1, precision: str = "fp64", **options):
+    def __init__(self, system=None, device: int = -1, precision: str = "fp64",
+                 grid_mode: str = "host", **options):
         self.L = load_library()
         h = C.c_void_p()
         _check(self.L.xcq_create(device, C.byref(h)), "xcq_create")
         self.h = h
         self.M = 0
         self.set_precision(precision)
+        self.set_grid_mode(grid_mode)
         if system is not None:
             self.set_system(system, **options)
 
     def set_precision(self, precision: str):
-        """'fp64' or 'fp32' arithmetic for the two contractions."""
+        """'fp64', 'fp32' or 'bf16x6' arithmetic for the two contractions."""
         _check(self.L.xcq_set_precision(self.h, PRECISION[precision]), "xcq_set_precision")
         self.precision = precision
+
+    def set_grid_mode(self, mode: str):
+        """'host' or 'device': where the next build_grid applies the Becke
+        weights and the zero-weight removal."""
+        _check(self.L.xcq_set_grid_mode(self.h, GRID_MODE[mode]), "xcq_set_grid_mode")
+        self.grid_mode = mode
 
     def set_system(self, system, **options):
         self._p, self._keep = make_problem(system, **options)
@@ -241,6 +257,29 @@ class Context:
 
     def stream(self) -> int:
         return self.L.xcq_stream(self.h) or 0
+
+    def solve_async(self, rho=None):
+        """Queues a build (density copied at once); see done() and collect()."""
+        rho = self.system.rho_packed if rho is None else rho
+        self._rho_async = np.ascontiguousarray(rho, np.float64)
+        _check(self.L.xcq_solve_async(self.h, _ptr(self._rho_async, C.c_double)), "xcq_solve_async")
+
+    def done(self) -> bool:
+        r = self.L.xcq_query(self.h)
+        if r < 0:
+            _check(r, "xcq_query")
+        return r == 1
+
+    def collect(self):
+        """Waits for the queued build; (energy, packed F, info)."""
+        F = np.zeros(self.M * (self.M + 1) // 2)
+        info = XcqSolveInfo()
+        _check(self.L.xcq_collect(self.h, _ptr(F, C.c_double), C.byref(info)), "xcq_collect")
+        return info.energy, F, dict(electrons=info.electrons)
+
+    def release_device(self):
+        """Frees the device memory of this molecule; build_grid restores it."""
+        _check(self.L.xcq_release_device(self.h), "xcq_release_device")
 
     def close(self):
         if getattr(self, "h", None):
@@ -339,7 +378,8 @@ class Batch:
     the total.  Per-molecule results are bitwise those of the molecule alone.
     """
 
-    def __init__(self, systems, device: int = -1, precision: str = "fp64", **options):
+    def __init__(self, systems, device: int = -1, precision: str = "fp64",
+                 grid_mode: str = "host", **options):
         self.L = load_library()
         self.systems = list(systems)
         n = len(self.systems)
@@ -351,8 +391,9 @@ class Batch:
             self._keep.append(keep)
         gts = np.array([s.grid_type for s in self.systems], np.uint32)
         h = C.c_void_p()
-        _check(self.L.xcq_batch_create(device, n, probs, _ptr(gts, C.c_uint), PRECISION[precision],
-                                       C.byref(h)), "xcq_batch_create")
+        _check(self.L.xcq_batch_create_ex(device, n, probs, _ptr(gts, C.c_uint),
+                                          PRECISION[precision], GRID_MODE[grid_mode], C.byref(h)),
+               "xcq_batch_create_ex")
         self.h = h
         self.n = n
         self.precision = precision
