@@ -18,8 +18,15 @@ Prints ONE JSON line:
   fp64           the FP64 build: builds/s, phases, grid points/s
   mixed          speedup over FP64 and the errors against it
   latency_ms     p50/p99 of single synchronised mixed builds
-  roofline       the dominant mixed kernel (k_density_bf<4>) from HIP events
-                 recorded on the engine's stream
+  batch          64 independent molecules of 3-60 atoms resident together,
+                 every phase one launch over all of them, replayed as one
+                 hipGraph: molecules/s in both modes
+  serve          the continuous-batching replica server (reduced replica
+                 set, Poisson arrivals, every request an MD step with a new
+                 geometry and density): p50/p99 latency and builds/s in both
+                 modes
+  roofline       the dominant mixed kernel (split-BF16 density) from HIP
+                 events recorded on the engine's stream
   cpu_baseline   LIO's own CPU g2g path (oracle/_ref, when built) or this
                  repository's FP64 CPU restatement, timed on this host
 """
@@ -125,6 +132,53 @@ def run_mode(s, precision, steps, warmup, latency_samples):
                 phases=phases, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)))
 
 
+def bench_batch(precision, nmol=64, seed=3, reps=5):
+    """One build of nmol molecules per graph replay; returns timings."""
+    import numpy as np
+    import torch
+    from lio_amd import g2g, systems
+    mols = systems.molecule_batch(nmol, seed=seed)
+    t0 = time.perf_counter()
+    b = g2g.Batch(mols, precision=precision, grid_mode="device")
+    t_build = time.perf_counter() - t0
+    rho = torch.from_numpy(b.packed_density()).cuda()
+    F = torch.zeros_like(rho)
+    b.solve_device(rho.data_ptr(), F.data_ptr())
+    b.energies()
+    b.capture(rho.data_ptr(), F.data_ptr())
+    b.replay()
+    b.energies()
+    stream = torch.cuda.ExternalStream(b.stream())
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        b.replay()
+    end.record(stream)
+    E, N = b.energies()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / reps
+    out = {"molecules": nmol, "atoms": int(sum(m.natom for m in mols)),
+           "functions": int(sum(m.M for m in mols)), "ms_per_batch_build": ms,
+           "molecules_per_s": nmol / (ms * 1e-3), "grid_build_s": t_build,
+           "arena_GB": b.arena_bytes() / 1e9, "energies": E.tolist()}
+    b.close()
+    del rho, F
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_serve(precision, nrep=32, nreq=96, rate=20.0, seed=4):
+    """Reduced replica set through the server: every request is one MD step."""
+    from lio_amd import serve
+    reps = serve.replica_set(nrep, seed=seed, atoms=(10, 60))
+    srv = serve.Server(reps, precision=precision, slots=8, grid_mode="device")
+    out = srv.run(serve.poisson_arrivals(nreq, rate, nrep, seed=seed))
+    srv.close()
+    return {"replicas": nrep, "atoms": "10-60", "requests": nreq, "rate_hz": rate,
+            "builds_per_s": out["builds_per_s"], "latency_ms": out["latency_ms"],
+            "service_ms": out["service_ms"], "evictions": out["evictions"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +186,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--latency-samples", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB read + 1 GiB write elementwise kernel "
                          "(calibrates HBM byte counters)")
@@ -201,12 +257,24 @@ def main():
         "energy": rmx["energy"],
         "electrons": rmx["electrons"],
         "energy_fp64": r64["energy"],
-        "roofline": {"kernel": "k_density_bf<4>", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": "k_density_bf2<4,64>", "bound": "mfma", "achieved": achieved,
                      "peak": peak_alg, "unit": "TFLOP/s", "frac": achieved / peak_alg,
                      "traffic": None,
                      "peak_note": "algorithmic FLOP (8 n m^2 per group) against the BF16 dense "
                                   "peak / 6 products per split-BF16 product"},
     }
+    if not args.no_batch:
+        log("batch of 64 molecules ...")
+        bb = {p: bench_batch(p) for p in ("fp64", MIXED)}
+        e64, emx = np.array(bb["fp64"].pop("energies")), np.array(bb[MIXED].pop("energies"))
+        result["batch"] = {"fp64": bb["fp64"], "mixed": bb[MIXED],
+                           "speedup": bb[MIXED]["molecules_per_s"] / bb["fp64"]["molecules_per_s"],
+                           "max_dExc_vs_fp64": float(np.max(np.abs(emx - e64)))}
+        log("batch:", json.dumps(result["batch"]))
+    if not args.no_serve:
+        log("serving replicas ...")
+        result["serve"] = {"fp64": bench_serve("fp64"), "mixed": bench_serve(MIXED)}
+        log("serve:", json.dumps(result["serve"]))
     if args.copy_probe:
         # 1 GiB read + 1 GiB write by one elementwise kernel; a 2 GiB write in
         # between evicts the source from the 256 MB last-level cache first

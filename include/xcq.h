@@ -170,12 +170,16 @@ int xcq_batch_energies(xcq_batch* b, double* energy, double* electrons);
 int xcq_batch_kernel_times(xcq_batch* b, double ms[4]);
 void* xcq_batch_stream(xcq_batch* b);
 long long xcq_batch_arena_bytes(xcq_batch* b);
-/* Autotuner: times `reps` builds for every candidate Fock slice length in
- * slices[0 .. ncand) on these buffers (rebuilding the work lists for each),
- * keeps the fastest and returns the per-candidate ms in ms_out; *best is its
- * index.  A captured graph is dropped. */
+/* Autotuner: times `reps` builds for every candidate schedule c < ncand --
+ * Fock slice length slices[c] (points, multiple of 128) and, when variants is
+ * not NULL, split-BF16 density kernel variants[c] (0: 64-function chunks, 1 and 2:
+ * 128- and 64-function chunks with LDS-staged density pieces) -- on these buffers
+ * (rebuilding the work lists for each), keeps the fastest and returns the
+ * per-candidate ms in ms_out; *best is its index.  A captured graph is
+ * dropped. */
 int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps,
-                   const int* slices, int ncand, double* ms_out, int* best);
+                   const int* slices, const int* variants, int ncand, double* ms_out,
+                   int* best);
 
 #ifdef __cplusplus
 }

@@ -456,7 +456,7 @@ long long xcq_batch_arena_bytes(xcq_batch* b) {
 }
 
 int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, const int* slices,
-                   int ncand, double* ms_out, int* best) {
+                   const int* variants, int ncand, double* ms_out, int* best) {
   return guarded([&] {
     if (ncand <= 0 || reps <= 0) throw std::runtime_error("g2g: nothing to tune");
     int ibest = -1;
@@ -464,6 +464,7 @@ int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, 
     for (int c = 0; c < ncand; ++c) {
       xcq::Tuning t = b->engine->tuning();
       t.fock_slice = slices[c];
+      if (variants) t.density_variant = variants[c];
       b->engine->set_tuning(t);
       load_batch(b);
       b->engine->solve_device(d_rho, d_fock, true);  // warm-up
@@ -481,6 +482,7 @@ int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, 
     }
     xcq::Tuning t = b->engine->tuning();
     t.fock_slice = slices[ibest];
+    if (variants) t.density_variant = variants[ibest];
     b->engine->set_tuning(t);
     load_batch(b);
     if (best) *best = ibest;

@@ -54,8 +54,9 @@ namespace {
 constexpr int kPtAlign = 16;     // point padding (one MFMA column tile)
 constexpr int kFnAlign = 16;     // function padding (one MFMA row tile)
 constexpr int kDensPts = 64;     // points per k_density block (16 per wave)
+constexpr int kRowSplitMin = 64; // split-BF16 row-split density above this m_pad
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
-constexpr int kSliceDefault = 512;  // points per Fock slice (split-K over blocks)
+constexpr int kSliceDefault = 1024;  // points per Fock slice (split-K over blocks)
 constexpr int kEvalPts = 256;    // points per evaluation block
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -487,6 +488,7 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
 // split-BF16 (BF16X6) contractions
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
 
@@ -740,6 +742,324 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
   if (threadIdx.x == 0) {
     epart[2 * blockIdx.x] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
     epart[2 * blockIdx.x + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
+  }
+}
+
+// Density + functional on split-BF16 MFMA, wide form: 64 points per block
+// (16 per wave) and 128-function row chunks (8 row tiles per wave), with the
+// P_g pieces of a chunk and K step (128 rows x 32 columns x 3 pieces, 24 KB)
+// staged once per block in LDS (double-buffered: the next step's tile is
+// fetched into registers while this step's MFMAs run) instead of being read
+// by each of the four waves; the wider chunk halves the re-reads of the B
+// operand.  B operand (split in registers), epilogue and reductions as in
+// k_density_bf.
+template <int NT, int ROWS>
+__global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const DensityItem* __restrict__ items,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const unsigned short* __restrict__ pg3,
+                                                     const float* __restrict__ vals,
+                                                     const double* __restrict__ pw,
+                                                     double* __restrict__ factor,
+                                                     double* __restrict__ epart, int iexch) {
+  constexpr int NS = (NT == 4) ? 10 : 1;
+  constexpr int RT = ROWS / 16;
+  constexpr int UNITS = 3 * ROWS * 4;      // 16-byte units of one staged tile
+  __shared__ u4 sA[2][UNITS];              // [buf][piece][row][k octet]
+  __shared__ double sE[4][2];
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;
+  const int np = g.npts_pad, mp = g.m_pad;
+  const long long plane = (long long)mp * np;
+  const long long psz = (long long)mp * mp;
+  const int pt = active ? p0 + lrow : it.p0;  // inactive waves still stage tiles
+  const float* Vp = vals + g.val_off + (long long)pt * mp;  // this lane's point
+  const unsigned short* Pp = pg3 + 3 * g.pg_off;
+  double s[NS];
+  for (int k = 0; k < NS; ++k) s[k] = 0.0;
+  // staging map: 16-byte unit c = threadIdx.x + 256 i of a tile is piece
+  // c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4
+  auto fetch = [&](int c0, int k0, u4 (&r)[UNITS / 256]) {
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) {
+      const int c = threadIdx.x + 256 * i;
+      const int sp = c / (4 * ROWS), row = (c >> 2) & (ROWS - 1), oc = c & 3;
+      const int j = c0 + row;
+      r[i] = j < mp ? *reinterpret_cast<const u4*>(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc)
+                    : u4{0, 0, 0, 0};
+    }
+  };
+  for (int c0 = 0; c0 < mp; c0 += ROWS) {
+    const int nrt = min(RT, (mp - c0) / 16);
+    f4 acc[RT][NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[rt][t] = f4{0, 0, 0, 0};
+    u4 nxt[UNITS / 256];
+    fetch(c0, 0, nxt);
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) sA[0][threadIdx.x + 256 * i] = nxt[i];
+    __syncthreads();
+    for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
+      const bool more = k0 + 32 < mp;
+      if (more) fetch(c0, k0 + 32, nxt);
+      const int kk = k0 + 8 * lk;
+      bf8 bh[NT], bm[NT], bl[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        float x[8];
+        load8(Vp + t * plane + kk, x);
+        split3(x, bh[t], bm[t], bl[t]);
+      }
+      if (active) {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          if (rt < nrt) {
+            const int u = (16 * rt + lrow) * 4 + lk;
+            const bf8 ah = __builtin_bit_cast(bf8, sA[buf][u]);
+            const bf8 am = __builtin_bit_cast(bf8, sA[buf][ROWS * 4 + u]);
+            const bf8 al = __builtin_bit_cast(bf8, sA[buf][2 * ROWS * 4 + u]);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[rt][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
+          }
+        }
+      }
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < UNITS / 256; ++i) sA[buf ^ 1][threadIdx.x + 256 * i] = nxt[i];
+      }
+      __syncthreads();
+    }
+    if (!active) continue;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      if (rt >= nrt) continue;
+      const float* vj = Vp + c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
+      if constexpr (NT == 4) {
+        fl4 v[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
+                       yz = acc[rt][3][i];
+          const double v1 = v[1][i], v2 = v[2][i], v3 = v[3][i];
+          s[0] += (double)v[0][i] * y0;
+          s[1] += v1 * y0;
+          s[2] += v2 * y0;
+          s[3] += v3 * y0;
+          s[4] += (double)v[4][i] * y0 + v1 * yx;
+          s[5] += (double)v[5][i] * y0 + v2 * yy;
+          s[6] += (double)v[6][i] * y0 + v3 * yz;
+          s[7] += (double)v[7][i] * y0 + v1 * yy;
+          s[8] += (double)v[8][i] * y0 + v1 * yz;
+          s[9] += (double)v[9][i] * y0 + v2 * yz;
+        }
+      } else {
+        const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s[0] += (double)v0[i] * (double)acc[rt][0][i];
+      }
+    }
+  }
+  double e_w = 0.0, n_w = 0.0;
+  if (active) {
+    for (int k = 0; k < NS; ++k) {  // the 4 lane groups holding the same point
+      s[k] += __shfl_xor(s[k], 16);
+      s[k] += __shfl_xor(s[k], 32);
+    }
+    if (lk == 0) {
+      const int q = g.pt_off + p0 + lrow;
+      const double dens = s[0];
+      double ex, ec, y2a;
+      if constexpr (NT == 4) {
+        const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
+        const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
+        const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
+        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+      } else {
+        xc::lda_point(dens, iexch, ex, ec, y2a);
+      }
+      const double w = pw[q];
+      factor[q] = w * y2a;
+      e_w = (dens * w) * (ex + ec);
+      n_w = dens * w;
+    }
+    for (int o = 8; o >= 1; o >>= 1) {
+      e_w += __shfl_xor(e_w, o);
+      n_w += __shfl_xor(n_w, o);
+    }
+  }
+  if (lane == 0) {
+    sE[wave][0] = e_w;
+    sE[wave][1] = n_w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    epart[2 * blockIdx.x] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
+    epart[2 * blockIdx.x + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
+  }
+}
+
+// Density + functional on split-BF16 MFMA, row-split form for groups of
+// more than 64 functions: a block owns 16 points and its four waves split the
+// function row tiles (wave w takes tiles w, w + 4, ..., RTW of them a pass).
+// The B operand of the block's points is read from HBM and split into BF16
+// pieces once per K step (wave t stages derivative t) and shared by the four
+// waves through LDS (double-buffered); each wave reads the P_g pieces of its
+// own rows, which stay L2-resident.  The row-dot partials of the waves are
+// added in wave order before the functional.
+template <int NT, int RTW>
+__global__ __launch_bounds__(256, 2) void k_density_bf3(const DensityItem* __restrict__ items,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const unsigned short* __restrict__ pg3,
+                                                     const float* __restrict__ vals,
+                                                     const double* __restrict__ pw,
+                                                     double* __restrict__ factor,
+                                                     double* __restrict__ epart, int iexch) {
+  constexpr int NS = (NT == 4) ? 10 : 1;
+  constexpr int UNITS = NT * 3 * 64;  // 16-byte units of one staged B step
+  __shared__ u4 sB[2][UNITS];         // [buf][t][piece][point][k octet]
+  __shared__ double sPart[4][NS][16];
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0;
+  const int np = g.npts_pad, mp = g.m_pad, nrt = mp / 16;
+  const long long plane = (long long)mp * np;
+  const long long psz = (long long)mp * mp;
+  const float* Vp = vals + g.val_off + (long long)(p0 + lrow) * mp;  // lane's point
+  const unsigned short* Pp = pg3 + 3 * g.pg_off;
+  const bool stager = wave < NT;  // wave t splits derivative t
+  double s[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) s[k] = 0.0;
+  auto stage = [&](int buf, const float (&x)[8]) {
+    bf8 h, m, l;
+    split3(x, h, m, l);
+    const int u = (3 * wave) * 64 + lrow * 4 + lk;
+    sB[buf][u] = __builtin_bit_cast(u4, h);
+    sB[buf][u + 64] = __builtin_bit_cast(u4, m);
+    sB[buf][u + 128] = __builtin_bit_cast(u4, l);
+  };
+  const int npass = (nrt + 4 * RTW - 1) / (4 * RTW);
+  for (int pass = 0; pass < npass; ++pass) {
+    const int rbase = pass * 4 * RTW;
+    f4 acc[RTW][NT];
+#pragma unroll
+    for (int i = 0; i < RTW; ++i)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[i][t] = f4{0, 0, 0, 0};
+    float x[8];
+    if (stager) {
+      load8(Vp + wave * plane + 8 * lk, x);
+      stage(0, x);
+    }
+    __syncthreads();
+    for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
+      const bool more = k0 + 32 < mp;
+      if (stager && more) load8(Vp + wave * plane + k0 + 32 + 8 * lk, x);
+      bf8 bh[NT], bm[NT], bl[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int u = (3 * t) * 64 + lrow * 4 + lk;
+        bh[t] = __builtin_bit_cast(bf8, sB[buf][u]);
+        bm[t] = __builtin_bit_cast(bf8, sB[buf][u + 64]);
+        bl[t] = __builtin_bit_cast(bf8, sB[buf][u + 128]);
+      }
+#pragma unroll
+      for (int i = 0; i < RTW; ++i) {
+        const int r = rbase + wave + 4 * i;
+        if (r < nrt) {
+          const unsigned short* pa = Pp + (long long)(16 * r + lrow) * mp + k0 + 8 * lk;
+          const bf8 ah = *reinterpret_cast<const bf8*>(pa);
+          const bf8 am = *reinterpret_cast<const bf8*>(pa + psz);
+          const bf8 al = *reinterpret_cast<const bf8*>(pa + 2 * psz);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[i][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
+        }
+      }
+      if (stager && more) stage(buf ^ 1, x);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < RTW; ++i) {
+      const int r = rbase + wave + 4 * i;
+      if (r >= nrt) continue;
+      const float* vj = Vp + 16 * r + 4 * lk;  // functions j .. j+3 of this point
+      if constexpr (NT == 4) {
+        fl4 v[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) {
+          const double y0 = acc[i][0][ii], yx = acc[i][1][ii], yy = acc[i][2][ii],
+                       yz = acc[i][3][ii];
+          const double v1 = v[1][ii], v2 = v[2][ii], v3 = v[3][ii];
+          s[0] += (double)v[0][ii] * y0;
+          s[1] += v1 * y0;
+          s[2] += v2 * y0;
+          s[3] += v3 * y0;
+          s[4] += (double)v[4][ii] * y0 + v1 * yx;
+          s[5] += (double)v[5][ii] * y0 + v2 * yy;
+          s[6] += (double)v[6][ii] * y0 + v3 * yz;
+          s[7] += (double)v[7][ii] * y0 + v1 * yy;
+          s[8] += (double)v[8][ii] * y0 + v1 * yz;
+          s[9] += (double)v[9][ii] * y0 + v2 * yz;
+        }
+      } else {
+        const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) s[0] += (double)v0[ii] * (double)acc[i][0][ii];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {  // the 4 lane groups holding the same point
+    s[k] += __shfl_xor(s[k], 16);
+    s[k] += __shfl_xor(s[k], 32);
+  }
+  if (lk == 0) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sPart[wave][k][lrow] = s[k];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double e_w = 0.0, n_w = 0.0;
+    if (lk == 0) {
+      double t[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        t[k] = ((sPart[0][k][lrow] + sPart[1][k][lrow]) + sPart[2][k][lrow]) + sPart[3][k][lrow];
+      const double dens = t[0];
+      double ex, ec, y2a;
+      if constexpr (NT == 4) {
+        const double gr[3] = {2.0 * t[1], 2.0 * t[2], 2.0 * t[3]};
+        const double h1[3] = {2.0 * t[4], 2.0 * t[5], 2.0 * t[6]};
+        const double h2[3] = {2.0 * t[7], 2.0 * t[8], 2.0 * t[9]};
+        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+      } else {
+        xc::lda_point(dens, iexch, ex, ec, y2a);
+      }
+      const int q = g.pt_off + p0 + lrow;
+      const double w = pw[q];
+      factor[q] = w * y2a;
+      e_w = (dens * w) * (ex + ec);
+      n_w = dens * w;
+    }
+    for (int o = 8; o >= 1; o >>= 1) {
+      e_w += __shfl_xor(e_w, o);
+      n_w += __shfl_xor(n_w, o);
+    }
+    if (lane == 0) {
+      epart[2 * blockIdx.x] = e_w;
+      epart[2 * blockIdx.x + 1] = n_w;
+    }
   }
 }
 
@@ -1096,6 +1416,8 @@ struct DeviceGrid {
   int M = 0, natom = 0, iexch = 9, nval = 10;  // M, natom: whole batch
   int nmol = 1, max_M = 0;
   int fock_slice = kSliceDefault;
+  int density_variant = 2;
+  int nrowsplit = 0;  // leading density items on the row-split kernel
   long long packed = 0;                 // packed entries of the whole batch
   std::vector<long long> mol_packed;    // per molecule offset, then the total
   DevMol* mols = nullptr;
@@ -1105,7 +1427,10 @@ struct DeviceGrid {
   int ngroups = 0, neval = 0, ngather = 0, ndensity = 0, nfock = 0, ngroup_tiles = 0;
   long long npts_pad = 0, val_elems = 0, pg_elems = 0;
   char* arena = nullptr;
-  size_t arena_bytes = 0;
+  size_t arena_bytes = 0;   // used by this grid
+  size_t arena_cap = 0;     // allocated (reused by the next grid when it fits)
+  size_t pinned_packed = 0; // capacity of h_rho / h_F (entries) and h_e (pairs)
+  int pinned_mols = 0;
   double eval_ms = 0.0;
   DevFn* bas = nullptr;
   DevGroup* groups = nullptr;
@@ -1183,6 +1508,7 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
   drop_graph();
   release(grid_);
+  if (stage_) (void)hipFree(stage_);
   for (auto& e : events_)
     if (e) (void)hipEventDestroy((hipEvent_t)e);
   if (stream_) (void)hipStreamDestroy((hipStream_t)stream_);
@@ -1214,7 +1540,14 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   XCQ_HIP(hipSetDevice(device_));
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
   drop_graph();
-  release(grid_);
+  // the previous grid's arena and pinned buffers are reused when large enough
+  // (a rebuilt grid of a moved molecule is about the same size): freeing
+  // device memory waits for the whole device, which would stall the builds
+  // of other molecules in flight on other streams
+  struct Old {
+    DeviceGrid* g;
+    ~Old() { release(g); }
+  } old{grid_};
   grid_ = nullptr;
   const int nmol = (int)mols.size();
   if (nmol == 0 || parts.size() != mols.size())
@@ -1227,6 +1560,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   auto* dg = new DeviceGrid;
   dg->nmol = nmol;
   dg->fock_slice = tuning_.fock_slice;
+  dg->density_variant = tuning_.density_variant;
+  if (dg->density_variant < 0 || dg->density_variant > 4)
+    throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
   dg->nval = dg->gga ? 10 : 1;
@@ -1337,7 +1673,10 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
                    [&](int a, int b) { return groups[a].m_pad > groups[b].m_pad; });
   std::vector<EvalItem> eitems;
   std::vector<GatherItem> gitems, pmitems;
-  std::vector<DensityItem> ditems;
+  // split-BF16 variant 3: groups of more than 64 functions run the row-split
+  // kernel on 16-point blocks (listed first), the rest 64-point blocks
+  const bool rowsplit = bf && (tuning_.density_variant == 3 || tuning_.density_variant == 4);
+  std::vector<DensityItem> ditems, dsmall;
   for (int g : order) {
     const DevGroup& d = groups[g];
     for (int k = 0; k < d.m_pad; ++k)
@@ -1345,8 +1684,13 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
     if (bf)
       for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += 256) pmitems.push_back(GatherItem{g, e0});
-    for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) ditems.push_back(DensityItem{g, p0});
+    if (rowsplit && d.m_pad > kRowSplitMin)
+      for (int p0 = 0; p0 < d.npts_pad; p0 += 16) ditems.push_back(DensityItem{g, p0});
+    else
+      for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) dsmall.push_back(DensityItem{g, p0});
   }
+  dg->nrowsplit = rowsplit ? (int)ditems.size() : 0;
+  ditems.insert(ditems.end(), dsmall.begin(), dsmall.end());
   // per molecule, its density blocks in ascending block order (the energy
   // reduction order, the same whether the molecule is alone or batched)
   std::vector<int> eidx;
@@ -1415,21 +1759,48 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   Carver sizing{nullptr};
   plan(sizing);
   dg->arena_bytes = sizing.used + 256;
-  size_t free_b = 0, total_b = 0;
-  XCQ_HIP(hipMemGetInfo(&free_b, &total_b));
-  if (dg->arena_bytes > free_b) {
-    const size_t need = dg->arena_bytes;
-    delete dg;
-    throw std::runtime_error("g2g: grid needs " + std::to_string(need >> 20) +
-                             " MiB of device memory, " + std::to_string(free_b >> 20) +
-                             " MiB free");
+  if (old.g && old.g->arena && old.g->arena_cap >= dg->arena_bytes) {
+    dg->arena = old.g->arena;
+    dg->arena_cap = old.g->arena_cap;
+    old.g->arena = nullptr;
+  } else {
+    const size_t reclaim = old.g && old.g->arena ? old.g->arena_cap : 0;
+    // a grown grid gets 25 % headroom so that the next steps reuse it
+    const size_t want = reclaim ? dg->arena_bytes + dg->arena_bytes / 4 : dg->arena_bytes;
+    size_t free_b = 0, total_b = 0;
+    XCQ_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (dg->arena_bytes > free_b + reclaim) {
+      const size_t need = dg->arena_bytes;
+      delete dg;
+      throw std::runtime_error("g2g: grid needs " + std::to_string(need >> 20) +
+                               " MiB of device memory, " + std::to_string(free_b >> 20) +
+                               " MiB free");
+    }
+    if (reclaim) {
+      (void)hipFree(old.g->arena);
+      old.g->arena = nullptr;
+    }
+    const size_t alloc = want <= free_b + reclaim ? want : dg->arena_bytes;
+    XCQ_HIP(hipMalloc(&dg->arena, alloc));
+    dg->arena_cap = alloc;
   }
-  XCQ_HIP(hipMalloc(&dg->arena, dg->arena_bytes));
   Carver carve{dg->arena};
   plan(carve);
-  XCQ_HIP(hipHostMalloc(&dg->h_rho, packed * sizeof(double), hipHostMallocDefault));
-  XCQ_HIP(hipHostMalloc(&dg->h_F, packed * sizeof(double), hipHostMallocDefault));
-  XCQ_HIP(hipHostMalloc(&dg->h_e, 2 * nmol * sizeof(double), hipHostMallocDefault));
+  if (old.g && old.g->h_rho && old.g->pinned_packed >= (size_t)packed &&
+      old.g->pinned_mols >= nmol) {
+    dg->h_rho = old.g->h_rho;
+    dg->h_F = old.g->h_F;
+    dg->h_e = old.g->h_e;
+    dg->pinned_packed = old.g->pinned_packed;
+    dg->pinned_mols = old.g->pinned_mols;
+    old.g->h_rho = old.g->h_F = old.g->h_e = nullptr;
+  } else {
+    dg->pinned_packed = (size_t)packed;
+    dg->pinned_mols = nmol;
+    XCQ_HIP(hipHostMalloc(&dg->h_rho, packed * sizeof(double), hipHostMallocDefault));
+    XCQ_HIP(hipHostMalloc(&dg->h_F, packed * sizeof(double), hipHostMallocDefault));
+    XCQ_HIP(hipHostMalloc(&dg->h_e, 2 * nmol * sizeof(double), hipHostMallocDefault));
+  }
 
   hipStream_t s = (hipStream_t)stream_;
   auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -1584,14 +1955,17 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
     d.w = c.take<double>(npts);
   };
   carve(plan);
-  char* base = nullptr;
-  XCQ_HIP(hipMalloc(&base, plan.used + 256));
-  struct Free {
-    char* p;
-    ~Free() {
-      if (p) (void)hipFree(p);
-    }
-  } guard{base};
+  // the staging buffer is kept (grow-only) across rebuilds, for the same
+  // reason as the arena
+  if (stage_cap_ < plan.used + 256) {
+    if (stage_) (void)hipFree(stage_);
+    stage_ = nullptr;
+    stage_cap_ = 0;
+    const size_t want = plan.used + 256 + (plan.used + 256) / 4;
+    XCQ_HIP(hipMalloc(&stage_, want));
+    stage_cap_ = want;
+  }
+  char* base = static_cast<char*>(stage_);
   Carver c{base};
   carve(c);
   hipStream_t s = (hipStream_t)stream_;
@@ -1722,12 +2096,50 @@ static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, b
                        g.mols, g.fnl, d_rho, pg3);
   mark(ev, 1, s);
   if (g.ndensity > 0) {
-    if (g.gga)
-      hipLaunchKernelGGL(k_density_bf<4>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
-                         pg3, vals, g.pw, g.factor, g.epart, g.iexch);
-    else
-      hipLaunchKernelGGL(k_density_bf<1>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
-                         pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+    if (g.density_variant == 1) {
+      if (g.gga)
+        hipLaunchKernelGGL((k_density_bf2<4, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+      else
+        hipLaunchKernelGGL((k_density_bf2<1, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+    } else if (g.density_variant == 2) {
+      if (g.gga)
+        hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+      else
+        hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+    } else if (g.density_variant == 3 || g.density_variant == 4) {
+      const int nr = g.nrowsplit, ns = g.ndensity - nr;
+      const DensityItem* si = g.ditems + nr;
+      double* se = g.epart + 2 * (size_t)nr;
+      if (g.gga) {
+        if (nr > 0 && g.density_variant == 3)
+          hipLaunchKernelGGL((k_density_bf3<4, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
+                             pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+        if (nr > 0 && g.density_variant == 4)
+          hipLaunchKernelGGL((k_density_bf3<4, 4>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
+                             pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+        if (ns > 0)
+          hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
+                             vals, g.pw, g.factor, se, g.iexch);
+      } else {
+        if (nr > 0)
+          hipLaunchKernelGGL((k_density_bf3<1, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
+                             pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+        if (ns > 0)
+          hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
+                             vals, g.pw, g.factor, se, g.iexch);
+      }
+    } else {
+      if (g.gga)
+        hipLaunchKernelGGL(k_density_bf<4>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
+                           pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+      else
+        hipLaunchKernelGGL(k_density_bf<1>, dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
+                           pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+    }
   }
   XCQ_HIP(hipGetLastError());
   mark(ev, 2, s);
@@ -1796,6 +2208,9 @@ void Engine::unload() {
   drop_graph();
   release(grid_);
   grid_ = nullptr;
+  if (stage_) (void)hipFree(stage_);
+  stage_ = nullptr;
+  stage_cap_ = 0;
 }
 
 void Engine::replay_graph() {

@@ -140,8 +140,16 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 //   fock_slice: points per Fock split-K slice (multiple of 128): longer
 //               slices leave fewer partial tiles to add, shorter ones more
 //               blocks for small groups.
+//   density_variant (split-BF16 mode): 0 = 64-function row chunks with each
+//               wave reading its P_g pieces; 1 = 128-function chunks and
+//               2 = 64-function chunks with the pieces staged once per block
+//               in LDS; 3 = 2 for groups of up to 64 functions and, above,
+//               16-point blocks whose waves split the rows (6 row tiles a
+//               wave per pass) and share the B operand through LDS; 4 = 3
+//               with 4 row tiles a wave per pass.
 struct Tuning {
-  int fock_slice = 512;
+  int fock_slice = 1024;
+  int density_variant = 2;
 };
 
 struct SolveResult {
@@ -224,6 +232,8 @@ class Engine {
   DeviceGrid* grid_ = nullptr;
   void* events_[8] = {};
   void* graph_ = nullptr;
+  void* stage_ = nullptr;     // device grid rebuild staging (kept, grow-only)
+  size_t stage_cap_ = 0;
   Tuning tuning_;
   void drop_graph();
   void load_batch_impl(const std::vector<const Molecule*>& mols,

@@ -108,8 +108,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_batch_stream.restype = vp
     L.xcq_batch_arena_bytes.argtypes = [vp]
     L.xcq_batch_arena_bytes.restype = C.c_longlong
-    L.xcq_batch_tune.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.c_int, _dp,
-                                 C.POINTER(C.c_int)]
+    L.xcq_batch_tune.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.c_int, _dp, C.POINTER(C.c_int)]
     _LIB = L
     return L
 
@@ -440,16 +440,25 @@ class Batch:
     def arena_bytes(self) -> int:
         return int(self.L.xcq_batch_arena_bytes(self.h))
 
-    def tune(self, d_rho: int, d_fock: int, slices=(256, 512, 1024, 2048), reps: int = 5):
-        """Autotunes the Fock slice length on these buffers; returns
-        (best slice, {slice: ms per build})."""
+    def tune(self, d_rho: int, d_fock: int, slices=(256, 512, 1024, 2048), variants=None,
+             reps: int = 5):
+        """Autotunes the schedule on these buffers over the candidates
+        (slices[c], variants[c]) -- or every slice with the current density
+        variant when variants is None; returns (best (slice, variant or None),
+        {candidate: ms per build})."""
         sl = np.array(slices, np.int32)
+        va = None if variants is None else np.array(variants, np.int32)
+        if va is not None and len(va) != len(sl):
+            raise ValueError("one density variant per candidate")
         ms = np.zeros(len(sl))
         best = C.c_int(-1)
+        vp_ = None if va is None else va.ctypes.data_as(C.POINTER(C.c_int))
         _check(self.L.xcq_batch_tune(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock), reps,
-                                     sl.ctypes.data_as(C.POINTER(C.c_int)), len(sl),
+                                     sl.ctypes.data_as(C.POINTER(C.c_int)), vp_, len(sl),
                                      _ptr(ms, C.c_double), C.byref(best)), "xcq_batch_tune")
-        return int(sl[best.value]), dict(zip(sl.tolist(), ms.tolist()))
+        keys = [(int(a), None if va is None else int(b))
+                for a, b in zip(sl, va if va is not None else sl)]
+        return keys[best.value], dict(zip(keys, ms.tolist()))
 
     def close(self):
         if getattr(self, "h", None):

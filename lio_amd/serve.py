@@ -58,20 +58,24 @@ class Replica:
         self.pos_sigma = pos_sigma_ang * ANG
         self.rho_sigma = rho_sigma
         self.trace0 = float(np.trace(system.P))
+        self._step = None  # cached walk position (step, positions, density)
 
     def state(self, step: int) -> systems.System:
         """The system at MD step ``step`` (step 0 is the initial one).  The walk
-        is regenerated from the seed, so a step is the same whoever asks."""
-        rng = np.random.default_rng((self.seed, 0))
-        pos = self.base.pos.copy()
-        P = self.base.P.copy()
-        M = P.shape[0]
-        for _ in range(step):
-            pos += rng.normal(0.0, self.pos_sigma, size=pos.shape)
-            D = rng.normal(0.0, self.rho_sigma, size=(M, M))
-            P += 0.5 * (D + D.T)
-            P *= self.trace0 / np.trace(P)
-        return dataclasses.replace(self.base, pos=pos, P=P, rho_packed=systems.pack_density(P))
+        is a fixed seeded sequence, so a step is the same whoever asks; asking
+        for the steps in order advances it by one step each time."""
+        if self._step is None or step < self._step:
+            self._rng = np.random.default_rng((self.seed, 0))
+            self._step, self._pos, self._P = 0, self.base.pos.copy(), self.base.P.copy()
+        M = self._P.shape[0]
+        while self._step < step:
+            self._pos = self._pos + self._rng.normal(0.0, self.pos_sigma, size=self._pos.shape)
+            D = self._rng.normal(0.0, self.rho_sigma, size=(M, M))
+            P = self._P + 0.5 * (D + D.T)
+            self._P = P * (self.trace0 / np.trace(P))
+            self._step += 1
+        return dataclasses.replace(self.base, pos=self._pos.copy(), P=self._P.copy(),
+                                   rho_packed=systems.pack_density(self._P))
 
 
 class Server:

@@ -25,6 +25,7 @@ with the Obara-Saika recursion for Cartesian Gaussians.
 """
 from __future__ import annotations
 
+import functools
 import json
 import math
 import os
@@ -130,7 +131,22 @@ def _orbit_d(r, u, w):
     return pts
 
 
+@functools.lru_cache(maxsize=None)
+def _lebedev_cached(npts: int) -> tuple[np.ndarray, np.ndarray]:
+    e, w = _lebedev(npts)
+    e.setflags(write=False)
+    w.setflags(write=False)
+    return e, w
+
+
 def lebedev(npts: int) -> tuple[np.ndarray, np.ndarray]:
+    """(directions (n, 3), weights x 4 pi) of the n-point Lebedev rule
+    (copies of a cached table)."""
+    e, w = _lebedev_cached(npts)
+    return e.copy(), w.copy()
+
+
+def _lebedev(npts: int) -> tuple[np.ndarray, np.ndarray]:
     """Lebedev angular rule as LIO tabulates it: unit vectors (npts x 3) and
     weights summing to 4*pi.  Orbit parameters and weights carry the 12 digits
     of LIO's tables (Lebedev, Zh. Vychisl. Mat. Mat. Fiz. 15 (1975), 16 (1976)),

@@ -84,6 +84,28 @@ def test_batch_graph_replays_match_eager():
     b.close()
 
 
+def test_batch_tuner_over_density_variants():
+    """Split-BF16: both density kernels give the same result to FP32-GEMM
+    round-off, and the tuner keeps one of them."""
+    torch = _torch()
+    from lio_amd import g2g
+    sys_ = _systems()
+    b = g2g.Batch(sys_, precision="bf16x6")
+    rho = torch.from_numpy(b.packed_density()).cuda()
+    F = torch.zeros_like(rho)
+    res = {}
+    for v in (0, 1):
+        b.tune(rho.data_ptr(), F.data_ptr(), slices=(512,), variants=(v,), reps=1)
+        res[v] = _batch_solve(b, torch)
+    (E0, N0, F0), (E1, N1, F1) = res[0], res[1]
+    print("variants", E0, E1, flush=True)
+    assert np.max(np.abs(E0 - E1)) < 1e-9 and np.max(np.abs(N0 - N1)) < 1e-9
+    assert np.max(np.abs(F0 - F1)) < 1e-9
+    best, table = b.tune(rho.data_ptr(), F.data_ptr(), slices=(512, 512), variants=(0, 1), reps=2)
+    assert best in table
+    b.close()
+
+
 def test_batch_tuner_keeps_results():
     torch = _torch()
     from lio_amd import g2g
@@ -94,7 +116,7 @@ def test_batch_tuner_keeps_results():
     F = torch.zeros_like(rho)
     best, table = b.tune(rho.data_ptr(), F.data_ptr(), slices=(128, 512, 2048), reps=2)
     print("tuned Fock slice", best, table, flush=True)
-    assert best in (128, 512, 2048) and all(v > 0 for v in table.values())
+    assert best[0] in (128, 512, 2048) and all(v > 0 for v in table.values())
     E, N, F1 = _batch_solve(b, torch)
     assert np.allclose(E, E0, rtol=0, atol=1e-11)
     assert np.allclose(N, N0, rtol=0, atol=1e-11)

@@ -29,18 +29,8 @@ def _gpu():
         pytest.skip("no GPU")
 
 
-# The BF16 matrix unit truncates the low bits of small products when it
-# aligns a 32-product sum to its largest term; on the larger systems that
-# leaves a one-signed error in rho (measured: (H2O)32 |dExc| 2.5e-6, dN
-# 4.1e-6), so those cases are expected to miss the bar for now.
-_BF16_BIAS = pytest.mark.xfail(reason="BF16 MFMA truncation bias above 1e-6 on >= 25 atoms",
-                               strict=False)
-CASES = [("c0_pbe", "fp32"), ("c0_lda3", "fp32"), ("mol25_s103_pbe", "fp32"),
-         ("mol45_s104_pbe", "fp32"), ("c1_pbe", "fp32"),
-         ("c0_pbe", "bf16x6"), ("c0_lda3", "bf16x6"),
-         pytest.param("mol25_s103_pbe", "bf16x6", marks=_BF16_BIAS),
-         pytest.param("mol45_s104_pbe", "bf16x6", marks=_BF16_BIAS),
-         pytest.param("c1_pbe", "bf16x6", marks=_BF16_BIAS)]
+CASES = [(n, p) for p in ("fp32", "bf16x6")
+         for n in ("c0_pbe", "c0_lda3", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe")]
 
 
 @pytest.mark.parametrize("name,precision", CASES)
