@@ -46,6 +46,10 @@ PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 PEAK_HBM_GBS = 8000.0
 MIXED = "bf16x6"
 BF16_PRODUCTS = 6          # BF16 MFMA products per split-BF16 tile product
+DENSITY_KERNEL = "k_density_bf2<4,64>"
+# HBM bytes per launch of DENSITY_KERNEL on (H2O)32 (rocprofv3 FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 in KiB, copy-kernel calibrated)
+DENSITY_TRAFFIC = (3429165 * 2.0 + 5712 * 1.0) * 1024
 
 
 def log(*a):
@@ -124,12 +128,22 @@ def run_mode(s, precision, steps, warmup, latency_samples):
         ctx.solve_device(rho.data_ptr(), F.data_ptr())
         ctx.sync_energy()
         lat.append(1e3 * (time.perf_counter() - t0))
+    groups = ctx.groups()
     ctx.close()
     lat = np.sort(np.array(lat))
     log("%s: %.3f ms/build, phases ms: expand %.4f density %.4f fock %.4f pack %.4f"
         % ((precision, ms_step) + tuple(phases)))
+    # algorithmic HBM bytes of one density launch: the ten value planes of
+    # every (point, function) pair once, the group density matrices once (FP64,
+    # or three BF16 pieces), the point weights read and the w*y2a factors written
+    vb = 8 if precision == "fp64" else 4
+    pb = 8 if precision == "fp64" else 6
+    pairs = sum(g["npoints"] * g["nfuncs"] for g in groups)
+    msq = sum(g["nfuncs"] ** 2 for g in groups)
+    dens_bytes = 10 * vb * pairs + pb * msq + 16 * info["npoints"]
     return dict(info=info, ms=ms_step, energy=energy, electrons=electrons, fock=fock,
-                phases=phases, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)))
+                phases=phases, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                dens_bytes=dens_bytes)
 
 
 def bench_batch(precision, nmol=64, seed=3, reps=5):
@@ -216,11 +230,12 @@ def main():
     dF = float(np.max(np.abs(rmx["fock"] - r64["fock"])))
     dN = rmx["electrons"] - r64["electrons"]
 
-    # dominant kernel of the mixed build: the density/functional launch
+    # dominant kernel of the mixed build: the density/functional launch, bound
+    # by HBM (its MFMA work runs far below the matrix peak)
     density_flops = 0.8 * info["flops"]  # 8 n m^2 of the 10 n m^2 per group
     t_dens = rmx["phases"][1] * 1e-3
-    achieved = density_flops / t_dens / 1e12
-    peak_alg = PEAK_BF16_TFLOPS / BF16_PRODUCTS
+    achieved = rmx["dens_bytes"] / t_dens / 1e9
+    mfma_tflops = density_flops * BF16_PRODUCTS / t_dens / 1e12
     fp64_density = 0.8 * r64["info"]["flops"] / (r64["phases"][1] * 1e-3) / 1e12
 
     result = {
@@ -257,11 +272,16 @@ def main():
         "energy": rmx["energy"],
         "electrons": rmx["electrons"],
         "energy_fp64": r64["energy"],
-        "roofline": {"kernel": "k_density_bf2<4,64>", "bound": "mfma", "achieved": achieved,
-                     "peak": peak_alg, "unit": "TFLOP/s", "frac": achieved / peak_alg,
-                     "traffic": None,
-                     "peak_note": "algorithmic FLOP (8 n m^2 per group) against the BF16 dense "
-                                  "peak / 6 products per split-BF16 product"},
+        "roofline": {"kernel": DENSITY_KERNEL, "bound": "hbm", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": DENSITY_TRAFFIC,
+                     "algorithmic_bytes": rmx["dens_bytes"], "kernel_ms": rmx["phases"][1],
+                     "note": "achieved = algorithmic bytes (40 B per point-function pair, 6 B per "
+                             "P_g element, 16 B per point) / HIP-event duration of the launch; "
+                             "traffic = TCC FETCH_SIZE + WRITE_SIZE per launch, calibrated "
+                             "against a plain copy kernel (profiles/xc_fock_h2o32.md)",
+                     "mfma_tflops_executed": mfma_tflops,
+                     "mfma_frac": mfma_tflops / PEAK_BF16_TFLOPS},
     }
     if not args.no_batch:
         log("batch of 64 molecules ...")
