@@ -119,6 +119,13 @@ int xcq_solve(xcq_context* ctx, const double* rho_packed, double* fock_accum,
  * copies and no synchronisation: the work is queued on the context stream
  * (xcq_stream); energy and electrons are read back by xcq_sync_energy. */
 int xcq_solve_device(xcq_context* ctx, const double* d_rho_packed, double* d_fock);
+/* Energy and XC forces (g2g_solve_groups_ computation types 2/3,
+ * g2g/cpu/iteration.cpp:213-268 for the force, g2g/partition.cpp:516-526 for
+ * the accumulation): forces_accum is the caller's natom x 3 column-major
+ * array with leading dimension ld >= natom, accumulated (+=); fock_accum
+ * (may be NULL) additionally receives += F_xc. */
+int xcq_solve_forces(xcq_context* ctx, const double* rho_packed, double* fock_accum,
+                     double* forces_accum, int ld, xcq_solve_info* info);
 int xcq_sync_energy(xcq_context* ctx, double* energy, double* electrons);
 /* Asynchronous build from a packed host density (copied at once): queued on
  * the context stream with the copy-back of F, Exc and N.  xcq_query returns
@@ -163,6 +170,12 @@ int xcq_batch_offsets(xcq_batch* b, long long* offsets);
 int xcq_batch_solve_device(xcq_batch* b, const double* d_rho, double* d_fock);
 /* Records one build on these buffers as a hipGraph (replayed by
  * xcq_batch_replay, one launch call per build). */
+/* Full build plus the XC forces of every batch atom into d_forces (device,
+ * atom-major: component c of batch atom a at 3 a + c, overwritten); d_fock
+ * may be NULL.  xcq_batch_atoms gives the first batch atom of each molecule
+ * (n + 1 entries, the last the total). */
+int xcq_batch_solve_forces(xcq_batch* b, const double* d_rho, double* d_fock, double* d_forces);
+int xcq_batch_atoms(xcq_batch* b, int* first_atom);
 int xcq_batch_capture(xcq_batch* b, const double* d_rho, double* d_fock);
 int xcq_batch_replay(xcq_batch* b);
 /* Waits; Exc and N of every molecule (n entries each). */

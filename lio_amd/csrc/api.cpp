@@ -274,6 +274,21 @@ int xcq_solve(xcq_context* ctx, const double* rho, double* fock_accum, xcq_solve
   });
 }
 
+int xcq_solve_forces(xcq_context* ctx, const double* rho, double* fock_accum,
+                     double* forces_accum, int ld, xcq_solve_info* info) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
+    if (!forces_accum) throw std::runtime_error("g2g: no force array");
+    xcq::SolveResult r = ctx->engine->solve(rho, fock_accum, true, forces_accum, ld);
+    if (info) {
+      info->energy = r.energy;
+      info->electrons = r.electrons;
+      info->seconds = r.seconds;
+      for (int k = 0; k < 4; ++k) info->kernel_ms[k] = r.kernel_ms[k];
+    }
+  });
+}
+
 int xcq_solve_device(xcq_context* ctx, const double* d_rho, double* d_fock) {
   return guarded([&] {
     if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
@@ -423,6 +438,20 @@ int xcq_batch_offsets(xcq_batch* b, long long* offsets) {
 
 int xcq_batch_solve_device(xcq_batch* b, const double* d_rho, double* d_fock) {
   return guarded([&] { b->engine->solve_device(d_rho, d_fock, true); });
+}
+
+int xcq_batch_solve_forces(xcq_batch* b, const double* d_rho, double* d_fock, double* d_forces) {
+  return guarded([&] {
+    if (!d_forces) throw std::runtime_error("g2g: no force array");
+    b->engine->solve_device(d_rho, d_fock, true, d_forces);
+  });
+}
+
+int xcq_batch_atoms(xcq_batch* b, int* first_atom) {
+  return guarded([&] {
+    const int n = b->engine->molecules();
+    for (int k = 0; k <= n; ++k) first_atom[k] = b->engine->atoms(k);
+  });
 }
 
 int xcq_batch_capture(xcq_batch* b, const double* d_rho, double* d_fock) {
@@ -633,16 +662,19 @@ void g2g_new_grid_(const unsigned* grid_type) {
 }
 
 void g2g_solve_groups_(const unsigned* computation_type, double* energy, double* forces) {
-  (void)forces;
   abi("g2g_solve_groups_", [&] {
     xcq_context* ctx = ctx_or_die("g2g_solve_groups_");
     const unsigned ct = *computation_type;
     if (ct > 3) throw std::runtime_error("unknown computation type");
-    if (ct >= 2) throw std::runtime_error("XC forces are not provided by this build");
     if (ctx->grid_type < 0) throw std::runtime_error("solve before g2g_reload_atom_positions_");
+    // 0 Fock, 1 energy, 2 energy + forces, 3 forces (init.cpp:340-351)
     const bool compute_rmm = (ct == 0);
-    const bool compute_energy = (ct == 1) || g_opt.energy_all_iterations;
-    xcq::SolveResult r = ctx->engine->solve(ctx->pv.rho, compute_rmm ? ctx->pv.fock : nullptr, compute_energy);
+    const bool compute_forces = (ct == 2 || ct == 3);
+    const bool compute_energy = (ct == 1 || ct == 2) || g_opt.energy_all_iterations;
+    if (compute_forces && !forces) throw std::runtime_error("forces requested without a force array");
+    xcq::SolveResult r = ctx->engine->solve(ctx->pv.rho, compute_rmm ? ctx->pv.fock : nullptr,
+                                            compute_energy, compute_forces ? forces : nullptr,
+                                            (int)ctx->pv.max_atoms);
     const double e = compute_energy ? r.energy : 0.0;
     *energy = e;
     if (std::isnan(e)) die("g2g_solve_groups_", "XC energy is NaN");

@@ -84,6 +84,7 @@ struct DevGroup {
   long long phi_off;   // function-major phi plane (split-BF16 mode): m_pad x npts_pad
   int mol;             // molecule of the batch
   int pad2_;
+  long long force_off; // force partials: one 3 x m_pad block per 64-point block
 };
 
 // One molecule of a batch: its functions are [fn_base, fn_base + M) of the
@@ -119,8 +120,8 @@ __device__ inline Axis axis(double v, int l) {
 }
 
 // Contracted Cartesian Gaussian x^lx y^ly z^lz sum_k c_k exp(-a_k r^2) and
-// its derivatives (g2g/cpu/functions.cpp:56-354): NV = 1 value, 10 adds the
-// gradient and the Hessian (xx, yy, zz, xy, xz, yz).  Primitives with
+// its derivatives (g2g/cpu/functions.cpp:56-354): NV = 1 value, 4 adds the
+// gradient (LDA with forces), 10 also the Hessian (xx, yy, zz, xy, xz, yz).  Primitives with
 // a r^2 > 70 are skipped as in the reference.
 template <int NV>
 __device__ inline void eval_basis(const DevFn& b, double px, double py, double pz, double* o) {
@@ -147,6 +148,7 @@ __device__ inline void eval_basis(const DevFn& b, double px, double py, double p
   o[1] = Px * t + P * gx;
   o[2] = Py * t + P * gy;
   o[3] = Pz * t + P * gz;
+  if (NV == 4) return;
   const double Pxx = X.dd * Y.m * Z.m, Pyy = X.m * Y.dd * Z.m, Pzz = X.m * Y.m * Z.dd;
   const double Pxy = X.d * Y.d * Z.m, Pxz = X.d * Y.m * Z.d, Pyz = X.m * Y.d * Z.d;
   o[4] = Pxx * t + 2.0 * Px * gx + P * (4.0 * vx * vx * th - 2.0 * tg);
@@ -1154,6 +1156,214 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
   for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) out[e] = sF[e];
 }
 
+// ---------------------------------------------------------------------------
+// XC forces (computation types 2 and 3)
+// ---------------------------------------------------------------------------
+// LIO's XC force on nucleus A (g2g/cpu/iteration.cpp:213-268) is
+//   f_A = sum_p factor_p dd_p(A),  dd_p(A) = -2 sum_{j on A} Y_j(p) grad phi_j(p)
+// with Y = P phi (true symmetric P) and factor_p = w_p y2a_p from the density
+// launch.  Per 64-point block the force kernels recompute Y on the matrix unit
+// (one product instead of the density's four) and write, for every function
+// j of the group, gpart[j][a] = sum over the block's points of
+// factor_p Y_j(p) d_a phi_j(p): lanes of one function are added by a fixed
+// butterfly over the wave's 16 points, the four waves in wave order.
+// k_force_group sums a group's blocks in order and k_force_atom the
+// functions of each nucleus (ascending) over the groups holding them
+// (ascending), so the forces are as reproducible as the energy.
+
+// Adds c over the 16 lanes of a lane group (one function, 16 points) and
+// stores the sum of wave w at sG[w][jj] from the lane group's first lane.
+__device__ inline void force_lanes(double (&c)[3], int lrow, double* slot) {
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) c[a] += __shfl_xor(c[a], o);
+  }
+  if (lrow == 0) {
+    slot[0] = c[0];
+    slot[1] = c[1];
+    slot[2] = c[2];
+  }
+}
+
+// FP64 / FP32 values (planes [value][function][point]).
+template <class T>
+__global__ __launch_bounds__(256, 2) void k_force(const DensityItem* __restrict__ items,
+                                                  const DevGroup* __restrict__ groups,
+                                                  const T* __restrict__ pg,
+                                                  const T* __restrict__ vals,
+                                                  const double* __restrict__ factor,
+                                                  double* __restrict__ gpart) {
+  __shared__ double sG[4][kFnChunk][3];
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;  // uniform over the wave
+  const int np = g.npts_pad, mp = g.m_pad;
+  const long long plane = (long long)mp * np;
+  const T* V = vals + g.val_off;
+  const T* Pm = pg + g.pg_off;
+  const double f = active ? factor[g.pt_off + p0 + lrow] : 0.0;
+  double* out = gpart + g.force_off + (long long)(it.p0 / kDensPts) * mp * 3;
+  for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
+    const int nrt = min(4, (mp - c0) / 16);
+    typename Mma<T>::acc acc[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) acc[rt] = typename Mma<T>::acc{0, 0, 0, 0};
+    if (active) {
+      const T* Bk = V + (long long)lk * np + p0 + lrow;   // phi[k][p]
+      const T* Ak = Pm + (long long)lk * mp + c0 + lrow;  // P[k][j] = P[j][k]
+      for (int k0 = 0; k0 < mp; k0 += 4) {
+        const T b = Bk[(long long)k0 * np];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+          if (rt < nrt) Mma<T>::op(Ak[(long long)k0 * mp + 16 * rt], b, acc[rt]);
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int jj = 16 * rt + Mma<T>::row(lk, i);
+        double c[3] = {0.0, 0.0, 0.0};
+        if (active && rt < nrt) {
+          const T* vj = V + (long long)(c0 + jj) * np + p0 + lrow;
+          const double y = f * (double)acc[rt][i];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) c[a] = y * (double)vj[(a + 1) * plane];
+        }
+        force_lanes(c, lrow, sG[wave][jj]);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * kFnChunk) {
+      const int jj = threadIdx.x / 3, a = threadIdx.x % 3;
+      if (c0 + jj < mp)
+        out[(long long)(c0 + jj) * 3 + a] =
+            ((sG[0][jj][a] + sG[1][jj][a]) + sG[2][jj][a]) + sG[3][jj][a];
+    }
+    __syncthreads();
+  }
+}
+
+// Split-BF16 values (point-major FP32 planes, P_g as three BF16 pieces): Y on
+// the six-product BF16 MFMA chain as in the density kernels.
+__global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restrict__ items,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const unsigned short* __restrict__ pg3,
+                                                     const float* __restrict__ vals,
+                                                     const double* __restrict__ factor,
+                                                     double* __restrict__ gpart) {
+  __shared__ double sG[4][kFnChunk][3];
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;  // uniform over the wave
+  const int np = g.npts_pad, mp = g.m_pad;
+  const long long plane = (long long)mp * np;
+  const long long psz = (long long)mp * mp;
+  const float* Vp = vals + g.val_off + (long long)(active ? p0 + lrow : 0) * mp;
+  const unsigned short* Pp = pg3 + 3 * g.pg_off;
+  const double f = active ? factor[g.pt_off + p0 + lrow] : 0.0;
+  double* out = gpart + g.force_off + (long long)(it.p0 / kDensPts) * mp * 3;
+  for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
+    const int nrt = min(4, (mp - c0) / 16);
+    f4 acc[4];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) acc[rt] = f4{0, 0, 0, 0};
+    if (active) {
+      for (int k0 = 0; k0 < mp; k0 += 32) {
+        const int kk = k0 + 8 * lk;
+        bf8 bh, bm, bl;
+        float x[8];
+        load8(Vp + kk, x);
+        split3(x, bh, bm, bl);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          if (rt < nrt) {
+            const unsigned short* pa = Pp + (long long)(c0 + 16 * rt + lrow) * mp + kk;
+            const bf8 ah = *reinterpret_cast<const bf8*>(pa);
+            const bf8 am = *reinterpret_cast<const bf8*>(pa + psz);
+            const bf8 al = *reinterpret_cast<const bf8*>(pa + 2 * psz);
+            acc[rt] += mma6(ah, am, al, bh, bm, bl);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      fl4 v[3] = {fl4{0, 0, 0, 0}, fl4{0, 0, 0, 0}, fl4{0, 0, 0, 0}};
+      if (active && rt < nrt) {
+        const float* vj = Vp + c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = *reinterpret_cast<const fl4*>(vj + (a + 1) * plane);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int jj = 16 * rt + 4 * lk + i;
+        const double y = f * (double)acc[rt][i];
+        double c[3] = {y * (double)v[0][i], y * (double)v[1][i], y * (double)v[2][i]};
+        force_lanes(c, lrow, sG[wave][jj]);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * kFnChunk) {
+      const int jj = threadIdx.x / 3, a = threadIdx.x % 3;
+      if (c0 + jj < mp)
+        out[(long long)(c0 + jj) * 3 + a] =
+            ((sG[0][jj][a] + sG[1][jj][a]) + sG[2][jj][a]) + sG[3][jj][a];
+    }
+    __syncthreads();
+  }
+}
+
+// gsum[fn_off + j][a] = sum over the group's blocks (in order) of gpart; one
+// block per group.
+__global__ __launch_bounds__(256) void k_force_group(const DevGroup* __restrict__ groups,
+                                                     const double* __restrict__ gpart,
+                                                     double* __restrict__ gsum) {
+  const DevGroup g = groups[blockIdx.x];
+  const int nblk = (g.npts_pad + kDensPts - 1) / kDensPts;
+  const long long stride = 3LL * g.m_pad;
+  const double* src = gpart + g.force_off;
+  for (int e = threadIdx.x; e < 3 * g.m_pad; e += 256) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += src[b * stride + e];
+    gsum[3LL * g.fn_off + e] = s;
+  }
+}
+
+// Force on batch atom A (atom-major out[3 A + a]) = -2 sum over A's functions
+// (ascending) of the group sums of the groups holding them (ascending).
+__global__ __launch_bounds__(64) void k_force_atom(const int* __restrict__ afn_off,
+                                                   const int* __restrict__ afn,
+                                                   const int* __restrict__ memb_off,
+                                                   const int2* __restrict__ memb,
+                                                   const DevGroup* __restrict__ groups,
+                                                   const double* __restrict__ gsum, int natom,
+                                                   double* __restrict__ out) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= natom) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int q = afn_off[A]; q < afn_off[A + 1]; ++q) {
+    const int I = afn[q];
+    for (int k = memb_off[I]; k < memb_off[I + 1]; ++k) {
+      const int2 u = memb[k];
+      const double* gs = gsum + 3LL * (groups[u.x].fn_off + u.y);
+      s[0] += gs[0];
+      s[1] += gs[1];
+      s[2] += gs[2];
+    }
+  }
+  out[3 * A + 0] = -2.0 * s[0];
+  out[3 * A + 1] = -2.0 * s[1];
+  out[3 * A + 2] = -2.0 * s[2];
+}
+
 // F_g tile (ti, tj) = sum over slices, in slice order.  Block b of a tile
 // pair handles elements [256 (b % 16), +256) of the 64 x 64 tile.
 __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__ groups,
@@ -1425,6 +1635,7 @@ struct DeviceGrid {
   int* eidx = nullptr;
   bool gga = true;
   int ngroups = 0, neval = 0, ngather = 0, ndensity = 0, nfock = 0, ngroup_tiles = 0;
+  int nforce = 0;                        // force blocks (64 points each)
   long long npts_pad = 0, val_elems = 0, pg_elems = 0;
   char* arena = nullptr;
   size_t arena_bytes = 0;   // used by this grid
@@ -1456,6 +1667,13 @@ struct DeviceGrid {
   int* memb_off = nullptr;
   int2* memb = nullptr;
   double* eout = nullptr;
+  DensityItem* citems = nullptr;  // force blocks
+  double* gpart = nullptr;        // per block and function (3)
+  double* gsum = nullptr;         // per group and function (3)
+  int* afn_off = nullptr;         // functions of each batch atom
+  int* afn = nullptr;
+  double* fout = nullptr;         // forces, natom x 3 (atom-major)
+  std::vector<int> mol_atom;      // first batch atom of each molecule, then natom
   double* h_rho = nullptr;
   double* h_F = nullptr;
   double* h_e = nullptr;
@@ -1565,7 +1783,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
-  dg->nval = dg->gga ? 10 : 1;
+  // LDA keeps the gradient too: the XC forces need it
+  dg->nval = dg->gga ? 10 : 4;
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const bool bf = prec == Precision::BF16X6;
@@ -1576,6 +1795,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   // batch numbering: molecule k owns functions [fn_base, fn_base + M) and
   // packed entries [rho_off, rho_off + M (M + 1) / 2)
   std::vector<DevMol> hmol(nmol);
+  dg->mol_atom.assign(nmol + 1, 0);
+  for (int k = 0; k < nmol; ++k) dg->mol_atom[k + 1] = dg->mol_atom[k] + mols[k]->natom;
   int M = 0, G = 0;
   long long packed = 0;
   for (int k = 0; k < nmol; ++k) {
@@ -1597,6 +1818,17 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
 
   std::vector<DevFn> bas(M);
   std::vector<int> fn_mol(M);
+  // functions of each batch atom, ascending
+  std::vector<int> afn_off(dg->natom + 1, 0), afn(M);
+  for (int k = 0; k < nmol; ++k)
+    for (int f = 0; f < mols[k]->M; ++f) ++afn_off[dg->mol_atom[k] + mols[k]->fn[f].atom + 1];
+  for (int a = 0; a < dg->natom; ++a) afn_off[a + 1] += afn_off[a];
+  {
+    std::vector<int> fill(afn_off.begin(), afn_off.end() - 1);
+    for (int k = 0; k < nmol; ++k)
+      for (int f = 0; f < mols[k]->M; ++f)
+        afn[fill[dg->mol_atom[k] + mols[k]->fn[f].atom]++] = hmol[k].fn_base + f;
+  }
   for (int k = 0; k < nmol; ++k) {
     const Molecule& mol = *mols[k];
     for (int f = 0; f < mol.M; ++f) {
@@ -1617,7 +1849,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   std::vector<DevGroup> groups(G);
   std::vector<int> fnl;
   std::vector<double> hx, hy, hz, hw;
-  long long pt = 0, vo = 0, po = 0, fo = 0;
+  long long pt = 0, vo = 0, po = 0, fo = 0, go = 0;
   int fock_base = 0;
   std::vector<std::vector<int2>> membership(M);
   for (int g = 0, k = 0, gk = 0; g < G; ++g, ++gk) {
@@ -1643,6 +1875,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     d.fock_base = fock_base;
     d.pad_ = 0;
     d.phi_off = fo;
+    d.force_off = go;
+    go += (long long)(d.npts_pad + kDensPts - 1) / kDensPts * d.m_pad * 3;
     fock_base += d.ntile * (d.ntile + 1) / 2 * d.nslice;
     for (int i = 0; i < d.m_pad; ++i) {
       const int f = i < m ? hmol[k].fn_base + part.fn[part.fn_off[gk] + i] : -1;
@@ -1676,9 +1910,10 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   // split-BF16 variant 3: groups of more than 64 functions run the row-split
   // kernel on 16-point blocks (listed first), the rest 64-point blocks
   const bool rowsplit = bf && (tuning_.density_variant == 3 || tuning_.density_variant == 4);
-  std::vector<DensityItem> ditems, dsmall;
+  std::vector<DensityItem> ditems, dsmall, citems;
   for (int g : order) {
     const DevGroup& d = groups[g];
+    for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) citems.push_back(DensityItem{g, p0});
     for (int k = 0; k < d.m_pad; ++k)
       for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalPts) eitems.push_back(EvalItem{g, k, p0, 0});
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
@@ -1717,6 +1952,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->ngather = (int)gitems.size();
   dg->npm = (int)pmitems.size();
   dg->ndensity = (int)ditems.size();
+  dg->nforce = (int)citems.size();
   dg->ngroup_tiles = (int)gtiles.size();
   std::vector<int> moff(M + 1, 0);
   std::vector<int2> memb;
@@ -1755,6 +1991,12 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->fn_mol = c.take<int>(M);
     dg->eidx = c.take<int>(eidx.size() + 1);
     dg->eout = c.take<double>(2 * nmol);
+    dg->citems = c.take<DensityItem>(citems.size());
+    dg->gpart = c.take<double>(go);
+    dg->gsum = c.take<double>(3 * fnl.size());
+    dg->afn_off = c.take<int>(afn_off.size());
+    dg->afn = c.take<int>(afn.size());
+    dg->fout = c.take<double>(3 * (size_t)dg->natom);
   };
   Carver sizing{nullptr};
   plan(sizing);
@@ -1824,6 +2066,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->mols, hmol.data(), hmol.size() * sizeof(DevMol));
   up(dg->fn_mol, fn_mol.data(), fn_mol.size() * sizeof(int));
   up(dg->eidx, eidx.data(), eidx.size() * sizeof(int));
+  up(dg->citems, citems.data(), citems.size() * sizeof(DensityItem));
+  up(dg->afn_off, afn_off.data(), afn_off.size() * sizeof(int));
+  up(dg->afn, afn.data(), afn.size() * sizeof(int));
   if (dpts) {  // points come from the device rebuild: compact them into place
     if ((int)dpts->src.size() != G) throw std::runtime_error("g2g: staged group map mismatch");
     std::vector<CompactItem> ci(G);
@@ -1849,7 +2094,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
         hipLaunchKernelGGL((k_eval_basis<10, double>), grid, block, 0, s, dg->eitems, dg->groups,
                            dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
       else
-        hipLaunchKernelGGL((k_eval_basis<1, double>), grid, block, 0, s, dg->eitems, dg->groups,
+        hipLaunchKernelGGL((k_eval_basis<4, double>), grid, block, 0, s, dg->eitems, dg->groups,
                            dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
     } else if (prec == Precision::FP32) {
       float* v = static_cast<float*>(dg->vals);
@@ -1857,7 +2102,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
         hipLaunchKernelGGL((k_eval_basis<10, float>), grid, block, 0, s, dg->eitems, dg->groups,
                            dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
       else
-        hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
+        hipLaunchKernelGGL((k_eval_basis<4, float>), grid, block, 0, s, dg->eitems, dg->groups,
                            dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
     } else {
       float* v = static_cast<float*>(dg->vals);
@@ -1866,7 +2111,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
           hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
         else
-          hipLaunchKernelGGL(k_eval_pm<1>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
+          hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
       }
       hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
@@ -2152,10 +2397,36 @@ static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, b
   XCQ_HIP(hipGetLastError());
 }
 
+// XC forces after the density launch (P_g and factor in place): per-block
+// partials, group sums, per-atom sums into d_forces (natom x 3, atom-major).
+static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
+  if (g.nforce > 0) {
+    if (g.prec == Precision::FP64)
+      hipLaunchKernelGGL(k_force<double>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                         static_cast<const double*>(g.pg), static_cast<const double*>(g.vals),
+                         g.factor, g.gpart);
+    else if (g.prec == Precision::FP32)
+      hipLaunchKernelGGL(k_force<float>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                         static_cast<const float*>(g.pg), static_cast<const float*>(g.vals),
+                         g.factor, g.gpart);
+    else
+      hipLaunchKernelGGL(k_force_bf, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                         static_cast<const unsigned short*>(g.pg),
+                         static_cast<const float*>(g.vals), g.factor, g.gpart);
+  }
+  if (g.ngroups > 0)
+    hipLaunchKernelGGL(k_force_group, dim3(g.ngroups), dim3(256), 0, s, g.groups, g.gpart, g.gsum);
+  if (g.natom > 0)
+    hipLaunchKernelGGL(k_force_atom, dim3((g.natom + 63) / 64), dim3(64), 0, s, g.afn_off, g.afn,
+                       g.memb_off, g.memb, g.groups, g.gsum, g.natom, d_forces);
+  XCQ_HIP(hipGetLastError());
+}
+
 // The whole build: gather, density + functional, Fock, packed reduction,
-// per-molecule energies.  Every launch covers all molecules of the batch.
+// per-molecule energies, and the XC forces when d_forces is given.  Every
+// launch covers all molecules of the batch.
 static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double* d_fock,
-                       hipEvent_t* ev) {
+                       hipEvent_t* ev, double* d_forces = nullptr) {
   mark(ev, 0, s);
   if (g.prec == Precision::FP64)
     launch_solve<double>(g, s, d_rho, d_fock != nullptr, ev);
@@ -2169,12 +2440,15 @@ static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double
                        g.memb, g.groups, g.mols, g.fn_mol, g.fg, d_fock);
   hipLaunchKernelGGL(k_energy, dim3(g.nmol), dim3(1024), 0, s, g.epart, g.mols, g.eidx, g.eout);
   XCQ_HIP(hipGetLastError());
+  if (d_forces) launch_forces(g, s, d_forces);
   mark(ev, 4, s);
 }
 
-SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy) {
+SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy,
+                                 double* d_forces) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
-  launch_all(*grid_, (hipStream_t)stream_, d_rho, d_fock, reinterpret_cast<hipEvent_t*>(events_));
+  launch_all(*grid_, (hipStream_t)stream_, d_rho, d_fock, reinterpret_cast<hipEvent_t*>(events_),
+             d_forces);
   (void)want_energy;
   return SolveResult{};
 }
@@ -2241,6 +2515,11 @@ void Engine::read_energies(double* out) {
 
 int Engine::molecules() const { return grid_ ? grid_->nmol : 0; }
 
+int Engine::atoms(int k) const {
+  if (!grid_ || k < 0 || k > grid_->nmol) throw std::runtime_error("g2g: molecule out of range");
+  return grid_->mol_atom[k];
+}
+
 long long Engine::packed_offset(int k) const {
   if (!grid_ || k < 0 || k > grid_->nmol) throw std::runtime_error("g2g: molecule out of range");
   return grid_->mol_packed[k];
@@ -2288,22 +2567,33 @@ SolveResult Engine::collect(double* fock_accum) {
   return res;
 }
 
-SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool want_energy) {
+SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool want_energy,
+                          double* forces_accum, int forces_ld) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
   const double t0 = wall();
   DeviceGrid& g = *grid_;
   hipStream_t s = (hipStream_t)stream_;
   XCQ_HIP(hipSetDevice(device_));
+  if (forces_accum && forces_ld < g.natom)
+    throw std::runtime_error("g2g: the force array's leading dimension is below the atom count");
   const size_t packed = (size_t)g.packed;
   std::memcpy(g.h_rho, rho_packed, packed * sizeof(double));
   XCQ_HIP(hipMemcpyAsync(g.rho, g.h_rho, packed * sizeof(double), hipMemcpyHostToDevice, s));
-  SolveResult res = solve_device(g.rho, fock_accum ? g.F : nullptr, want_energy);
+  SolveResult res = solve_device(g.rho, fock_accum ? g.F : nullptr, want_energy,
+                                 forces_accum ? g.fout : nullptr);
   if (fock_accum)
     XCQ_HIP(hipMemcpyAsync(g.h_F, g.F, packed * sizeof(double), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipMemcpyAsync(g.h_e, g.eout, 2 * g.nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  std::vector<double> f(forces_accum ? 3 * (size_t)g.natom : 0);
+  if (forces_accum)
+    XCQ_HIP(hipMemcpyAsync(f.data(), g.fout, f.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipStreamSynchronize(s));
   if (fock_accum)
     for (size_t k = 0; k < packed; ++k) fock_accum[k] += g.h_F[k];
+  // Fortran layout of the caller's forces: column c of atom a at c * ld + a
+  if (forces_accum)
+    for (int a = 0; a < g.natom; ++a)
+      for (int c = 0; c < 3; ++c) forces_accum[(size_t)c * forces_ld + a] += f[3 * (size_t)a + c];
   res.energy = g.h_e[0];
   res.electrons = g.h_e[1];
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);

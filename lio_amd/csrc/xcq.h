@@ -192,16 +192,22 @@ class Engine {
   void download_points(double* x, double* y, double* z, double* w);
   long long padded_points() const;
   // One XC Fock build from packed rho (host memory).  fock_accum (host) gets
-  // += F_xc; pass nullptr to skip the copy-back.
-  SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy);
+  // += F_xc; pass nullptr to skip the Fock matrix.  forces_accum (host,
+  // Fortran layout: component c of atom a at c * forces_ld + a) gets += the
+  // XC forces of g2g/cpu/iteration.cpp:213-268; nullptr skips them.
+  SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy,
+                    double* forces_accum = nullptr, int forces_ld = 0);
   // Queues a build from a packed host density (copied to pinned memory at
   // once) with its copy-back; idle() tells when it has finished and
   // collect() waits for it and adds F onto fock_accum (may be null).
   void solve_async(const double* rho_packed);
   bool idle() const;
   SolveResult collect(double* fock_accum);
-  // Same with device-resident input/output (no PCIe copies), used for timing.
-  SolveResult solve_device(const double* d_rho_packed, double* d_fock, bool want_energy);
+  // Same with device-resident input/output (no PCIe copies).  d_fock (may be
+  // null) receives F_xc; d_forces (may be null) the forces of every batch
+  // atom, atom-major (3 a + c), molecule k's atoms from atoms(k).
+  SolveResult solve_device(const double* d_rho_packed, double* d_fock, bool want_energy,
+                           double* d_forces = nullptr);
   // Waits for the stream and reads the last solve's energy and electron count
   // (of the first molecule); read_energies: (Exc, N) of every molecule.
   void read_energy(double* energy, double* electrons);
@@ -212,6 +218,7 @@ class Engine {
   void replay_graph();
   bool has_graph() const { return graph_ != nullptr; }
   int molecules() const;
+  int atoms(int k) const;  // first batch atom of molecule k; k = molecules() gives the total
   long long packed_offset(int k) const;  // k = molecules() gives the total
   // Waits for the stream; HIP-event durations (ms) of the last solve's phases.
   void kernel_times(double ms[4]);

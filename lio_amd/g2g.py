@@ -89,6 +89,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_groups.argtypes = [vp, C.POINTER(XcqGroupInfo), C.c_int, C.POINTER(C.c_int)]
     L.xcq_solve.argtypes = [vp, _dp, _dp, C.POINTER(XcqSolveInfo)]
     L.xcq_solve_device.argtypes = [vp, vp, vp]
+    L.xcq_solve_forces.argtypes = [vp, _dp, _dp, _dp, C.c_int, C.POINTER(XcqSolveInfo)]
+    L.xcq_batch_solve_forces.argtypes = [vp, vp, vp, vp]
+    L.xcq_batch_atoms.argtypes = [vp, C.POINTER(C.c_int)]
     L.xcq_sync_energy.argtypes = [vp, _dp, _dp]
     L.xcq_kernel_times.argtypes = [vp, _dp]
     L.xcq_stream.argtypes = [vp]
@@ -239,6 +242,22 @@ class Context:
         return info.energy, F, dict(electrons=info.electrons, seconds=info.seconds,
                                     kernel_ms=list(info.kernel_ms))
 
+    def solve_forces(self, rho=None, fock=None, ld=None, out=None):
+        """Energy and XC forces (g2g computation type 2; the Fock matrix too
+        when ``fock`` is given); returns (energy, forces (natom, 3), info).
+        The library adds onto a Fortran-layout array of leading dimension
+        ``ld`` (default natom): ``out`` (3 * ld values) when given."""
+        rho = self.system.rho_packed if rho is None else rho
+        rho = np.ascontiguousarray(rho, np.float64)
+        n = self.system.natom
+        ld = n if ld is None else ld
+        f = np.zeros(3 * ld) if out is None else out
+        info = XcqSolveInfo()
+        fp = None if fock is None else _ptr(fock, C.c_double)
+        _check(self.L.xcq_solve_forces(self.h, _ptr(rho, C.c_double), fp, _ptr(f, C.c_double), ld,
+                                       C.byref(info)), "xcq_solve_forces")
+        return info.energy, f.reshape(3, ld)[:, :n].T.copy(), dict(electrons=info.electrons)
+
     def solve_device(self, d_rho: int, d_fock: int):
         """Queues a solve on device pointers (integers); no synchronisation."""
         _check(self.L.xcq_solve_device(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),
@@ -360,10 +379,13 @@ class G2G:
         """The caller-owned packed Fock the library accumulates into."""
         return self._arr["fock"]
 
-    def solve(self, computation_type: int = 0) -> float:
+    def solve(self, computation_type: int = 0, forces=None) -> float:
+        """g2g_solve_groups_(computation_type, energy, forces): ``forces`` is
+        the caller's (max_atoms x 3, Fortran order) array for types 2 and 3."""
         e = np.zeros(1)
         ct = np.array([computation_type], np.uint32)
-        self.L.g2g_solve_groups_(_ptr(ct, C.c_uint), _ptr(e, C.c_double), None)
+        fp = None if forces is None else _ptr(forces, C.c_double)
+        self.L.g2g_solve_groups_(_ptr(ct, C.c_uint), _ptr(e, C.c_double), fp)
         return float(e[0])
 
     def deinit(self):
@@ -413,6 +435,18 @@ class Batch:
     def solve_device(self, d_rho: int, d_fock: int):
         _check(self.L.xcq_batch_solve_device(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),
                "xcq_batch_solve_device")
+
+    def solve_forces(self, d_rho: int, d_fock: int, d_forces: int):
+        """Full build plus the forces of all batch atoms (device, atom-major)."""
+        _check(self.L.xcq_batch_solve_forces(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock),
+                                             C.c_void_p(d_forces)), "xcq_batch_solve_forces")
+
+    def atoms(self):
+        """First batch atom of each molecule, then the total."""
+        a = np.zeros(self.n + 1, np.int32)
+        _check(self.L.xcq_batch_atoms(self.h, a.ctypes.data_as(C.POINTER(C.c_int))),
+               "xcq_batch_atoms")
+        return a
 
     def capture(self, d_rho: int, d_fock: int):
         _check(self.L.xcq_batch_capture(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),

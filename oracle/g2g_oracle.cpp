@@ -12,6 +12,7 @@
 //   basis values/derivs ..... g2g/cpu/functions.cpp:14-359 (restated as the
 //                             general derivative of x^l y^m z^n g(r))
 //   density + Fock .......... g2g/cpu/iteration.cpp:39-309 (closed shell)
+//   XC forces ............... g2g/cpu/iteration.cpp:213-268
 //   packed P / F I/O ........ g2g/partition.cpp:77-219
 //   PBE ..................... g2g/pointxc/calc_ggaCS.h:35-160, pbeCS.h:29-185,
 //                             gcorc.h:21-33 (PBE 1996 with the PW92 fit)
@@ -660,9 +661,10 @@ inline size_t packed_index(int i, int j, int M) {
 }
 
 void solve_group(const Problem& pb, const Group& g, const double* rho, double& energy,
-                 double& electrons, double* F) {
+                 double& electrons, double* F, double* forces) {
   const int m = g.m(), n = (int)g.pts.size(), M = pb.M;
-  const int nv = pb.gga ? 10 : 1;
+  // the gradient is also needed for LDA forces (cpu/functions.cpp:25-32)
+  const int nv = pb.gga ? 10 : (forces ? 4 : 1);
   // values: [point][value][function]
   std::vector<double> fv((size_t)n * nv * m);
   for (int p = 0; p < n; ++p)
@@ -728,6 +730,31 @@ void solve_group(const Problem& pb, const Group& g, const double* rho, double& e
     electrons += dens * wp;
     factor[p] = wp * y2a;
   }
+  // XC forces (cpu/iteration.cpp:213-268): per point and nucleus,
+  // dd = -sum_{i on the nucleus} w_i grad phi_i with w_i = sum_j P'_ji phi_j
+  // (x2 on the diagonal), scaled by the point's factor; summed over points in
+  // order, then added to the nucleus' row of the (natom x 3) force array
+  if (forces) {
+    std::vector<double> fsum((size_t)g.nucs.size() * 3, 0.0);
+    std::vector<double> dd(fsum.size());
+    for (int p = 0; p < n; ++p) {
+      const double* f = &fv[(size_t)p * nv * m];
+      std::fill(dd.begin(), dd.end(), 0.0);
+      for (int i = 0; i < m; ++i) {
+        double w = 0.0;
+        for (int j = 0; j < m; ++j) w += Pl[(size_t)j * m + i] * f[j] * (i == j ? 2 : 1);
+        const int a = (int)(std::lower_bound(g.nucs.begin(), g.nucs.end(),
+                                             pb.nuc[g.fglobal[i]]) - g.nucs.begin());
+        dd[3 * a + 0] -= w * f[m + i];
+        dd[3 * a + 1] -= w * f[2 * m + i];
+        dd[3 * a + 2] -= w * f[3 * m + i];
+      }
+      for (size_t k = 0; k < dd.size(); ++k) fsum[k] += dd[k] * factor[p];
+    }
+    for (size_t a = 0; a < g.nucs.size(); ++a)
+      for (int c = 0; c < 3; ++c) forces[3 * (size_t)g.nucs[a] + c] += fsum[3 * a + c];
+  }
+  if (!F) return;
   // Fock: F_big += sum_p phi_i phi_j factor_p over local pairs i <= j
   for (int i = 0; i < m; ++i)
     for (int j = i; j < m; ++j) {
@@ -741,20 +768,23 @@ void solve_group(const Problem& pb, const Group& g, const double* rho, double& e
 }
 
 double solve_range(Handle& h, const double* rho_in, int first, int count, double* energy,
-                   double* F, double* electrons, double* points) {
+                   double* F, double* electrons, double* points, double* forces = nullptr) {
   const Problem& pb = h.pb;
   const double* rho = rho_in ? rho_in : pb.rho.data();
   const size_t packed = (size_t)pb.M * (pb.M + 1) / 2;
+  const size_t nforce = forces ? 3 * (size_t)pb.natom : 0;
   const int last = std::min<int>((int)h.groups.size(), first + count);
   double e = 0.0, ne = 0.0, pts = 0.0;
   const double t0 = now();
 #pragma omp parallel reduction(+ : e, ne, pts)
   {
     std::vector<double> Fl(F ? packed : 0, 0.0);
+    std::vector<double> fl(nforce, 0.0);
 #pragma omp for schedule(dynamic, 1)
     for (int gi = first; gi < last; ++gi) {
       double ge = 0.0, gn = 0.0;
-      solve_group(pb, h.groups[gi], rho, ge, gn, F ? Fl.data() : nullptr);
+      solve_group(pb, h.groups[gi], rho, ge, gn, F ? Fl.data() : nullptr,
+                  forces ? fl.data() : nullptr);
       e += ge;
       ne += gn;
       pts += (double)h.groups[gi].pts.size();
@@ -762,6 +792,10 @@ double solve_range(Handle& h, const double* rho_in, int first, int count, double
     if (F) {
 #pragma omp critical
       for (size_t k = 0; k < packed; ++k) F[k] += Fl[k];
+    }
+    if (forces) {
+#pragma omp critical
+      for (size_t k = 0; k < nforce; ++k) forces[k] += fl[k];
     }
   }
   const double t1 = now();
@@ -799,6 +833,15 @@ double oracle_solve(void* hp, const double* rho, double* energy, double* fock,
                     double* electrons) {
   auto* h = static_cast<Handle*>(hp);
   return solve_range(*h, rho, 0, (int)h->groups.size(), energy, fock, electrons, nullptr);
+}
+
+// Energy and XC forces (computation type 2, no Fock): energy assigned,
+// forces (natom x 3, atom-major) accumulated (+=), electrons assigned.
+double oracle_solve_forces(void* hp, const double* rho, double* energy, double* forces,
+                           double* electrons) {
+  auto* h = static_cast<Handle*>(hp);
+  return solve_range(*h, rho, 0, (int)h->groups.size(), energy, nullptr, electrons, nullptr,
+                     forces);
 }
 
 // Bounded sample for CPU timing: groups [first, first+count) of the cost-sorted

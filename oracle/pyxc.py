@@ -111,6 +111,8 @@ class RefG2G(_Checker):
         L.ref_setup.restype = C.c_double
         L.ref_solve.argtypes = [_dp, _dp, _dp]
         L.ref_solve.restype = C.c_double
+        L.ref_solve_ct.argtypes = [C.c_uint, _dp, _dp, _dp, _dp]
+        L.ref_solve_ct.restype = C.c_double
         L.ref_electrons.restype = C.c_double
         L.ref_groups.argtypes = [C.POINTER(XcGroupInfo), C.c_int]
         L.ref_groups.restype = C.c_int
@@ -118,6 +120,7 @@ class RefG2G(_Checker):
     def setup(self, system, **options):
         self._inp, self._keep = make_input(system, **options)
         self.M = system.M
+        self.natom = system.natom
         return self.lib.ref_setup(C.byref(self._inp))
 
     def solve(self, rho=None):
@@ -126,6 +129,14 @@ class RefG2G(_Checker):
         rp = None if rho is None else _ptr(np.ascontiguousarray(rho, np.float64), C.c_double)
         t = self.lib.ref_solve(rp, C.byref(e), _ptr(F, C.c_double))
         return e.value, F, t
+
+    def solve_forces(self, rho=None, computation_type=2):
+        """(energy, forces (natom, 3), seconds) from g2g_solve_groups_(2 or 3)."""
+        e = C.c_double(0.0)
+        f = np.zeros(3 * self.natom)
+        rp = None if rho is None else _ptr(np.ascontiguousarray(rho, np.float64), C.c_double)
+        t = self.lib.ref_solve_ct(computation_type, rp, C.byref(e), None, _ptr(f, C.c_double))
+        return e.value, f.reshape(3, self.natom).T.copy(), t
 
     def electrons(self):
         return self.lib.ref_electrons()
@@ -147,6 +158,8 @@ class Oracle(_Checker):
         L.oracle_setup.restype = C.c_void_p
         L.oracle_solve.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.oracle_solve.restype = C.c_double
+        L.oracle_solve_forces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.oracle_solve_forces.restype = C.c_double
         L.oracle_solve_sample.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
         L.oracle_solve_sample.restype = C.c_double
         L.oracle_groups.argtypes = [C.c_void_p, C.POINTER(XcGroupInfo), C.c_int]
@@ -178,6 +191,7 @@ class Oracle(_Checker):
         self.free()
         self._inp, self._keep = make_input(system, **options)
         self.M = system.M
+        self.natom = system.natom
         self.h = self.lib.oracle_setup(C.byref(self._inp))
         if not self.h:
             raise RuntimeError("oracle_setup failed")
@@ -191,6 +205,16 @@ class Oracle(_Checker):
         t = self.lib.oracle_solve(self.h, rp, C.byref(e), _ptr(F, C.c_double), C.byref(n))
         self._nelec = n.value
         return e.value, F, t
+
+    def solve_forces(self, rho=None):
+        """(energy, forces (natom, 3), seconds): computation type 2, no Fock."""
+        e = C.c_double(0.0)
+        n = C.c_double(0.0)
+        f = np.zeros(3 * self.natom)
+        rp = None if rho is None else _ptr(np.ascontiguousarray(rho, np.float64), C.c_double)
+        t = self.lib.oracle_solve_forces(self.h, rp, C.byref(e), _ptr(f, C.c_double), C.byref(n))
+        self._nelec = n.value
+        return e.value, f.reshape(self.natom, 3), t
 
     def solve_sample(self, first, count, rho=None):
         """Solves only groups [first, first+count) of the cost-sorted list; returns

@@ -142,6 +142,23 @@ double ref_solve(const double* rho, double* energy, double* fock_out) {
   return t1 - t0;
 }
 
+// computation_type ct (0 Fock, 1 energy, 2 energy + forces, 3 forces) with
+// energy_all_iterations set: energy assigned, fock_out receives the packed
+// Fock accumulated onto zeros, forces_out the (natom x 3, column-major,
+// leading dimension natom) forces accumulated onto zeros.  Returns seconds.
+double ref_solve_ct(unsigned ct, const double* rho, double* energy, double* fock_out,
+                    double* forces_out) {
+  if (rho) std::memcpy(S.rho.data(), rho, S.rho.size() * sizeof(double));
+  std::fill(S.fock.begin(), S.fock.end(), 0.0);
+  std::vector<double> forces(3 * (size_t)S.natom, 0.0);
+  double t0 = now();
+  g2g_solve_groups_(ct, energy, forces.data());
+  double t1 = now();
+  if (fock_out) std::memcpy(fock_out, S.fock.data(), S.fock.size() * sizeof(double));
+  if (forces_out) std::memcpy(forces_out, forces.data(), forces.size() * sizeof(double));
+  return t1 - t0;
+}
+
 // Integrated electron count sum_p w_p rho(r_p) over the reference's groups,
 // using the function values the CPU path keeps after a solve.
 double ref_electrons(void) {

@@ -3,8 +3,8 @@
 Runs the reference harness (oracle/_ref/libg2g_ref.so, built from the LIO
 sources by `make -C oracle ref`) on seeded systems from lio_amd.systems and
 stores, per case: the XC energy, the integrated electron count, the packed
-XC Fock (sparse: exact zeros are pairs of functions that share no group) and
-the group inventory.  The inputs are regenerated from the recorded seeds; a
+XC Fock (sparse: exact zeros are pairs of functions that share no group), the
+XC forces of computation type 2 (natom x 3) and the group inventory.  The inputs are regenerated from the recorded seeds; a
 checksum of the packed density guards against drift.
 
     python tests/golden/make_golden.py          (takes about a minute)
@@ -50,19 +50,20 @@ def main():
         E, F, t = ref.solve()
         N = ref.electrons()
         groups = ref.groups()
+        Ef, forces, _ = ref.solve_forces(computation_type=2)
         nz = np.nonzero(F)[0]
         out = dict(
             energy=E, electrons=N, M=s.M, natom=s.natom, iexch=s.iexch,
             rho_sum=float(s.rho_packed.sum()), rho_norm=float(np.linalg.norm(s.rho_packed)),
             fock_index=nz.astype(np.int64), fock_value=F[nz],
-            solve_seconds=t,
+            solve_seconds=t, forces=forces, energy_forces=Ef,
         )
         for f in GROUP_FIELDS:
             out["g_" + f] = np.array([g[f] for g in groups],
                                      dtype=np.uint64 if f == "fhash" else None)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
-        print("%-18s M=%4d groups=%4d Exc=% .12f N=%.9f nnz(F)=%d" %
-              (name, s.M, len(groups), E, N, len(nz)), flush=True)
+        print("%-18s M=%4d groups=%4d Exc=% .12f N=%.9f nnz(F)=%d max|f|=%.6f" %
+              (name, s.M, len(groups), E, N, len(nz), np.abs(forces).max()), flush=True)
     ref.teardown()
 
 

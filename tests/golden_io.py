@@ -33,7 +33,8 @@ def load(name: str):
         groups.append({f: (int(z["g_" + f][k]) if f != "wsum" and f != "xsum" else float(z["g_" + f][k]))
                        for f in GROUP_FIELDS})
     return dict(energy=float(z["energy"]), electrons=float(z["electrons"]), fock=F, M=M,
-                groups=groups, rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]))
+                groups=groups, rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]),
+                forces=np.array(z["forces"]), energy_forces=float(z["energy_forces"]))
 
 
 def names():

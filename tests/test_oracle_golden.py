@@ -21,6 +21,13 @@ def _compare(E, F, N, gold):
     assert np.max(np.abs(F - gold["fock"])) <= REL * max(1.0, np.max(np.abs(gold["fock"])))
 
 
+def _compare_forces(o, gold):
+    """XC forces of computation type 2 (cpu/iteration.cpp:213-268)."""
+    E, f, _ = o.solve_forces()
+    assert abs(E - gold["energy_forces"]) <= REL * abs(gold["energy_forces"])
+    assert np.max(np.abs(f - gold["forces"])) <= REL * max(1.0, np.max(np.abs(gold["forces"])))
+
+
 @pytest.mark.parametrize("name", ["c0_pbe", "c0_lda1", "c0_lda2", "c0_lda3", "mol3_s101_pbe",
                                   "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe"])
 def test_oracle_matches_golden(name):
@@ -32,6 +39,7 @@ def test_oracle_matches_golden(name):
     assert same_partition(o.groups(), gold["groups"])
     E, F, _ = o.solve()
     _compare(E, F, o.electrons(), gold)
+    _compare_forces(o, gold)
 
 
 def test_oracle_matches_golden_water_cluster():
@@ -44,6 +52,7 @@ def test_oracle_matches_golden_water_cluster():
     assert same_partition(o.groups(), gold["groups"])
     E, F, _ = o.solve()
     _compare(E, F, o.electrons(), gold)
+    _compare_forces(o, gold)
 
 
 needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO),
@@ -66,9 +75,12 @@ def test_oracle_matches_reference_options(case):
     E_r, F_r, _ = ref.solve()
     N_r = ref.electrons()
     g_r = ref.groups()
+    _, f_r, _ = ref.solve_forces(computation_type=3)
     ref.teardown()
     o = Oracle()
     o.setup(s, **case["opts"])
+    _, f, _ = o.solve_forces()
+    assert np.max(np.abs(f - f_r)) <= REL * max(1.0, np.max(np.abs(f_r)))
     E, F, _ = o.solve()
     assert same_partition(o.groups(), g_r)
     assert abs(E - E_r) <= REL * abs(E_r)

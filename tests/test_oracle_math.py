@@ -221,3 +221,43 @@ def test_solve_equals_brute_force_contraction(oracle, iexch):
     assert abs(N - N_bf) < 1e-10 * N
     assert np.max(np.abs(Fp - F_packed)) < 1e-10
     assert abs(N - 2 * s.nocc) < 1e-3  # quadrature of tr(P S)
+
+
+@pytest.mark.parametrize("iexch", [1, 2])
+def test_lda_forces_are_energy_derivatives(oracle, iexch):
+    """With the grid held fixed, LIO's XC force on atom A is
+    sum_p w_p v(rho_p) d rho_p / d R_A, the derivative of Exc with respect to
+    the centres of A's basis functions; for the LDAs whose potential is the
+    exact d(rho eps)/d rho it must equal central differences of the
+    brute-force energy (functions moved, points and weights not)."""
+    import dataclasses
+    s = systems.config0()
+    s.iexch = iexch
+    oracle.setup(s)
+    _, forces, _ = oracle.solve_forces()
+    xyzw, grp, flists = oracle.grid()
+    P = s.P
+
+    def energy(sys_):
+        vals = oracle.basis_values(sys_, xyzw[:, :3])
+        E = 0.0
+        for g, fl in enumerate(flists):
+            sel = grp == g
+            phi = vals[sel][:, fl, 0]
+            rho = (phi * (phi @ P[np.ix_(fl, fl)])).sum(1)
+            ex, ec, _ = oracle.lda_points(rho, iexch)
+            E += (rho * xyzw[sel, 3] * (ex + ec)).sum()
+        return E
+
+    h = 1e-4
+    fd = np.zeros_like(forces)
+    for a in range(s.natom):
+        for c in range(3):
+            pp, pm = s.pos.copy(), s.pos.copy()
+            pp[a, c] += h
+            pm[a, c] -= h
+            fd[a, c] = (energy(dataclasses.replace(s, pos=pp)) -
+                        energy(dataclasses.replace(s, pos=pm))) / (2 * h)
+    assert np.max(np.abs(forces - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
+    # a plausible slip (a dropped factor 2 or a sign) fails by far more
+    assert np.max(np.abs(fd)) > 1e-2
