@@ -180,5 +180,183 @@ __device__ __host__ inline void lda_point(double rho, int iexch, double& ex, dou
   y2a = v0 + vcl;
 }
 
+
+// ---------------------------------------------------------------------------
+// Open shell (spin-polarised) PBE in the same Laplacian potential form
+// (g2g/pointxc/calc_ggaOS.h:10-108, pbeOS_main.h:24-202, pbeOS_exch.h:36-72,
+// pbeOS_corr.h:54-247): exchange by exact spin scaling
+// Ex[a, b] = (Ex[2a] + Ex[2b]) / 2, correlation from the PW92 interpolation
+// between the unpolarised, fully polarised and spin-stiffness fits with the
+// PBE gradient term H(rs, zeta, t) and its potential (Perdew, Burke & Wang,
+// appendix E).  Constants are the reference's float literals widened to
+// double; its float quotients 4/3 and -1/6 are kept as such.
+// ---------------------------------------------------------------------------
+struct PwFit {
+  double a, a1, b1, b2, b3, b4;
+};
+
+// G(rs) of Perdew & Wang (1992) eq. (10) and dG/drs, from sqrt(rs).
+__device__ __host__ inline void pw92_fit(const PwFit& f, double srs, double& e, double& de_drs) {
+  const double pre = -2.0 * f.a * (1.0 + f.a1 * srs * srs);
+  const double den = 2.0 * f.a * srs * (f.b1 + srs * (f.b2 + srs * (f.b3 + f.b4 * srs)));
+  const double lg = log(1.0 + 1.0 / den);
+  e = pre * lg;
+  const double dden = f.a * (f.b1 / srs + 2.0 * f.b2 + srs * (3.0 * f.b3 + 4.0 * f.b4 * srs));
+  de_drs = -2.0 * f.a * f.a1 * lg - pre * dden / (den * (1.0 + den));
+}
+
+// Exchange per particle and potential of one spin channel, evaluated at the
+// doubled density n = 2 rho_s with s, u, v of that doubled density.
+__device__ __host__ inline void pbe_exchange_channel(double n, double s, double u, double v,
+                                                     double& ex, double& vx) {
+  using C = PbeConst;
+  const double exunif = C::ax * cbrt(n);
+  const double s2 = s * s, s3 = s * s2;
+  const double p0 = 1.0 + C::mu_over_kappa * s2;
+  const double fx = 1.0 + C::kappa - C::kappa / p0;
+  ex = exunif * fx;
+  const double fs = 2.0 * C::mu / (p0 * p0);
+  const double fss = (-4.0 * C::mu_over_kappa * s * fs) / p0;
+  vx = exunif * (C::four_thirds * fx - (u - C::four_thirds * s3) * fss - v * fs);
+}
+
+// Spin-resolved inputs: rho, |grad rho|, grad rho . grad |grad rho| and the
+// Laplacian of each channel, |grad (a + b)| and its delgrad.  Outputs the
+// exchange and correlation energies per particle and the two potentials.
+__device__ __host__ inline void pbe_open(double ra, double ga, double da, double la, double rb,
+                                         double gb, double db, double lb, double gt, double dt,
+                                         double& ex, double& ec, double& va, double& vb) {
+  using C = PbeConst;
+  const double kmin = 1e-18;
+  const double pi = (double)3.141592653589793238462643383279f;
+  const double kf1 = cbrt(C::three_pi2);
+  double exa = 0.0, vxa = 0.0, exb = 0.0, vxb = 0.0;
+  ex = ec = va = vb = 0.0;
+  const double chan[2][4] = {{ra, ga, da, la}, {rb, gb, db, lb}};
+  for (int c = 0; c < 2; ++c) {
+    const double n = 2.0 * chan[c][0];
+    if (n > kmin) {
+      const double tk = 2.0 * (kf1 * cbrt(n));
+      const double s = (2.0 * chan[c][1]) / (n * tk);
+      const double v = (2.0 * chan[c][3]) / (n * (tk * tk));
+      const double u = (4.0 * chan[c][2]) / ((n * n) * (tk * tk * tk));
+      if (c == 0)
+        pbe_exchange_channel(n, s, u, v, exa, vxa);
+      else
+        pbe_exchange_channel(n, s, u, v, exb, vxb);
+    }
+  }
+  const double rho = ra + rb;
+  ex = exa * (ra / rho) + exb * (rb / rho);
+  va = vxa;
+  vb = vxb;
+  if (rho < kmin * 1e5) return;
+
+  const double zet = (ra - rb) / rho;
+  const double kf = kf1 * cbrt(rho);
+  const double rs = cbrt((9.0 * pi) / 4.0) / kf;
+  const double phi = (cbrt((1.0 + zet) * (1.0 + zet)) + cbrt((1.0 - zet) * (1.0 - zet))) / 2.0;
+  const double tks = 2.0 * sqrt(4.0 * kf / pi) * phi;
+  const double t = gt / (tks * rho);
+  const double uu = dt / (rho * rho * tks * tks * tks);
+  const double vv = (la + lb) / (rho * tks * tks);
+  const double ww = (ga * ga - gb * gb - zet * gt * gt) / ((rho * rho) * (tks * tks));
+
+  // local spin density correlation (PW92 eqs. 8-9 and A1-A4)
+  const PwFit fu{(double)0.0310907f, (double)0.21370f, (double)7.5957f, (double)3.5876f,
+                 (double)1.6382f, (double)0.49294f};
+  const PwFit fp{(double)0.01554535f, (double)0.20548f, (double)14.1189f, (double)6.1977f,
+                 (double)3.3662f, (double)0.62517f};
+  const PwFit fa{(double)0.0168869f, (double)0.11125f, (double)10.357f, (double)3.6231f,
+                 (double)0.88026f, (double)0.49671f};
+  const double srs = sqrt(rs);
+  double eu, eurs, ep, eprs, alfm, alfrsm;
+  pw92_fit(fu, srs, eu, eurs);
+  pw92_fit(fp, srs, ep, eprs);
+  pw92_fit(fa, srs, alfm, alfrsm);
+  const double z4 = (zet * zet) * (zet * zet);
+  const double gam = pow(2.0, 4.0 / 3.0) - 2.0;
+  const double fzz = 8.0 / (9.0 * gam);
+  const double f = (cbrt(pow(1.0 + zet, 4)) + cbrt(pow(1.0 - zet, 4)) - 2.0) / gam;
+  const double elsd = eu * (1.0 - f * z4) + ep * f * z4 - alfm * f * (1.0 - z4) / fzz;
+  const double ecrs = eurs * (1.0 - f * z4) + eprs * f * z4 - alfrsm * f * (1.0 - z4) / fzz;
+  const double fz = C::four_thirds * (cbrt(1.0 + zet) - cbrt(1.0 - zet)) / gam;
+  const double eczet = 4.0 * pow(zet, 3) * f * (ep - eu + alfm / fzz) +
+                       fz * (z4 * ep - z4 * eu - (1.0 - z4) * alfm / fzz);
+  const double lsd = elsd - rs * ecrs / 3.0 - zet * eczet;
+
+  // gradient correction H and its potential
+  const double gamma = (double)0.03109069086965489503494086371273f;
+  const double delta = C::beta / gamma;
+  const double g3 = pow(phi, 3);
+  const double A = delta / (exp(-elsd / (g3 * gamma)) - 1.0);
+  const double A2 = A * A;
+  const double t2 = t * t, t4 = t2 * t2, t6 = t4 * t2;
+  const double q4 = 1.0 + A * t2;
+  const double q5 = 1.0 + A * t2 + A2 * t4;
+  const double H = g3 * (C::beta / delta) * log(1.0 + delta * q4 * t2 / q5);
+  const double g4 = g3 * phi;
+  const double rs3 = rs / 3.0;
+  const double eta = (double)1e-12f;
+  const double sixth = (double)(-1.0f / 6.0f);
+  const double gz = (pow(pow(1.0 + zet, 2) + eta, sixth) - pow(pow(1.0 - zet, 2) + eta, sixth)) / 3.0;
+  const double fac = delta / A + 1.0;
+  const double bg = -3.0 * A2 * elsd * fac / (C::beta * g4);
+  const double bec = A2 * fac / (C::beta * g3);
+  const double q8 = pow(q5, 2) + delta * q4 * q5 * t2;
+  const double q9 = 1.0 + 2.0 * A * t2;
+  const double hA = -C::beta * g3 * A * (t6 / q8) * (2.0 + A * t2);
+  const double hrs = -rs3 * hA * bec * ecrs;
+  const double f0 = 2.0 * delta - 6.0 * A;
+  const double f1 = q5 * q9 + q4 * q9 * q9;
+  const double hAt = 2.0 * C::beta * g3 * t4 * ((q4 * q5 * f0 - delta * f1) / q8) / q8;
+  const double hrst = rs3 * t2 * hAt * bec * ecrs;
+  const double hz = 3.0 * gz * H / phi + hA * (bg * gz + bec * eczet);
+  const double ht = 2.0 * C::beta * g3 * q9 / q8;
+  const double hzt = 3.0 * gz * ht / phi + hAt * (bg * gz + bec * eczet);
+  const double f2 = q4 * q5 + A * t2 * (q4 * q9 + q5);
+  const double f3 = 2.0 * A * q5 * q9 + delta * f2;
+  const double htt = 4.0 * C::beta * g3 * t * (2.0 * A / q8 - ((q9 / q8) * f3) / q8);
+  double comm = H + hrs + hrst + t2 * ht / 6.0 + 7.0 * t2 * t * htt / 6.0;
+  const double pref = hz - gz * t2 * ht / phi;
+  const double f5 = gz * (2.0 * ht + t * htt) / phi;
+  comm = comm - pref * zet - uu * htt - vv * ht - ww * (hzt - f5);
+  ec = elsd + H;
+  va = vxa + ((lsd + eczet) + (comm + pref));
+  vb = vxb + ((lsd - eczet) + (comm - pref));
+}
+
+// Open-shell GGA point from the two channels' rho, gradient and second
+// derivatives (h1 = xx,yy,zz; h2 = xy,xz,yz).  Below 1e-18 in both channels
+// everything is zero; a channel with a zero gradient divides by zero, as the
+// reference does.
+__device__ __host__ inline void gga_point_open(double ra, const double ga[3], const double h1a[3],
+                                               const double h2a[3], double rb, const double gb[3],
+                                               const double h1b[3], const double h2b[3],
+                                               double& ex, double& ec, double& va, double& vb) {
+  ex = ec = va = vb = 0.0;
+  if (!(ra > 1e-18 || rb > 1e-18)) return;
+  double g[3], h1[3], h2[3];
+  for (int k = 0; k < 3; ++k) {
+    g[k] = ga[k] + gb[k];
+    h1[k] = h1a[k] + h1b[k];
+    h2[k] = h2a[k] + h2b[k];
+  }
+  auto del_spin = [](const double* q, const double* p1, const double* p2, double n) {
+    return (q[0] * q[0] * p1[0] + 2.0 * q[0] * q[1] * p2[0] + 2.0 * q[1] * q[2] * p2[2] +
+            2.0 * q[0] * q[2] * p2[1] + q[1] * q[1] * p1[1] + q[2] * q[2] * p1[2]) / n;
+  };
+  const double na = sqrt(ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2]);
+  const double nb = sqrt(gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2]);
+  const double nt = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  const double da = del_spin(ga, h1a, h2a, na);
+  const double db = del_spin(gb, h1b, h2b, nb);
+  const double dt = (g[0] * g[0] * h1[0] + g[1] * g[1] * h1[1] + g[2] * g[2] * h1[2] +
+                     2.0 * g[0] * g[1] * h2[0] + 2.0 * g[1] * g[2] * h2[2] +
+                     2.0 * g[0] * g[2] * h2[1]) / nt;
+  pbe_open(ra, na, da, h1a[0] + h1a[1] + h1a[2], rb, nb, db, h1b[0] + h1b[1] + h1b[2], nt, dt,
+           ex, ec, va, vb);
+}
+
 }  // namespace xc
 }  // namespace xcq

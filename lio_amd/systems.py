@@ -29,6 +29,7 @@ import functools
 import json
 import math
 import os
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -468,6 +469,12 @@ class System:
     iexch: int = 9
     normalize: bool = True
     meta: dict = field(default_factory=dict)
+    # open shell: the alpha and beta densities (P = Pa + Pb, rho_packed its packing)
+    open: bool = False
+    Pa: np.ndarray | None = None
+    Pb: np.ndarray | None = None
+    rho_a_packed: np.ndarray | None = None
+    rho_b_packed: np.ndarray | None = None
 
     @property
     def natom(self) -> int:
@@ -495,7 +502,9 @@ class System:
             a=np.asfortranarray(b.a).ravel(order="F"),
             c=np.asfortranarray(b.c).ravel(order="F"),
             rho=self.rho_packed.copy(), nco=self.nocc, iexch=self.iexch,
-            norm=int(self.normalize),
+            norm=int(self.normalize), open=bool(self.open),
+            rho_a=None if not self.open else self.rho_a_packed.copy(),
+            rho_b=None if not self.open else self.rho_b_packed.copy(),
             e1=e[50].ravel(order="F"), e2=e[116].ravel(order="F"), e3=e[194].ravel(order="F"),
             w1=w[50], w2=w[116], w3=w[194], grid_type=self.grid_type)
 
@@ -515,6 +524,24 @@ def make_system(name: str, Z, pos_ang, nocc: int | None = None, seed: int = 0,
     return System(name=name, Z=Z, pos=pos, basis=b, nocc=nocc,
                   rho_packed=pack_density(P), P=P, S=S, shells=shells, iexch=iexch,
                   meta={"seed": seed})
+
+
+def open_shell(base: System, nunp: int = 2, seed: int | None = None) -> System:
+    """The same molecule with nunp more alpha than beta electrons: seeded
+    Loewdin-orthonormal orbitals, the first nocc + nunp/2 occupied in alpha and
+    the first nocc - nunp/2 in beta."""
+    if nunp % 2:
+        raise ValueError("nunp must keep the electron count")
+    seed = base.meta.get("seed", 0) + 7 if seed is None else seed
+    na, nb = base.nocc + nunp // 2, base.nocc - nunp // 2
+    C = lowdin_orbitals(base.S, na, seed)
+    Pa = C @ C.T
+    Pb = C[:, :nb] @ C[:, :nb].T
+    P = Pa + Pb
+    return dataclasses.replace(base, name=base.name + "_open%d" % nunp, P=P,
+                               rho_packed=pack_density(P), open=True, Pa=Pa, Pb=Pb,
+                               rho_a_packed=pack_density(Pa), rho_b_packed=pack_density(Pb),
+                               meta=dict(base.meta, open_seed=seed, nunp=nunp))
 
 
 def config0(seed: int = 0) -> System:

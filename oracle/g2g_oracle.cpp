@@ -60,7 +60,9 @@ struct Problem {
   std::vector<int> nuc, ncont;  // nuc 0-based
   std::vector<double> a, c;     // M x 13, row i = function i
   std::vector<int> lx, ly, lz;  // Cartesian powers of each function
-  std::vector<double> rho;      // packed
+  std::vector<double> rho;      // packed (alpha when open)
+  bool open = false;
+  std::vector<double> rho_b;    // packed beta density (open shell)
   int ngrid = 0;
   std::vector<V3> e;
   std::vector<double> wang;
@@ -155,6 +157,11 @@ void load(Problem& pb, const XcInput* in) {
   }
   const size_t packed = (size_t)M * (M + 1) / 2;
   pb.rho.assign(in->rho, in->rho + packed);
+  pb.open = in->open != 0;
+  if (pb.open) {
+    if (!in->rho_b) throw std::runtime_error("open shell without a beta density");
+    pb.rho_b.assign(in->rho_b, in->rho_b + packed);
+  }
   static const int sizes[3] = {50, 116, 194};
   if (in->grid_type < 0 || in->grid_type > 2) throw std::runtime_error("grid type");
   pb.ngrid = sizes[in->grid_type];
@@ -603,6 +610,162 @@ void gga_point(double rho, const double g[3], const double h1[3], const double h
   y2a = vx + vc;
 }
 
+// Spin-polarised PBE (Burke's EASYPBE driver with PBE exchange by spin
+// scaling and CORPBE correlation, as LIO ships them: calc_ggaOS.h:10-108,
+// pbeOS_main.h:24-202, pbeOS_exch.h:36-72, pbeOS_corr.h:54-247, gcorc.h).
+// Constants are the reference's float literals widened to double; 4/3 and
+// -1/6 in the potentials are float quotients there and here.
+namespace os {
+
+void pw_fit(const double c[6], double srs, double& g, double& dg) {
+  const double q0 = -2.0 * c[0] * (1.0 + c[1] * srs * srs);
+  const double q1 = 2.0 * c[0] * srs * (c[2] + srs * (c[3] + srs * (c[4] + c[5] * srs)));
+  const double q2 = std::log(1.0 + 1.0 / q1);
+  g = q0 * q2;
+  const double q3 = c[0] * (c[2] / srs + 2.0 * c[3] + srs * (3.0 * c[4] + 4.0 * c[5] * srs));
+  dg = -2.0 * c[0] * c[1] * q2 - q0 * q3 / (q1 * (1.0 + q1));
+}
+
+// exchange of one channel at density n = 2 rho_sigma
+void exch(double n, double s, double u, double v, double& ex, double& vx) {
+  const double ax = (double)-0.738558766382022405884230032680836f;
+  const double um = (double)0.2195149727645171f, uk = (double)0.804f;
+  const double ul = (double)0.273028573090195f;
+  const double f43 = (double)(4.0f / 3.0f);
+  const double exunif = ax * std::cbrt(n);
+  const double s2 = s * s, s3 = s * s2;
+  const double p0 = 1.0 + ul * s2;
+  const double fx = 1.0 + uk - uk / p0;
+  ex = exunif * fx;
+  const double Fs = 2.0 * um / (p0 * p0);
+  const double Fss = -4.0 * ul * s * Fs / p0;
+  const double t2 = f43 * fx;
+  const double t3 = v * Fs;
+  const double t4 = (u - f43 * s3) * Fss;
+  vx = exunif * (t2 - t4 - t3);
+}
+
+void pbe(double ra, double ga, double da, double la, double rb, double gb, double db, double lb,
+         double gt, double dt, double& ex, double& ec, double& va, double& vb) {
+  const double tiny = 1e-18;
+  const double pi = (double)3.141592653589793238462643383279f;
+  const double pi32 = (double)29.608813203268075856503472999628f;
+  double exs[2] = {0, 0}, vxs[2] = {0, 0};
+  const double rs_[2] = {ra, rb}, gs_[2] = {ga, gb}, ds_[2] = {da, db}, ls_[2] = {la, lb};
+  for (int c = 0; c < 2; ++c) {
+    const double n = 2.0 * rs_[c];
+    if (!(n > tiny)) continue;
+    const double fk = std::cbrt(pi32) * std::cbrt(n);
+    const double k2 = 2.0 * fk;
+    const double s = (2.0 * gs_[c]) / (n * k2);
+    const double v = (2.0 * ls_[c]) / (n * (k2 * k2));
+    const double u = (4.0 * ds_[c]) / ((n * n) * (k2 * (k2 * k2)));
+    exch(n, s, u, v, exs[c], vxs[c]);
+  }
+  const double rho = ra + rb;
+  ex = exs[0] * (ra / rho) + exs[1] * (rb / rho);
+  ec = 0.0;
+  va = vxs[0];
+  vb = vxs[1];
+  if (rho < tiny * 1e5) return;
+  const double zet = (ra - rb) / rho;
+  const double fk = std::cbrt(pi32) * std::cbrt(rho);
+  const double rs = std::cbrt((9.0 * pi) / 4.0) / fk;
+  const double g = (std::cbrt((1.0 + zet) * (1.0 + zet)) + std::cbrt((1.0 - zet) * (1.0 - zet))) / 2.0;
+  const double twoksg = 2.0 * std::sqrt(4.0 * fk / pi) * g;
+  const double t = gt / (twoksg * rho);
+  const double uu = dt / (rho * rho * twoksg * twoksg * twoksg);
+  const double vv = (la + lb) / (rho * twoksg * twoksg);
+  const double ww = (ga * ga - gb * gb - zet * gt * gt) / ((rho * rho) * (twoksg * twoksg));
+  // LSD part (PW92)
+  static const double cu[6] = {(double)0.0310907f, (double)0.21370f, (double)7.5957f,
+                               (double)3.5876f, (double)1.6382f, (double)0.49294f};
+  static const double cp[6] = {(double)0.01554535f, (double)0.20548f, (double)14.1189f,
+                               (double)6.1977f, (double)3.3662f, (double)0.62517f};
+  static const double ca[6] = {(double)0.0168869f, (double)0.11125f, (double)10.357f,
+                               (double)3.6231f, (double)0.88026f, (double)0.49671f};
+  const double rt = std::sqrt(rs);
+  double EU, EURS, EP, EPRS, AM, AMRS;
+  pw_fit(cu, rt, EU, EURS);
+  pw_fit(cp, rt, EP, EPRS);
+  pw_fit(ca, rt, AM, AMRS);
+  const double Z4 = (zet * zet) * (zet * zet);
+  const double GAM = std::pow(2.0, 4.0 / 3.0) - 2.0;
+  const double F = (std::cbrt(std::pow(1.0 + zet, 4)) + std::cbrt(std::pow(1.0 - zet, 4)) - 2.0) / GAM;
+  const double FZZ = 8.0 / (9.0 * GAM);
+  const double e0 = EU * (1.0 - F * Z4) + EP * F * Z4 - AM * F * (1.0 - Z4) / FZZ;
+  const double ECRS = EURS * (1.0 - F * Z4) + EPRS * F * Z4 - AMRS * F * (1.0 - Z4) / FZZ;
+  const double FZ = (double)(4.0f / 3.0f) * (std::cbrt(1.0 + zet) - std::cbrt(1.0 - zet)) / GAM;
+  const double ECZET = 4.0 * std::pow(zet, 3) * F * (EP - EU + AM / FZZ) +
+                       FZ * (Z4 * EP - Z4 * EU - (1.0 - Z4) * AM / FZZ);
+  const double COMM0 = e0 - rs * ECRS / 3.0 - zet * ECZET;
+  // gradient part
+  const double bet = (double)0.06672455060314922f;
+  const double gama = (double)0.03109069086965489503494086371273f;
+  const double delt = bet / gama;
+  const double G = (std::cbrt(std::pow(1.0 + zet, 2)) + std::cbrt(std::pow(1.0 - zet, 2))) / 2.0;
+  const double G3 = std::pow(G, 3);
+  const double B = delt / (std::exp(-e0 / (G3 * gama)) - 1.0);
+  const double B2 = B * B, T2 = t * t, T4 = T2 * T2, T6 = T4 * T2;
+  const double Q4 = 1.0 + B * T2;
+  const double Q5 = 1.0 + B * T2 + B2 * T4;
+  const double H = G3 * (bet / delt) * std::log(1.0 + delt * Q4 * T2 / Q5);
+  const double G4 = G3 * G;
+  const double rsthrd = rs / 3.0;
+  const double m16 = (double)(-1.0f / 6.0f);
+  const double eta = (double)1e-12f;
+  const double GZ = (std::pow(std::pow(1.0 + zet, 2) + eta, m16) -
+                     std::pow(std::pow(1.0 - zet, 2) + eta, m16)) / 3.0;
+  const double FAC = delt / B + 1.0;
+  const double BG = -3.0 * B2 * e0 * FAC / (bet * G4);
+  const double BEC = B2 * FAC / (bet * G3);
+  const double Q8 = std::pow(Q5, 2) + delt * Q4 * Q5 * T2;
+  const double Q9 = 1.0 + 2.0 * B * T2;
+  const double hB = -bet * G3 * B * (T6 / Q8) * (2.0 + B * T2);
+  const double hRS = -rsthrd * hB * BEC * ECRS;
+  const double FACT0 = 2.0 * delt - 6.0 * B;
+  const double FACT1 = Q5 * Q9 + Q4 * Q9 * Q9;
+  const double hBT = 2.0 * bet * G3 * T4 * ((Q4 * Q5 * FACT0 - delt * FACT1) / Q8) / Q8;
+  const double hRST = rsthrd * T2 * hBT * BEC * ECRS;
+  const double hZ = 3.0 * GZ * H / G + hB * (BG * GZ + BEC * ECZET);
+  const double hT = 2.0 * bet * G3 * Q9 / Q8;
+  const double hZT = 3.0 * GZ * hT / G + hBT * (BG * GZ + BEC * ECZET);
+  const double FACT2 = Q4 * Q5 + B * T2 * (Q4 * Q9 + Q5);
+  const double FACT3 = 2.0 * B * Q5 * Q9 + delt * FACT2;
+  const double hTT = 4.0 * bet * G3 * t * (2.0 * B / Q8 - ((Q9 / Q8) * FACT3) / Q8);
+  double COMM = H + hRS + hRST + T2 * hT / 6.0 + 7.0 * T2 * t * hTT / 6.0;
+  const double PREF = hZ - GZ * T2 * hT / G;
+  const double FACT5 = GZ * (2.0 * hT + t * hTT) / G;
+  COMM = COMM - PREF * zet - uu * hTT - vv * hT - ww * (hZT - FACT5);
+  ec = e0 + H;
+  va = vxs[0] + ((COMM0 + ECZET) + (COMM + PREF));
+  vb = vxs[1] + ((COMM0 - ECZET) + (COMM - PREF));
+}
+
+// calc_ggaOS: the spin channels' rho, gradient and second derivatives
+void point(double ra, const double ga[3], const double h1a[3], const double h2a[3], double rb,
+           const double gb[3], const double h1b[3], const double h2b[3], double& ex, double& ec,
+           double& va, double& vb) {
+  ex = ec = va = vb = 0.0;
+  if (!(ra > 1e-18) && !(rb > 1e-18)) return;
+  const double g[3] = {ga[0] + gb[0], ga[1] + gb[1], ga[2] + gb[2]};
+  const double h1[3] = {h1a[0] + h1b[0], h1a[1] + h1b[1], h1a[2] + h1b[2]};
+  const double h2[3] = {h2a[0] + h2b[0], h2a[1] + h2b[1], h2a[2] + h2b[2]};
+  auto mod = [](const double* q) { return std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]); };
+  auto dspin = [](const double* q, const double* p1, const double* p2) {
+    return q[0] * q[0] * p1[0] + 2.0 * q[0] * q[1] * p2[0] + 2.0 * q[1] * q[2] * p2[2] +
+           2.0 * q[0] * q[2] * p2[1] + q[1] * q[1] * p1[1] + q[2] * q[2] * p1[2];
+  };
+  const double na = mod(ga), nb = mod(gb), nt = mod(g);
+  const double dt = (g[0] * g[0] * h1[0] + g[1] * g[1] * h1[1] + g[2] * g[2] * h1[2] +
+                     2.0 * g[0] * g[1] * h2[0] + 2.0 * g[1] * g[2] * h2[2] +
+                     2.0 * g[0] * g[2] * h2[1]) / nt;
+  pbe(ra, na, dspin(ga, h1a, h2a) / na, h1a[0] + h1a[1] + h1a[2], rb, nb, dspin(gb, h1b, h2b) / nb,
+      h1b[0] + h1b[1] + h1b[2], nt, dt, ex, ec, va, vb);
+}
+
+}  // namespace os
+
 // LDA: Slater exchange plus (1) nothing, (2) Gunnarsson-Lundqvist or (3) VWN
 // correlation (calc_ldaCS.h:32-102).
 void lda_point(double rho, int iexch, double& ex, double& ec, double& y2a) {
@@ -767,10 +930,129 @@ void solve_group(const Problem& pb, const Group& g, const double* rho, double& e
     }
 }
 
+// per-group open-shell solve (cpu/iteration.cpp:311-622).  The reference
+// computes nothing for an LDA in open shell (its LDA branch is empty): no
+// energy, no Fock, no forces; for PBE it builds both spin densities, calls the
+// spin-polarised functional and adds sum_p phi_i phi_j w_p v_sigma(p) to each
+// spin's Fock.  Forces add both spins' contributions per nucleus.
+void solve_group_open(const Problem& pb, const Group& g, const double* rho_a, const double* rho_b,
+                      double& energy, double& electrons, double* Fa, double* Fb, double* forces) {
+  if (!pb.gga) return;
+  const int m = g.m(), n = (int)g.pts.size(), M = pb.M;
+  const int nv = 10;
+  std::vector<double> fv((size_t)n * nv * m);
+  for (int p = 0; p < n; ++p)
+    for (int i = 0; i < m; ++i) {
+      double o[10];
+      eval_function(pb, g.fglobal[i], g.pts[p], o);
+      for (int v = 0; v < nv; ++v) fv[((size_t)p * nv + v) * m + i] = o[v];
+    }
+  std::vector<double> Pa((size_t)m * m), Pb((size_t)m * m);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      const size_t k = packed_index(g.fglobal[i], g.fglobal[j], M);
+      Pa[(size_t)i * m + j] = rho_a[k];
+      Pb[(size_t)i * m + j] = rho_b[k];
+    }
+  // rho, gradient, (xx,yy,zz), (xy,xz,yz) of one spin at point p
+  auto channel = [&](const std::vector<double>& Pl, const double* f, double q[10]) {
+    const double *gx = f + m, *gy = f + 2 * m, *gz = f + 3 * m;
+    const double *hxx = f + 4 * m, *hyy = f + 5 * m, *hzz = f + 6 * m;
+    const double *hxy = f + 7 * m, *hxz = f + 8 * m, *hyz = f + 9 * m;
+    for (int k = 0; k < 10; ++k) q[k] = 0.0;
+    for (int i = 0; i < m; ++i) {
+      double w = 0, wx = 0, wy = 0, wz = 0, w1x = 0, w1y = 0, w1z = 0, w2x = 0, w2y = 0, w2z = 0;
+      const double* Pi = &Pl[(size_t)i * m];
+      for (int j = 0; j <= i; ++j) {
+        const double pij = Pi[j];
+        w += f[j] * pij;
+        wx += gx[j] * pij;
+        wy += gy[j] * pij;
+        wz += gz[j] * pij;
+        w1x += hxx[j] * pij;
+        w1y += hyy[j] * pij;
+        w1z += hzz[j] * pij;
+        w2x += hxy[j] * pij;
+        w2y += hxz[j] * pij;
+        w2z += hyz[j] * pij;
+      }
+      const double fi = f[i];
+      q[0] += fi * w;
+      q[1] += gx[i] * w + wx * fi;
+      q[4] += gx[i] * wx * 2 + hxx[i] * w + w1x * fi;
+      q[7] += gx[i] * wy + gy[i] * wx + hxy[i] * w + w2x * fi;
+      q[2] += gy[i] * w + wy * fi;
+      q[5] += gy[i] * wy * 2 + hyy[i] * w + w1y * fi;
+      q[8] += gx[i] * wz + gz[i] * wx + hxz[i] * w + w2y * fi;
+      q[3] += gz[i] * w + wz * fi;
+      q[6] += gz[i] * wz * 2 + hzz[i] * w + w1z * fi;
+      q[9] += gy[i] * wz + gz[i] * wy + hyz[i] * w + w2z * fi;
+    }
+  };
+  std::vector<double> fa(n), fb(n);
+  for (int p = 0; p < n; ++p) {
+    const double* f = &fv[(size_t)p * nv * m];
+    double qa[10], qb[10];
+    channel(Pa, f, qa);
+    channel(Pb, f, qb);
+    double ex, ec, va, vb;
+    os::point(qa[0], qa + 1, qa + 4, qa + 7, qb[0], qb + 1, qb + 4, qb + 7, ex, ec, va, vb);
+    const double wp = g.w[p];
+    energy += ((qa[0] + qb[0]) * wp) * (ex + ec);
+    electrons += (qa[0] + qb[0]) * wp;
+    fa[p] = wp * va;
+    fb[p] = wp * vb;
+  }
+  if (forces) {
+    const size_t nn = g.nucs.size();
+    std::vector<double> sa(3 * nn, 0.0), sb(3 * nn, 0.0), da(3 * nn), db(3 * nn);
+    for (int p = 0; p < n; ++p) {
+      const double* f = &fv[(size_t)p * nv * m];
+      std::fill(da.begin(), da.end(), 0.0);
+      std::fill(db.begin(), db.end(), 0.0);
+      for (int i = 0; i < m; ++i) {
+        double wa = 0.0, wb = 0.0;
+        for (int j = 0; j < m; ++j) {
+          const double two = (i == j ? 2 : 1);
+          wa += Pa[(size_t)j * m + i] * f[j] * two;
+          wb += Pb[(size_t)j * m + i] * f[j] * two;
+        }
+        const size_t a = std::lower_bound(g.nucs.begin(), g.nucs.end(), pb.nuc[g.fglobal[i]]) -
+                         g.nucs.begin();
+        for (int c = 0; c < 3; ++c) {
+          da[3 * a + c] -= wa * f[(1 + c) * m + i];
+          db[3 * a + c] -= wb * f[(1 + c) * m + i];
+        }
+      }
+      for (size_t k = 0; k < 3 * nn; ++k) {
+        sa[k] += da[k] * fa[p];
+        sb[k] += db[k] * fb[p];
+      }
+    }
+    for (size_t a = 0; a < nn; ++a)
+      for (int c = 0; c < 3; ++c) forces[3 * (size_t)g.nucs[a] + c] += sa[3 * a + c] + sb[3 * a + c];
+  }
+  if (!Fa) return;
+  for (int i = 0; i < m; ++i)
+    for (int j = i; j < m; ++j) {
+      double acc_a = 0.0, acc_b = 0.0;
+      for (int p = 0; p < n; ++p) {
+        const double* f = &fv[(size_t)p * nv * m];
+        acc_a += f[i] * f[j] * fa[p];
+        acc_b += f[i] * f[j] * fb[p];
+      }
+      const size_t k = packed_index(g.fglobal[i], g.fglobal[j], M);
+      Fa[k] += acc_a;
+      Fb[k] += acc_b;
+    }
+}
+
 double solve_range(Handle& h, const double* rho_in, int first, int count, double* energy,
-                   double* F, double* electrons, double* points, double* forces = nullptr) {
+                   double* F, double* electrons, double* points, double* forces = nullptr,
+                   const double* rho_b_in = nullptr, double* Fb = nullptr) {
   const Problem& pb = h.pb;
   const double* rho = rho_in ? rho_in : pb.rho.data();
+  const double* rho_b = pb.open ? (rho_b_in ? rho_b_in : pb.rho_b.data()) : nullptr;
   const size_t packed = (size_t)pb.M * (pb.M + 1) / 2;
   const size_t nforce = forces ? 3 * (size_t)pb.natom : 0;
   const int last = std::min<int>((int)h.groups.size(), first + count);
@@ -779,12 +1061,17 @@ double solve_range(Handle& h, const double* rho_in, int first, int count, double
 #pragma omp parallel reduction(+ : e, ne, pts)
   {
     std::vector<double> Fl(F ? packed : 0, 0.0);
+    std::vector<double> Flb(F && pb.open ? packed : 0, 0.0);
     std::vector<double> fl(nforce, 0.0);
 #pragma omp for schedule(dynamic, 1)
     for (int gi = first; gi < last; ++gi) {
       double ge = 0.0, gn = 0.0;
-      solve_group(pb, h.groups[gi], rho, ge, gn, F ? Fl.data() : nullptr,
-                  forces ? fl.data() : nullptr);
+      if (pb.open)
+        solve_group_open(pb, h.groups[gi], rho, rho_b, ge, gn, F ? Fl.data() : nullptr,
+                         F ? Flb.data() : nullptr, forces ? fl.data() : nullptr);
+      else
+        solve_group(pb, h.groups[gi], rho, ge, gn, F ? Fl.data() : nullptr,
+                    forces ? fl.data() : nullptr);
       e += ge;
       ne += gn;
       pts += (double)h.groups[gi].pts.size();
@@ -792,6 +1079,10 @@ double solve_range(Handle& h, const double* rho_in, int first, int count, double
     if (F) {
 #pragma omp critical
       for (size_t k = 0; k < packed; ++k) F[k] += Fl[k];
+    }
+    if (F && pb.open && Fb) {
+#pragma omp critical
+      for (size_t k = 0; k < packed; ++k) Fb[k] += Flb[k];
     }
     if (forces) {
 #pragma omp critical
@@ -842,6 +1133,17 @@ double oracle_solve_forces(void* hp, const double* rho, double* energy, double* 
   auto* h = static_cast<Handle*>(hp);
   return solve_range(*h, rho, 0, (int)h->groups.size(), energy, nullptr, electrons, nullptr,
                      forces);
+}
+
+// Open shell: energy assigned, the two spins' Fock matrices accumulated (+=)
+// when fock_a is given, forces (natom x 3, atom-major) accumulated when
+// given, electrons assigned.
+double oracle_solve_open(void* hp, const double* rho_a, const double* rho_b, double* energy,
+                         double* fock_a, double* fock_b, double* forces, double* electrons) {
+  auto* h = static_cast<Handle*>(hp);
+  if (!h->pb.open) return -1.0;
+  return solve_range(*h, rho_a, 0, (int)h->groups.size(), energy, fock_a, electrons, nullptr,
+                     forces, rho_b, fock_b);
 }
 
 // Bounded sample for CPU timing: groups [first, first+count) of the cost-sorted
@@ -937,6 +1239,17 @@ int oracle_basis_values(const XcInput* in, int npts, const double* pts, int M, d
 void oracle_gga_points(int n, const double* rho, const double* grad, const double* h1,
                        const double* h2, double* ex, double* ec, double* y2a) {
   for (int i = 0; i < n; ++i) gga_point(rho[i], grad + 3 * i, h1 + 3 * i, h2 + 3 * i, ex[i], ec[i], y2a[i]);
+}
+
+// Spin-polarised PBE points: per point the two channels' rho, gradient (3),
+// (xx,yy,zz) and (xy,xz,yz); outputs ex, ec per particle and both potentials.
+void oracle_gga_points_open(int n, const double* ra, const double* ga, const double* h1a,
+                            const double* h2a, const double* rb, const double* gb,
+                            const double* h1b, const double* h2b, double* ex, double* ec,
+                            double* va, double* vb) {
+  for (int i = 0; i < n; ++i)
+    os::point(ra[i], ga + 3 * i, h1a + 3 * i, h2a + 3 * i, rb[i], gb + 3 * i, h1b + 3 * i,
+              h2b + 3 * i, ex[i], ec[i], va[i], vb[i]);
 }
 
 void oracle_lda_points(int n, const double* rho, int iexch, double* ex, double* ec, double* y2a) {

@@ -31,7 +31,8 @@ class XcInput(C.Structure):
                 ("rho", _dp), ("e", _dp * 3), ("wang", _dp * 3),
                 ("little_cube_size", C.c_double), ("sphere_radius", C.c_double),
                 ("assign_all_functions", C.c_int), ("remove_zero_weights", C.c_int),
-                ("min_points_per_cube", C.c_uint), ("max_function_exponent", C.c_uint)]
+                ("min_points_per_cube", C.c_uint), ("max_function_exponent", C.c_uint),
+                ("open", C.c_int), ("rho_b", _dp)]
 
 
 class XcGroupInfo(C.Structure):
@@ -77,7 +78,12 @@ def make_input(system, **options):
     s.nshell = u("nshell", ab["nshell"])
     s.a = d("a", ab["a"])
     s.c = d("c", ab["c"])
-    s.rho = d("rho", ab["rho"])
+    if ab.get("open"):
+        s.rho = d("rho", ab["rho_a"])
+        s.rho_b = d("rho_b", ab["rho_b"])
+        s.open = 1
+    else:
+        s.rho = d("rho", ab["rho"])
     for k, (en, wn) in enumerate((("e1", "w1"), ("e2", "w2"), ("e3", "w3"))):
         s.e[k] = d(en, ab[en])
         s.wang[k] = d(wn, ab[wn])
@@ -113,6 +119,8 @@ class RefG2G(_Checker):
         L.ref_solve.restype = C.c_double
         L.ref_solve_ct.argtypes = [C.c_uint, _dp, _dp, _dp, _dp]
         L.ref_solve_ct.restype = C.c_double
+        L.ref_solve_open.argtypes = [C.c_uint, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ref_solve_open.restype = C.c_double
         L.ref_electrons.restype = C.c_double
         L.ref_groups.argtypes = [C.POINTER(XcGroupInfo), C.c_int]
         L.ref_groups.restype = C.c_int
@@ -138,6 +146,18 @@ class RefG2G(_Checker):
         t = self.lib.ref_solve_ct(computation_type, rp, C.byref(e), None, _ptr(f, C.c_double))
         return e.value, f.reshape(3, self.natom).T.copy(), t
 
+    def solve_open(self, computation_type=0):
+        """Open shell: (energy, F_alpha, F_beta, forces (natom, 3), seconds)."""
+        e = C.c_double(0.0)
+        P = self.M * (self.M + 1) // 2
+        Fa, Fb = np.zeros(P), np.zeros(P)
+        f = np.zeros(3 * self.natom)
+        t = self.lib.ref_solve_open(computation_type, None, None, C.byref(e), _ptr(Fa, C.c_double),
+                                    _ptr(Fb, C.c_double), _ptr(f, C.c_double))
+        if t < 0:
+            raise RuntimeError("ref_solve_open on a closed-shell problem")
+        return e.value, Fa, Fb, f.reshape(3, self.natom).T.copy(), t
+
     def electrons(self):
         return self.lib.ref_electrons()
 
@@ -160,6 +180,8 @@ class Oracle(_Checker):
         L.oracle_solve.restype = C.c_double
         L.oracle_solve_forces.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.oracle_solve_forces.restype = C.c_double
+        L.oracle_solve_open.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_solve_open.restype = C.c_double
         L.oracle_solve_sample.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
         L.oracle_solve_sample.restype = C.c_double
         L.oracle_groups.argtypes = [C.c_void_p, C.POINTER(XcGroupInfo), C.c_int]
@@ -177,6 +199,8 @@ class Oracle(_Checker):
         L.oracle_basis_values.restype = C.c_int
         L.oracle_gga_points.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_gga_points.restype = None
+        L.oracle_gga_points_open.argtypes = [C.c_int] + [_dp] * 12
+        L.oracle_gga_points_open.restype = None
         L.oracle_lda_points.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_lda_points.restype = None
         L.oracle_pw92.argtypes = [C.c_double, _dp, _dp]
@@ -215,6 +239,22 @@ class Oracle(_Checker):
         t = self.lib.oracle_solve_forces(self.h, rp, C.byref(e), _ptr(f, C.c_double), C.byref(n))
         self._nelec = n.value
         return e.value, f.reshape(self.natom, 3), t
+
+    def solve_open(self, forces=False, fock=True):
+        """Open shell: (energy, F_alpha, F_beta, forces (natom, 3) or None, seconds)."""
+        e = C.c_double(0.0)
+        n = C.c_double(0.0)
+        P = self.M * (self.M + 1) // 2
+        Fa, Fb = np.zeros(P), np.zeros(P)
+        f = np.zeros(3 * self.natom)
+        t = self.lib.oracle_solve_open(self.h, None, None, C.byref(e),
+                                       _ptr(Fa, C.c_double) if fock else None,
+                                       _ptr(Fb, C.c_double) if fock else None,
+                                       _ptr(f, C.c_double) if forces else None, C.byref(n))
+        if t < 0:
+            raise RuntimeError("oracle_solve_open on a closed-shell problem")
+        self._nelec = n.value
+        return e.value, Fa, Fb, (f.reshape(self.natom, 3) if forces else None), t
 
     def solve_sample(self, first, count, rho=None):
         """Solves only groups [first, first+count) of the cost-sorted list; returns
@@ -271,6 +311,17 @@ class Oracle(_Checker):
                                    _ptr(h1, C.c_double), _ptr(h2, C.c_double),
                                    _ptr(ex, C.c_double), _ptr(ec, C.c_double), _ptr(v, C.c_double))
         return ex, ec, v
+
+    def gga_points_open(self, ra, ga, h1a, h2a, rb, gb, h1b, h2b):
+        """Spin-polarised PBE: (ex, ec, v_alpha, v_beta) per point."""
+        ra = np.ascontiguousarray(ra, np.float64)
+        n = len(ra)
+        arrs = [ra] + [np.ascontiguousarray(x, np.float64).reshape(n, 3) for x in (ga, h1a, h2a)]
+        arrs += [np.ascontiguousarray(rb, np.float64)]
+        arrs += [np.ascontiguousarray(x, np.float64).reshape(n, 3) for x in (gb, h1b, h2b)]
+        outs = [np.zeros(n) for _ in range(4)]
+        self.lib.oracle_gga_points_open(n, *[_ptr(a, C.c_double) for a in arrs + outs])
+        return tuple(outs)
 
     def lda_points(self, rho, iexch):
         rho = np.ascontiguousarray(rho, np.float64)

@@ -42,7 +42,7 @@ namespace {
 
 struct State {
   // Copies that stay alive while the reference holds raw pointers to them.
-  std::vector<double> r, Rm, a, c, rho, fock, fockb, e1, e2, e3, w1, w2, w3;
+  std::vector<double> r, Rm, a, c, rho, fock, fockb, rho_a, rho_b, e1, e2, e3, w1, w2, w3;
   std::vector<unsigned> Iz, Nr, Nr2, Nuc, ncont, nshell;
   unsigned natom = 0, M = 0;
   bool open = false, use_libxc = false;
@@ -94,7 +94,18 @@ double ref_setup(const XcInput* in) {
   S.nshell = copy_of(in->nshell, 3);
   S.a = copy_of(in->a, M * MAX_CONTRACTIONS);
   S.c = copy_of(in->c, M * MAX_CONTRACTIONS);
-  S.rho = copy_of(in->rho, packed);
+  S.open = in->open != 0;
+  if (S.open) {
+    // lioamber passes the total density as rho_vec and the two spins apart
+    S.rho_a = copy_of(in->rho, packed);
+    S.rho_b = copy_of(in->rho_b, packed);
+    S.rho.resize(packed);
+    for (size_t k = 0; k < packed; ++k) S.rho[k] = S.rho_a[k] + S.rho_b[k];
+  } else {
+    S.rho = copy_of(in->rho, packed);
+    S.rho_a.clear();
+    S.rho_b.clear();
+  }
   S.fock.assign(packed, 0.0);
   S.fockb.assign(packed, 0.0);
   S.e1 = copy_of(in->e[0], 50 * 3);
@@ -117,8 +128,9 @@ double ref_setup(const XcInput* in) {
   g2g_parameter_init_(norm, natom, natom, Mu, S.r.data(), S.Rm.data(),
                       S.Iz.data(), S.Nr.data(), S.Nr2.data(), S.Nuc.data(), Mu,
                       S.ncont.data(), S.nshell.data(), S.c.data(), S.a.data(),
-                      S.rho.data(), S.fock.data(), S.fockb.data(), nullptr,
-                      nullptr, nco, S.open, nunp, nopt, iexch, S.e1.data(),
+                      S.rho.data(), S.fock.data(), S.fockb.data(),
+                      S.open ? S.rho_a.data() : nullptr, S.open ? S.rho_b.data() : nullptr,
+                      nco, S.open, nunp, nopt, iexch, S.e1.data(),
                       S.e2.data(), S.e3.data(), S.w1.data(), S.w2.data(),
                       S.w3.data(), S.use_libxc, exid, ecid);
   const unsigned gt = in->grid_type;
@@ -155,6 +167,27 @@ double ref_solve_ct(unsigned ct, const double* rho, double* energy, double* fock
   g2g_solve_groups_(ct, energy, forces.data());
   double t1 = now();
   if (fock_out) std::memcpy(fock_out, S.fock.data(), S.fock.size() * sizeof(double));
+  if (forces_out) std::memcpy(forces_out, forces.data(), forces.size() * sizeof(double));
+  return t1 - t0;
+}
+
+// Open shell, computation type ct: the spin densities are copied into the
+// arrays the reference reads, fock_a / fock_b receive the two spins' packed
+// Fock matrices accumulated onto zeros, forces_out the forces (natom x 3,
+// column-major).  Returns seconds, or -1 when the problem is closed shell.
+double ref_solve_open(unsigned ct, const double* rho_a, const double* rho_b, double* energy,
+                      double* fock_a, double* fock_b, double* forces_out) {
+  if (!S.open) return -1.0;
+  if (rho_a) std::memcpy(S.rho_a.data(), rho_a, S.rho_a.size() * sizeof(double));
+  if (rho_b) std::memcpy(S.rho_b.data(), rho_b, S.rho_b.size() * sizeof(double));
+  std::fill(S.fock.begin(), S.fock.end(), 0.0);
+  std::fill(S.fockb.begin(), S.fockb.end(), 0.0);
+  std::vector<double> forces(3 * (size_t)S.natom, 0.0);
+  double t0 = now();
+  g2g_solve_groups_(ct, energy, forces.data());
+  double t1 = now();
+  if (fock_a) std::memcpy(fock_a, S.fock.data(), S.fock.size() * sizeof(double));
+  if (fock_b) std::memcpy(fock_b, S.fockb.data(), S.fockb.size() * sizeof(double));
   if (forces_out) std::memcpy(forces_out, forces.data(), forces.size() * sizeof(double));
   return t1 - t0;
 }

@@ -40,6 +40,9 @@ typedef struct {
   int remove_zero_weights;
   unsigned min_points_per_cube;
   unsigned max_function_exponent;
+  /* open shell: rho is the alpha density, rho_b the beta one (same packing) */
+  int open;
+  const double* rho_b;
 } XcInput;
 
 /* One point group as the partition defines it. */

@@ -38,6 +38,12 @@ def cases():
         yield "mol%d_s%d_pbe" % (n, seed), (lambda n=n, seed=seed: systems.random_molecule(n, seed))
 
 
+# open shell (two more alpha than beta electrons), PBE
+OPEN_CASES = [("c0_pbe_open2", lambda: systems.open_shell(systems.config0())),
+              ("mol10_s102_pbe_open2", lambda: systems.open_shell(systems.random_molecule(10, 102))),
+              ("mol25_s103_pbe_open2", lambda: systems.open_shell(systems.random_molecule(25, 103)))]
+
+
 GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsum", "fhash")
 
 
@@ -64,6 +70,26 @@ def main():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print("%-18s M=%4d groups=%4d Exc=% .12f N=%.9f nnz(F)=%d max|f|=%.6f" %
               (name, s.M, len(groups), E, N, len(nz), np.abs(forces).max()), flush=True)
+    for name, builder in OPEN_CASES:
+        s = builder()
+        ref.setup(s)
+        E, Fa, Fb, _, t = ref.solve_open(0)
+        N = ref.electrons()
+        groups = ref.groups()
+        Ef, _, _, forces, _ = ref.solve_open(2)
+        nz = np.nonzero(Fa)[0]
+        out = dict(
+            energy=E, electrons=N, M=s.M, natom=s.natom, iexch=s.iexch, open=1,
+            rho_sum=float(s.rho_packed.sum()), rho_norm=float(np.linalg.norm(s.rho_packed)),
+            fock_index=nz.astype(np.int64), fock_value=Fa[nz], fock_b_value=Fb[nz],
+            solve_seconds=t, forces=forces, energy_forces=Ef,
+        )
+        for f in GROUP_FIELDS:
+            out["g_" + f] = np.array([g[f] for g in groups],
+                                     dtype=np.uint64 if f == "fhash" else None)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print("%-22s M=%4d groups=%4d Exc=% .12f N=%.9f max|f|=%.6f" %
+              (name, s.M, len(groups), E, N, np.abs(forces).max()), flush=True)
     ref.teardown()
 
 

@@ -10,6 +10,8 @@ GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsu
 
 
 def system_for(name: str):
+    if name.endswith("_open2"):
+        return systems.open_shell(system_for(name[:-len("_open2")]))
     if name.startswith("c0_"):
         s = systems.config0()
         if name.startswith("c0_lda"):
@@ -28,13 +30,18 @@ def load(name: str):
     M = int(z["M"])
     F = np.zeros(M * (M + 1) // 2)
     F[z["fock_index"]] = z["fock_value"]
+    Fb = None
+    if "fock_b_value" in z.files:  # open shell: F is the alpha Fock, Fb the beta one
+        Fb = np.zeros(M * (M + 1) // 2)
+        Fb[z["fock_index"]] = z["fock_b_value"]
     groups = []
     for k in range(len(z["g_kind"])):
         groups.append({f: (int(z["g_" + f][k]) if f != "wsum" and f != "xsum" else float(z["g_" + f][k]))
                        for f in GROUP_FIELDS})
     return dict(energy=float(z["energy"]), electrons=float(z["electrons"]), fock=F, M=M,
                 groups=groups, rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]),
-                forces=np.array(z["forces"]), energy_forces=float(z["energy_forces"]))
+                forces=np.array(z["forces"]), energy_forces=float(z["energy_forces"]),
+                fock_b=Fb)
 
 
 def names():

@@ -55,6 +55,24 @@ def test_oracle_matches_golden_water_cluster():
     _compare_forces(o, gold)
 
 
+@pytest.mark.parametrize("name", ["c0_pbe_open2", "mol10_s102_pbe_open2", "mol25_s103_pbe_open2"])
+def test_oracle_open_shell_matches_golden(name):
+    """Open shell (two unpaired electrons, PBE): energy, both spins' Fock
+    matrices, electron count and forces of LIO's CPU path."""
+    gold = load(name)
+    s = system_for(name)
+    check_inputs(s, gold)
+    o = Oracle()
+    o.setup(s)
+    assert same_partition(o.groups(), gold["groups"])
+    E, Fa, Fb, _, _ = o.solve_open()
+    _compare(E, Fa, o.electrons(), gold)
+    assert np.max(np.abs(Fb - gold["fock_b"])) <= REL * max(1.0, np.max(np.abs(gold["fock_b"])))
+    Ef, _, _, f, _ = o.solve_open(forces=True, fock=False)
+    assert abs(Ef - gold["energy_forces"]) <= REL * abs(gold["energy_forces"])
+    assert np.max(np.abs(f - gold["forces"])) <= REL * max(1.0, np.max(np.abs(gold["forces"])))
+
+
 needs_ref = pytest.mark.skipif(not os.path.exists(REF_SO),
                                reason="LIO's CPU path is built only where its sources exist")
 

@@ -261,3 +261,52 @@ def test_lda_forces_are_energy_derivatives(oracle, iexch):
     assert np.max(np.abs(forces - fd)) < 1e-7 * max(1.0, np.max(np.abs(fd)))
     # a plausible slip (a dropped factor 2 or a sign) fails by far more
     assert np.max(np.abs(fd)) > 1e-2
+
+
+def test_pbe_open_unpolarised_equals_closed(oracle):
+    """Equal spin channels: the spin-polarised PBE reduces to the closed-shell
+    one (exchange by spin scaling, correlation at zeta = 0).  The two routines
+    form beta/gamma and 1/gamma differently (float literals in the closed-shell
+    routine, double quotients of float literals in the open one): a 3e-8
+    relative difference in the constants, amplified in H at low density, so
+    correlation agrees to ~1e-6 (observed 8.5e-7) and the potential to ~1e-7
+    (observed 6e-8); a dropped or mis-signed term is off by far more."""
+    rng = np.random.default_rng(5)
+    n = 64
+    rho = 10 ** rng.uniform(-3, 1, n)
+    g = rng.normal(size=(n, 3)) * rho[:, None]
+    h1 = rng.normal(size=(n, 3)) * rho[:, None]
+    h2 = rng.normal(size=(n, 3)) * rho[:, None]
+    ex, ec, v = oracle.gga_points(rho, g, h1, h2)
+    exo, eco, va, vb = oracle.gga_points_open(rho / 2, g / 2, h1 / 2, h2 / 2,
+                                              rho / 2, g / 2, h1 / 2, h2 / 2)
+    assert np.allclose(exo, ex, rtol=1e-12, atol=0)
+    assert np.allclose(eco, ec, rtol=2e-6, atol=0)
+    assert np.array_equal(va, vb)
+    assert np.allclose(va, v, rtol=3e-7, atol=1e-12)
+
+
+@pytest.mark.parametrize("spin", [0, 1])
+def test_pbe_open_potentials_are_functional_derivatives(oracle, spin):
+    """v_sigma = dE/d rho_sigma for the spin-polarised PBE in LIO's Laplacian
+    form, checked by central differences of the quadrature energy along a
+    perturbation of one spin density."""
+    pts, wt = _sphere_grid()
+    ca = dict(centers=[np.array([0, 0, 0.35]), np.array([0, 0, -0.35])], alphas=[1.1, 0.8],
+              coefs=[0.9, 0.7])
+    cb = dict(centers=[np.array([0, 0.1, 0.3]), np.array([0.05, 0, -0.4])], alphas=[1.3, 0.7],
+              coefs=[0.5, 0.6])
+    dr = dict(centers=[np.array([0.2, -0.1, 0.15])], alphas=[0.9], coefs=[0.3])
+    A, B, D = _gaussians(pts, **ca), _gaussians(pts, **cb), _gaussians(pts, **dr)
+
+    def energy(lam):
+        a = [x + (lam * d if spin == 0 else 0) for x, d in zip(A, D)]
+        b = [x + (lam * d if spin == 1 else 0) for x, d in zip(B, D)]
+        ex, ec, va, vb = oracle.gga_points_open(*a, *b)
+        return (wt * (a[0] + b[0]) * (ex + ec)).sum(), (va, vb)
+
+    lam = 1e-4
+    fd = (energy(lam)[0] - energy(-lam)[0]) / (2 * lam)
+    v = energy(0.0)[1][spin]
+    analytic = (wt * v * D[0]).sum()
+    assert abs(fd - analytic) < 1e-6 * abs(analytic)
