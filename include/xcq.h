@@ -43,6 +43,7 @@ typedef struct {
   int remove_zero_weights;  /* default 1 */
   unsigned min_points_per_cube;   /* default 1 */
   unsigned max_function_exponent; /* default 10 */
+  int open;                 /* 1: open shell (PBE only); solve with xcq_solve_open */
 } xcq_problem;
 
 /* Partition inventory of one point group. */
@@ -126,6 +127,15 @@ int xcq_solve_device(xcq_context* ctx, const double* d_rho_packed, double* d_foc
  * (may be NULL) additionally receives += F_xc. */
 int xcq_solve_forces(xcq_context* ctx, const double* rho_packed, double* fock_accum,
                      double* forces_accum, int ld, xcq_solve_info* info);
+/* Open shell (g2g/cpu/iteration.cpp:311-622, spin-polarised PBE of
+ * g2g/pointxc/calc_ggaOS.h): the packed alpha and beta densities in (same
+ * packing as rho), both spins' Fock matrices accumulated (+=) -- both or
+ * neither -- and the forces as in xcq_solve_forces (forces_accum may be
+ * NULL).  Batches of open-shell problems use the xcq_batch_* entry points
+ * with d_rho and d_fock holding the alpha then the beta packed blocks
+ * (2 x offsets[n] entries each). */
+int xcq_solve_open(xcq_context* ctx, const double* rho_a, const double* rho_b, double* fock_a,
+                   double* fock_b, double* forces_accum, int ld, xcq_solve_info* info);
 int xcq_sync_energy(xcq_context* ctx, double* energy, double* electrons);
 /* Asynchronous build from a packed host density (copied at once): queued on
  * the context stream with the copy-back of F, Exc and N.  xcq_query returns

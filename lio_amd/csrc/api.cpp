@@ -62,6 +62,7 @@ int guarded(F&& f) {
 
 void view_from_problem(const xcq_problem* p, xcq::ProblemView& pv, xcq::Options& opt) {
   pv.natom = p->natom;
+  pv.open = p->open != 0;
   pv.max_atoms = p->max_atoms ? p->max_atoms : p->natom;
   pv.M = p->M;
   pv.ngaussians = p->ngaussians ? p->ngaussians : p->M;
@@ -113,7 +114,9 @@ void check_problem(const xcq::ProblemView& pv) {
   if (!pv.r || !pv.Rm || !pv.Iz || !pv.Nuc || !pv.ncont || !pv.nshell || !pv.a || !pv.c)
     throw std::runtime_error("g2g: problem arrays missing");
   if (pv.natom == 0 || pv.M == 0) throw std::runtime_error("g2g: empty problem");
-  if (pv.open) throw std::runtime_error("g2g: open-shell solve is not provided by this build");
+  if (pv.open && pv.iexch <= 3)
+    throw std::runtime_error("g2g: open-shell LDA is not provided (LIO's CPU path evaluates no "
+                             "functional there and reads unset factors)");
 }
 
 void build(xcq_context* ctx, unsigned grid_type) {
@@ -280,6 +283,22 @@ int xcq_solve_forces(xcq_context* ctx, const double* rho, double* fock_accum,
     if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
     if (!forces_accum) throw std::runtime_error("g2g: no force array");
     xcq::SolveResult r = ctx->engine->solve(rho, fock_accum, true, forces_accum, ld);
+    if (info) {
+      info->energy = r.energy;
+      info->electrons = r.electrons;
+      info->seconds = r.seconds;
+      for (int k = 0; k < 4; ++k) info->kernel_ms[k] = r.kernel_ms[k];
+    }
+  });
+}
+
+int xcq_solve_open(xcq_context* ctx, const double* rho_a, const double* rho_b, double* fock_a,
+                   double* fock_b, double* forces_accum, int ld, xcq_solve_info* info) {
+  return guarded([&] {
+    if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");
+    if (!ctx->engine->open_shell()) throw std::runtime_error("g2g: the problem is closed shell");
+    xcq::SolveResult r =
+        ctx->engine->solve_open(rho_a, rho_b, fock_a, fock_b, true, forces_accum, ld);
     if (info) {
       info->energy = r.energy;
       info->electrons = r.electrons;
@@ -605,9 +624,6 @@ void g2g_parameter_init_(const unsigned* norm, const unsigned* natom, const unsi
                          const unsigned* nopt, const unsigned* Iexch, double* e, double* e2,
                          double* e3, double* wang, double* wang2, double* wang3, bool* use_libxc,
                          const unsigned* ex_functional_id, const unsigned* ec_functional_id) {
-  (void)fockb_vec;
-  (void)rhoalpha;
-  (void)rhobeta;
   (void)ex_functional_id;
   (void)ec_functional_id;
   abi("g2g_parameter_init_", [&] {
@@ -636,8 +652,14 @@ void g2g_parameter_init_(const unsigned* norm, const unsigned* natom, const unsi
     pv.nshell = nshell;
     pv.c = c;
     pv.a = a;
-    pv.rho = rho_vec;
+    // open shell reads the two spin densities and writes both Fock matrices
+    // (g2g/init.cpp:161-178); rho_vec (the total) is not used by the solve
+    pv.rho = *OPEN ? rhoalpha : rho_vec;
     pv.fock = fock_vec;
+    pv.rho_b = *OPEN ? rhobeta : nullptr;
+    pv.fock_b = *OPEN ? fockb_vec : nullptr;
+    if (*OPEN && (!rhoalpha || !rhobeta || !fockb_vec))
+      throw std::runtime_error("open shell without alpha/beta densities or beta Fock");
     pv.e[0] = e;
     pv.e[1] = e2;
     pv.e[2] = e3;
@@ -672,9 +694,15 @@ void g2g_solve_groups_(const unsigned* computation_type, double* energy, double*
     const bool compute_forces = (ct == 2 || ct == 3);
     const bool compute_energy = (ct == 1 || ct == 2) || g_opt.energy_all_iterations;
     if (compute_forces && !forces) throw std::runtime_error("forces requested without a force array");
-    xcq::SolveResult r = ctx->engine->solve(ctx->pv.rho, compute_rmm ? ctx->pv.fock : nullptr,
-                                            compute_energy, compute_forces ? forces : nullptr,
-                                            (int)ctx->pv.max_atoms);
+    xcq::SolveResult r =
+        ctx->pv.open
+            ? ctx->engine->solve_open(ctx->pv.rho, ctx->pv.rho_b,
+                                      compute_rmm ? ctx->pv.fock : nullptr,
+                                      compute_rmm ? ctx->pv.fock_b : nullptr, compute_energy,
+                                      compute_forces ? forces : nullptr, (int)ctx->pv.max_atoms)
+            : ctx->engine->solve(ctx->pv.rho, compute_rmm ? ctx->pv.fock : nullptr,
+                                 compute_energy, compute_forces ? forces : nullptr,
+                                 (int)ctx->pv.max_atoms);
     const double e = compute_energy ? r.energy : 0.0;
     *energy = e;
     if (std::isnan(e)) die("g2g_solve_groups_", "XC energy is NaN");

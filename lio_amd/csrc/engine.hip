@@ -251,6 +251,60 @@ __global__ __launch_bounds__(256) void k_gather_density(const GatherItem* __rest
   pg[g.pg_off + e] = (T)v;
 }
 
+// Functional stage of a density launch for point q from its row-dot sums s
+// (rho and the halves of grad rho and of its second derivatives).  spin 0:
+// closed shell.  Open shell runs the density launch once per spin: spin 1
+// (alpha) stores its ten sums in `sums` (plane k at k * stride + q) and
+// contributes no energy; spin 2 (beta) reads them back and evaluates the
+// spin-polarised functional (g2g/cpu/iteration.cpp:311-488), writing both
+// spins' factors w v_sigma.  Open shell is GGA only.
+template <int NT>
+__device__ inline void point_stage(const double* s, int q, int iexch, int spin,
+                                   const double* __restrict__ pw, double* __restrict__ factor,
+                                   double* __restrict__ factor_b, double* __restrict__ sums,
+                                   long long stride, double& e_w, double& n_w) {
+  if constexpr (NT == 4) {
+    if (spin == 1) {
+      for (int k = 0; k < 10; ++k) sums[k * stride + q] = s[k];
+      e_w = n_w = 0.0;
+      return;
+    }
+    if (spin == 2) {
+      double a[10];
+      for (int k = 0; k < 10; ++k) a[k] = sums[k * stride + q];
+      const double ga[3] = {2.0 * a[1], 2.0 * a[2], 2.0 * a[3]};
+      const double h1a[3] = {2.0 * a[4], 2.0 * a[5], 2.0 * a[6]};
+      const double h2a[3] = {2.0 * a[7], 2.0 * a[8], 2.0 * a[9]};
+      const double gb[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
+      const double h1b[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
+      const double h2b[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
+      double ex, ec, va, vb;
+      xc::gga_point_open(a[0], ga, h1a, h2a, s[0], gb, h1b, h2b, ex, ec, va, vb);
+      const double w = pw[q];
+      factor[q] = w * va;
+      factor_b[q] = w * vb;
+      const double dens = a[0] + s[0];
+      e_w = (dens * w) * (ex + ec);
+      n_w = dens * w;
+      return;
+    }
+  }
+  const double dens = s[0];
+  double ex, ec, y2a;
+  if constexpr (NT == 4) {
+    const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
+    const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
+    const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
+    xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+  } else {
+    xc::lda_point(dens, iexch, ex, ec, y2a);
+  }
+  const double w = pw[q];
+  factor[q] = w * y2a;
+  e_w = (dens * w) * (ex + ec);
+  n_w = dens * w;
+}
+
 // ---------------------------------------------------------------------------
 // density + functional: 64 points per block, 16 per wave
 // ---------------------------------------------------------------------------
@@ -267,7 +321,9 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
                                                  const T* __restrict__ vals,
                                                  const double* __restrict__ pw,
                                                  double* __restrict__ factor,
-                                                 double* __restrict__ epart, int iexch) {
+                                                 double* __restrict__ epart, int iexch,
+                                                 int spin, double* __restrict__ factor_b,
+                                                 double* __restrict__ sums, long long sstride) {
   __shared__ double sE[4][2];
   constexpr int NS = (NT == 4) ? 10 : 1;
   const DensityItem it = items[blockIdx.x];
@@ -352,23 +408,9 @@ __global__ __launch_bounds__(256, 2) void k_density(const DensityItem* __restric
       s[k] += __shfl_xor(s[k], 16);
       s[k] += __shfl_xor(s[k], 32);
     }
-    if (lk == 0) {
-      const int q = g.pt_off + p0 + lrow;
-      const double dens = s[0];
-      double ex, ec, y2a;
-      if constexpr (NT == 4) {
-        const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
-        const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
-        const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
-        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
-      } else {
-        xc::lda_point(dens, iexch, ex, ec, y2a);
-      }
-      const double w = pw[q];
-      factor[q] = w * y2a;
-      e_w = (dens * w) * (ex + ec);
-      n_w = dens * w;
-    }
+    if (lk == 0)
+      point_stage<NT>(s, g.pt_off + p0 + lrow, iexch, spin, pw, factor, factor_b, sums, sstride,
+                      e_w, n_w);
     for (int o = 8; o >= 1; o >>= 1) {
       e_w += __shfl_xor(e_w, o);
       n_w += __shfl_xor(n_w, o);
@@ -762,7 +804,9 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
                                                      const float* __restrict__ vals,
                                                      const double* __restrict__ pw,
                                                      double* __restrict__ factor,
-                                                     double* __restrict__ epart, int iexch) {
+                                                     double* __restrict__ epart, int iexch,
+                                                     int spin, double* __restrict__ factor_b,
+                                                     double* __restrict__ sums, long long sstride) {
   constexpr int NS = (NT == 4) ? 10 : 1;
   constexpr int RT = ROWS / 16;
   constexpr int UNITS = 3 * ROWS * 4;      // 16-byte units of one staged tile
@@ -874,23 +918,9 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       s[k] += __shfl_xor(s[k], 16);
       s[k] += __shfl_xor(s[k], 32);
     }
-    if (lk == 0) {
-      const int q = g.pt_off + p0 + lrow;
-      const double dens = s[0];
-      double ex, ec, y2a;
-      if constexpr (NT == 4) {
-        const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
-        const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
-        const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
-        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
-      } else {
-        xc::lda_point(dens, iexch, ex, ec, y2a);
-      }
-      const double w = pw[q];
-      factor[q] = w * y2a;
-      e_w = (dens * w) * (ex + ec);
-      n_w = dens * w;
-    }
+    if (lk == 0)
+      point_stage<NT>(s, g.pt_off + p0 + lrow, iexch, spin, pw, factor, factor_b, sums, sstride,
+                      e_w, n_w);
     for (int o = 8; o >= 1; o >>= 1) {
       e_w += __shfl_xor(e_w, o);
       n_w += __shfl_xor(n_w, o);
@@ -1323,9 +1353,10 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
 
 // gsum[fn_off + j][a] = sum over the group's blocks (in order) of gpart; one
 // block per group.
+// accumulate: add onto gsum (the second spin of an open shell).
 __global__ __launch_bounds__(256) void k_force_group(const DevGroup* __restrict__ groups,
                                                      const double* __restrict__ gpart,
-                                                     double* __restrict__ gsum) {
+                                                     double* __restrict__ gsum, int accumulate) {
   const DevGroup g = groups[blockIdx.x];
   const int nblk = (g.npts_pad + kDensPts - 1) / kDensPts;
   const long long stride = 3LL * g.m_pad;
@@ -1333,7 +1364,8 @@ __global__ __launch_bounds__(256) void k_force_group(const DevGroup* __restrict_
   for (int e = threadIdx.x; e < 3 * g.m_pad; e += 256) {
     double s = 0.0;
     for (int b = 0; b < nblk; ++b) s += src[b * stride + e];
-    gsum[3LL * g.fn_off + e] = s;
+    double* dst = gsum + 3LL * g.fn_off + e;
+    *dst = accumulate ? *dst + s : s;
   }
 }
 
@@ -1660,6 +1692,10 @@ struct DeviceGrid {
   void* vals = nullptr;   // value planes, FP64 or FP32 by precision
   void* pg = nullptr;     // P_g blocks (per solve), same type
   Precision prec = Precision::FP64;
+  bool open = false;      // open shell: rho/F hold alpha then beta packed blocks
+  void* pg2 = nullptr;    // beta P_g blocks (open shell)
+  double* factor_b = nullptr;  // beta factors (open shell)
+  double* sums = nullptr;      // alpha row-dot sums, 10 x npts_pad (open shell)
   double* fg = nullptr;  // F_g blocks (per solve)
   double* factor = nullptr;
   double* epart = nullptr;
@@ -1783,6 +1819,21 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
+  dg->open = mols[0]->open;
+  for (const Molecule* m : mols)
+    if (m->open != dg->open)
+      throw std::runtime_error("g2g: the molecules of a batch must all be open or all closed shell");
+  if (dg->open && !dg->gga) {
+    delete dg;
+    throw std::runtime_error("g2g: open-shell LDA is not provided (LIO's CPU path evaluates no "
+                             "functional there and reads unset factors)");
+  }
+  if (dg->open && prec == Precision::BF16X6 && dg->density_variant != 1 &&
+      dg->density_variant != 2) {
+    delete dg;
+    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1 or 2");
+  }
+  const int nspin = dg->open ? 2 : 1;
   // LDA keeps the gradient too: the XC forces need it
   dg->nval = dg->gga ? 10 : 4;
   dg->prec = prec;
@@ -1977,8 +2028,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->py = c.take<double>(pt);
     dg->pz = c.take<double>(pt);
     dg->pw = c.take<double>(pt);
-    dg->rho = c.take<double>(packed);
-    dg->F = c.take<double>(packed);
+    dg->rho = c.take<double>(nspin * packed);
+    dg->F = c.take<double>(nspin * packed);
     dg->vals = c.take<char>(vo * vbytes);
     dg->pg = c.take<char>(po * pbytes);
     dg->fg = c.take<double>(po);
@@ -1997,6 +2048,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->afn_off = c.take<int>(afn_off.size());
     dg->afn = c.take<int>(afn.size());
     dg->fout = c.take<double>(3 * (size_t)dg->natom);
+    dg->pg2 = c.take<char>(dg->open ? po * pbytes : 0);
+    dg->factor_b = c.take<double>(dg->open ? pt : 0);
+    dg->sums = c.take<double>(dg->open ? 10 * pt : 0);
   };
   Carver sizing{nullptr};
   plan(sizing);
@@ -2028,7 +2082,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   Carver carve{dg->arena};
   plan(carve);
-  if (old.g && old.g->h_rho && old.g->pinned_packed >= (size_t)packed &&
+  if (old.g && old.g->h_rho && old.g->pinned_packed >= (size_t)(nspin * packed) &&
       old.g->pinned_mols >= nmol) {
     dg->h_rho = old.g->h_rho;
     dg->h_F = old.g->h_F;
@@ -2037,10 +2091,10 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->pinned_mols = old.g->pinned_mols;
     old.g->h_rho = old.g->h_F = old.g->h_e = nullptr;
   } else {
-    dg->pinned_packed = (size_t)packed;
+    dg->pinned_packed = (size_t)(nspin * packed);
     dg->pinned_mols = nmol;
-    XCQ_HIP(hipHostMalloc(&dg->h_rho, packed * sizeof(double), hipHostMallocDefault));
-    XCQ_HIP(hipHostMalloc(&dg->h_F, packed * sizeof(double), hipHostMallocDefault));
+    XCQ_HIP(hipHostMalloc(&dg->h_rho, nspin * packed * sizeof(double), hipHostMallocDefault));
+    XCQ_HIP(hipHostMalloc(&dg->h_F, nspin * packed * sizeof(double), hipHostMallocDefault));
     XCQ_HIP(hipHostMalloc(&dg->h_e, 2 * nmol * sizeof(double), hipHostMallocDefault));
   }
 
@@ -2303,58 +2357,60 @@ static void mark(hipEvent_t* ev, int k, hipStream_t s) {
   if (ev) XCQ_HIP(hipEventRecord(ev[k], s));
 }
 
+// Gather and density + functional launches of one spin.  spin 0 closed
+// shell, 1 the alpha pass (P_g from the first packed block into pg), 2 the
+// beta pass (the second block into pg2).
 template <class T>
-static void launch_solve(DeviceGrid& g, hipStream_t s, const double* d_rho, bool fock,
-                         hipEvent_t* ev) {
-  T* pg = static_cast<T*>(g.pg);
+static void launch_density(DeviceGrid& g, hipStream_t s, const double* d_rho, int spin,
+                           hipEvent_t* ev) {
+  T* pg = static_cast<T*>(spin == 2 ? g.pg2 : g.pg);
+  const double* rho = spin == 2 ? d_rho + g.packed : d_rho;
   const T* vals = static_cast<const T*>(g.vals);
   if (g.ngather > 0)
     hipLaunchKernelGGL(k_gather_density<T>, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
-                       g.mols, g.fnl, d_rho, pg);
-  mark(ev, 1, s);
+                       g.mols, g.fnl, rho, pg);
+  if (spin < 2) mark(ev, 1, s);
   if (g.ndensity > 0) {
     if (g.gga)
       hipLaunchKernelGGL((k_density<4, T>), dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
-                         pg, vals, g.pw, g.factor, g.epart, g.iexch);
+                         pg, vals, g.pw, g.factor, g.epart, g.iexch, spin, g.factor_b, g.sums,
+                         g.npts_pad);
     else
       hipLaunchKernelGGL((k_density<1, T>), dim3(g.ndensity), dim3(256), 0, s, g.ditems, g.groups,
-                         pg, vals, g.pw, g.factor, g.epart, g.iexch);
-  }
-  XCQ_HIP(hipGetLastError());
-  mark(ev, 2, s);
-  if (fock && g.nfock > 0) {
-    hipLaunchKernelGGL(k_fock<T>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, vals,
-                       g.factor, g.fpart, g.fock_slice);
-    hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
-                       g.gtiles, g.fpart, g.fg);
+                         pg, vals, g.pw, g.factor, g.epart, g.iexch, spin, g.factor_b, g.sums,
+                         g.npts_pad);
   }
   XCQ_HIP(hipGetLastError());
 }
 
-// Split-BF16 solve: pieces of P_g, BF16 MFMA density and Fock.
-static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, bool fock,
-                            hipEvent_t* ev) {
-  unsigned short* pg3 = static_cast<unsigned short*>(g.pg);
+static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, int spin,
+                              hipEvent_t* ev) {
+  unsigned short* pg3 = static_cast<unsigned short*>(spin == 2 ? g.pg2 : g.pg);
+  const double* rho = spin == 2 ? d_rho + g.packed : d_rho;
   const float* vals = static_cast<const float*>(g.vals);
   if (g.ngather > 0)
     hipLaunchKernelGGL(k_gather_density_bf, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
-                       g.mols, g.fnl, d_rho, pg3);
-  mark(ev, 1, s);
+                       g.mols, g.fnl, rho, pg3);
+  if (spin < 2) mark(ev, 1, s);
   if (g.ndensity > 0) {
     if (g.density_variant == 1) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
-                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
-                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad);
     } else if (g.density_variant == 2) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
-                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
-                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch);
+                           g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad);
     } else if (g.density_variant == 3 || g.density_variant == 4) {
       const int nr = g.nrowsplit, ns = g.ndensity - nr;
       const DensityItem* si = g.ditems + nr;
@@ -2368,14 +2424,16 @@ static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, b
                              pg3, vals, g.pw, g.factor, g.epart, g.iexch);
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
-                             vals, g.pw, g.factor, se, g.iexch);
+                             vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
+                             g.npts_pad);
       } else {
         if (nr > 0)
           hipLaunchKernelGGL((k_density_bf3<1, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
                              pg3, vals, g.pw, g.factor, g.epart, g.iexch);
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
-                             vals, g.pw, g.factor, se, g.iexch);
+                             vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
+                             g.npts_pad);
       }
     } else {
       if (g.gga)
@@ -2387,35 +2445,51 @@ static void launch_solve_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, b
     }
   }
   XCQ_HIP(hipGetLastError());
-  mark(ev, 2, s);
-  if (fock && g.nfock > 0) {
+}
+
+// F_g blocks of one spin from its factors w v (slices, then slice sums).
+static void launch_fock(DeviceGrid& g, hipStream_t s, const double* factor) {
+  if (g.nfock <= 0) return;
+  if (g.prec == Precision::FP64)
+    hipLaunchKernelGGL(k_fock<double>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups,
+                       static_cast<const double*>(g.vals), factor, g.fpart, g.fock_slice);
+  else if (g.prec == Precision::FP32)
+    hipLaunchKernelGGL(k_fock<float>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups,
+                       static_cast<const float*>(g.vals), factor, g.fpart, g.fock_slice);
+  else
     hipLaunchKernelGGL(k_fock_bf, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.phiT,
-                       g.factor, g.fpart, g.fock_slice);
-    hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups,
-                       g.gtiles, g.fpart, g.fg);
-  }
+                       factor, g.fpart, g.fock_slice);
+  hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups, g.gtiles,
+                     g.fpart, g.fg);
   XCQ_HIP(hipGetLastError());
 }
 
-// XC forces after the density launch (P_g and factor in place): per-block
-// partials, group sums, per-atom sums into d_forces (natom x 3, atom-major).
+// XC forces after the density launches (P_g blocks and factors in place):
+// per-block partials and group sums per spin, then per-atom sums into
+// d_forces (natom x 3, atom-major).
 static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
-  if (g.nforce > 0) {
-    if (g.prec == Precision::FP64)
-      hipLaunchKernelGGL(k_force<double>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
-                         static_cast<const double*>(g.pg), static_cast<const double*>(g.vals),
-                         g.factor, g.gpart);
-    else if (g.prec == Precision::FP32)
-      hipLaunchKernelGGL(k_force<float>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
-                         static_cast<const float*>(g.pg), static_cast<const float*>(g.vals),
-                         g.factor, g.gpart);
-    else
-      hipLaunchKernelGGL(k_force_bf, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
-                         static_cast<const unsigned short*>(g.pg),
-                         static_cast<const float*>(g.vals), g.factor, g.gpart);
+  const int nspin = g.open ? 2 : 1;
+  for (int sp = 0; sp < nspin; ++sp) {
+    const void* pg = sp == 1 ? g.pg2 : g.pg;
+    const double* factor = sp == 1 ? g.factor_b : g.factor;
+    if (g.nforce > 0) {
+      if (g.prec == Precision::FP64)
+        hipLaunchKernelGGL(k_force<double>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                           static_cast<const double*>(pg), static_cast<const double*>(g.vals),
+                           factor, g.gpart);
+      else if (g.prec == Precision::FP32)
+        hipLaunchKernelGGL(k_force<float>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                           static_cast<const float*>(pg), static_cast<const float*>(g.vals),
+                           factor, g.gpart);
+      else
+        hipLaunchKernelGGL(k_force_bf, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                           static_cast<const unsigned short*>(pg),
+                           static_cast<const float*>(g.vals), factor, g.gpart);
+    }
+    if (g.ngroups > 0)
+      hipLaunchKernelGGL(k_force_group, dim3(g.ngroups), dim3(256), 0, s, g.groups, g.gpart,
+                         g.gsum, sp);
   }
-  if (g.ngroups > 0)
-    hipLaunchKernelGGL(k_force_group, dim3(g.ngroups), dim3(256), 0, s, g.groups, g.gpart, g.gsum);
   if (g.natom > 0)
     hipLaunchKernelGGL(k_force_atom, dim3((g.natom + 63) / 64), dim3(64), 0, s, g.afn_off, g.afn,
                        g.memb_off, g.memb, g.groups, g.gsum, g.natom, d_forces);
@@ -2424,20 +2498,30 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
 
 // The whole build: gather, density + functional, Fock, packed reduction,
 // per-molecule energies, and the XC forces when d_forces is given.  Every
-// launch covers all molecules of the batch.
+// launch covers all molecules of the batch.  Open shell: d_rho and d_fock
+// hold the alpha then the beta packed blocks; the density runs once per spin
+// and the Fock matrix is built and packed per spin.
 static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double* d_fock,
                        hipEvent_t* ev, double* d_forces = nullptr) {
   mark(ev, 0, s);
-  if (g.prec == Precision::FP64)
-    launch_solve<double>(g, s, d_rho, d_fock != nullptr, ev);
-  else if (g.prec == Precision::FP32)
-    launch_solve<float>(g, s, d_rho, d_fock != nullptr, ev);
-  else
-    launch_solve_bf(g, s, d_rho, d_fock != nullptr, ev);
-  mark(ev, 3, s);
-  if (d_fock)
+  const int nspin = g.open ? 2 : 1;
+  for (int sp = 0; sp < nspin; ++sp) {
+    const int spin = g.open ? sp + 1 : 0;
+    if (g.prec == Precision::FP64)
+      launch_density<double>(g, s, d_rho, spin, ev);
+    else if (g.prec == Precision::FP32)
+      launch_density<float>(g, s, d_rho, spin, ev);
+    else
+      launch_density_bf(g, s, d_rho, spin, ev);
+  }
+  mark(ev, 2, s);
+  for (int sp = 0; d_fock && sp < nspin; ++sp) {
+    launch_fock(g, s, sp == 1 ? g.factor_b : g.factor);
+    if (sp == 0) mark(ev, 3, s);
     hipLaunchKernelGGL(k_fock_pack, dim3((g.max_M + 127) / 128, g.M), dim3(128), 0, s, g.memb_off,
-                       g.memb, g.groups, g.mols, g.fn_mol, g.fg, d_fock);
+                       g.memb, g.groups, g.mols, g.fn_mol, g.fg, d_fock + sp * g.packed);
+  }
+  if (!d_fock) mark(ev, 3, s);
   hipLaunchKernelGGL(k_energy, dim3(g.nmol), dim3(1024), 0, s, g.epart, g.mols, g.eidx, g.eout);
   XCQ_HIP(hipGetLastError());
   if (d_forces) launch_forces(g, s, d_forces);
@@ -2537,6 +2621,7 @@ void Engine::kernel_times(double ms[4]) {
 
 void Engine::solve_async(const double* rho_packed) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  if (grid_->open) throw std::runtime_error("g2g: asynchronous builds are closed shell only");
   DeviceGrid& g = *grid_;
   hipStream_t s = (hipStream_t)stream_;
   XCQ_HIP(hipSetDevice(device_));
@@ -2569,6 +2654,14 @@ SolveResult Engine::collect(double* fock_accum) {
 
 SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool want_energy,
                           double* forces_accum, int forces_ld) {
+  if (grid_ && grid_->open)
+    throw std::runtime_error("g2g: this grid is open shell: solve it with both spin densities");
+  return solve_open(rho_packed, nullptr, fock_accum, nullptr, want_energy, forces_accum, forces_ld);
+}
+
+SolveResult Engine::solve_open(const double* rho_a, const double* rho_b, double* fock_a,
+                               double* fock_b, bool want_energy, double* forces_accum,
+                               int forces_ld) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
   const double t0 = wall();
   DeviceGrid& g = *grid_;
@@ -2576,20 +2669,27 @@ SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool wan
   XCQ_HIP(hipSetDevice(device_));
   if (forces_accum && forces_ld < g.natom)
     throw std::runtime_error("g2g: the force array's leading dimension is below the atom count");
+  if (g.open && (!rho_b || (fock_a != nullptr) != (fock_b != nullptr)))
+    throw std::runtime_error("g2g: an open-shell solve needs both spin densities (and both Fock "
+                             "matrices or neither)");
   const size_t packed = (size_t)g.packed;
-  std::memcpy(g.h_rho, rho_packed, packed * sizeof(double));
-  XCQ_HIP(hipMemcpyAsync(g.rho, g.h_rho, packed * sizeof(double), hipMemcpyHostToDevice, s));
-  SolveResult res = solve_device(g.rho, fock_accum ? g.F : nullptr, want_energy,
+  const int nspin = g.open ? 2 : 1;
+  std::memcpy(g.h_rho, rho_a, packed * sizeof(double));
+  if (g.open) std::memcpy(g.h_rho + packed, rho_b, packed * sizeof(double));
+  XCQ_HIP(hipMemcpyAsync(g.rho, g.h_rho, nspin * packed * sizeof(double), hipMemcpyHostToDevice, s));
+  SolveResult res = solve_device(g.rho, fock_a ? g.F : nullptr, want_energy,
                                  forces_accum ? g.fout : nullptr);
-  if (fock_accum)
-    XCQ_HIP(hipMemcpyAsync(g.h_F, g.F, packed * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (fock_a)
+    XCQ_HIP(hipMemcpyAsync(g.h_F, g.F, nspin * packed * sizeof(double), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipMemcpyAsync(g.h_e, g.eout, 2 * g.nmol * sizeof(double), hipMemcpyDeviceToHost, s));
   std::vector<double> f(forces_accum ? 3 * (size_t)g.natom : 0);
   if (forces_accum)
     XCQ_HIP(hipMemcpyAsync(f.data(), g.fout, f.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipStreamSynchronize(s));
-  if (fock_accum)
-    for (size_t k = 0; k < packed; ++k) fock_accum[k] += g.h_F[k];
+  if (fock_a)
+    for (size_t k = 0; k < packed; ++k) fock_a[k] += g.h_F[k];
+  if (fock_b)
+    for (size_t k = 0; k < packed; ++k) fock_b[k] += g.h_F[packed + k];
   // Fortran layout of the caller's forces: column c of atom a at c * ld + a
   if (forces_accum)
     for (int a = 0; a < g.natom; ++a)
@@ -2604,5 +2704,7 @@ SolveResult Engine::solve(const double* rho_packed, double* fock_accum, bool wan
   res.seconds = wall() - t0;
   return res;
 }
+
+bool Engine::open_shell() const { return grid_ && grid_->open; }
 
 }  // namespace xcq

@@ -190,6 +190,7 @@ Molecule capture(const ProblemView& pv, unsigned grid_type) {
   mol.M = (int)pv.M;
   mol.iexch = (int)pv.iexch;
   mol.gga = pv.iexch > 3;
+  mol.open = pv.open;
   mol.ns = (int)pv.nshell[0];
   mol.np = (int)pv.nshell[1] / 3;
   mol.nd = (int)pv.nshell[2] / 6;

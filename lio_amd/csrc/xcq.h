@@ -48,8 +48,10 @@ struct ProblemView {
   const unsigned* nshell = nullptr; // 3
   const double* c = nullptr;        // M x 13, ld = ngaussians
   const double* a = nullptr;        // M x 13, ld = ngaussians
-  const double* rho = nullptr;      // packed density
-  double* fock = nullptr;           // packed Fock, accumulated (+=)
+  const double* rho = nullptr;      // packed density (alpha when open)
+  double* fock = nullptr;           // packed Fock, accumulated (+=) (alpha when open)
+  const double* rho_b = nullptr;    // open shell: packed beta density
+  double* fock_b = nullptr;         // open shell: packed beta Fock (+=)
   const double* e[3] = {nullptr, nullptr, nullptr};
   const double* wang[3] = {nullptr, nullptr, nullptr};
 };
@@ -68,6 +70,7 @@ struct Molecule {
   int natom = 0, M = 0, ns = 0, np = 0, nd = 0;  // np/nd count shells
   int iexch = 9;
   bool gga = true;
+  bool open = false;            // open shell (two spin densities)
   std::vector<double> x, y, z;  // atom positions (bohr)
   std::vector<int> Z;
   std::vector<double> rm;       // Becke size parameter per atom
@@ -197,6 +200,12 @@ class Engine {
   // XC forces of g2g/cpu/iteration.cpp:213-268; nullptr skips them.
   SolveResult solve(const double* rho_packed, double* fock_accum, bool want_energy,
                     double* forces_accum = nullptr, int forces_ld = 0);
+  // Open shell (g2g/cpu/iteration.cpp:311-622): the alpha and beta densities
+  // in, both spins' Fock matrices accumulated (both or neither), forces as
+  // in solve.  Also accepts a closed-shell grid with rho_b = fock_b = null.
+  SolveResult solve_open(const double* rho_a, const double* rho_b, double* fock_a, double* fock_b,
+                         bool want_energy, double* forces_accum = nullptr, int forces_ld = 0);
+  bool open_shell() const;
   // Queues a build from a packed host density (copied to pinned memory at
   // once) with its copy-back; idle() tells when it has finished and
   // collect() waits for it and adds F onto fock_accum (may be null).

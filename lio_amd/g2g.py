@@ -37,7 +37,8 @@ class XcqProblem(C.Structure):
                 ("e", _dp * 3), ("wang", _dp * 3),
                 ("little_cube_size", C.c_double), ("sphere_radius", C.c_double),
                 ("assign_all_functions", C.c_int), ("remove_zero_weights", C.c_int),
-                ("min_points_per_cube", C.c_uint), ("max_function_exponent", C.c_uint)]
+                ("min_points_per_cube", C.c_uint), ("max_function_exponent", C.c_uint),
+                ("open", C.c_int)]
 
 
 class XcqGroupInfo(C.Structure):
@@ -90,6 +91,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_solve.argtypes = [vp, _dp, _dp, C.POINTER(XcqSolveInfo)]
     L.xcq_solve_device.argtypes = [vp, vp, vp]
     L.xcq_solve_forces.argtypes = [vp, _dp, _dp, _dp, C.c_int, C.POINTER(XcqSolveInfo)]
+    L.xcq_solve_open.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.POINTER(XcqSolveInfo)]
     L.xcq_batch_solve_forces.argtypes = [vp, vp, vp, vp]
     L.xcq_batch_atoms.argtypes = [vp, C.POINTER(C.c_int)]
     L.xcq_sync_energy.argtypes = [vp, _dp, _dp]
@@ -151,6 +153,7 @@ def make_problem(system, **options):
     p.M = p.ngaussians = ab["M"]
     p.normalize = ab["norm"]
     p.iexch = ab["iexch"]
+    p.open = int(bool(ab.get("open")))
     p.r, p.Rm, p.Iz = d("r", ab["r"]), d("Rm", ab["Rm"]), u("Iz", ab["Iz"])
     p.Nr, p.Nr2 = u("Nr", ab["Nr"]), u("Nr2", ab["Nr2"])
     p.Nuc, p.ncont, p.nshell = u("Nuc", ab["Nuc"]), u("ncont", ab["ncont"]), u("nshell", ab["nshell"])
@@ -258,6 +261,25 @@ class Context:
                                        C.byref(info)), "xcq_solve_forces")
         return info.energy, f.reshape(3, ld)[:, :n].T.copy(), dict(electrons=info.electrons)
 
+    def solve_open(self, fock=True, forces=False):
+        """Open-shell build from the system's alpha and beta densities;
+        returns (energy, F_alpha, F_beta, forces (natom, 3) or None, info)."""
+        s = self.system
+        ra = np.ascontiguousarray(s.rho_a_packed, np.float64)
+        rb = np.ascontiguousarray(s.rho_b_packed, np.float64)
+        P = self.M * (self.M + 1) // 2
+        Fa, Fb = (np.zeros(P), np.zeros(P)) if fock else (None, None)
+        n = s.natom
+        f = np.zeros(3 * n) if forces else None
+        info = XcqSolveInfo()
+        _check(self.L.xcq_solve_open(self.h, _ptr(ra, C.c_double), _ptr(rb, C.c_double),
+                                     None if Fa is None else _ptr(Fa, C.c_double),
+                                     None if Fb is None else _ptr(Fb, C.c_double),
+                                     None if f is None else _ptr(f, C.c_double), n, C.byref(info)),
+               "xcq_solve_open")
+        ff = None if f is None else f.reshape(3, n).T.copy()
+        return info.energy, Fa, Fb, ff, dict(electrons=info.electrons)
+
     def solve_device(self, d_rho: int, d_fock: int):
         """Queues a solve on device pointers (integers); no synchronisation."""
         _check(self.L.xcq_solve_device(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock)),
@@ -357,13 +379,18 @@ class G2G:
             arr[name] = np.ascontiguousarray(ab[name], np.uint32)
         packed = self.M * (self.M + 1) // 2
         arr["fock"] = np.zeros(packed)
-        arr["fockb"] = np.zeros(1)
+        self.open = bool(ab.get("open"))
+        arr["fockb"] = np.zeros(packed if self.open else 1)
+        if self.open:
+            arr["rhoa"] = np.ascontiguousarray(ab["rho_a"], np.float64)
+            arr["rhob"] = np.ascontiguousarray(ab["rho_b"], np.float64)
         dp = lambda n: _ptr(arr[n], C.c_double)
         up = lambda n: _ptr(arr[n], C.c_uint)
         args = [U(ab["norm"]), U(ab["natom"]), U(ab["max_atoms"]), U(ab["ngaussians"]),
                 dp("r"), dp("Rm"), up("Iz"), up("Nr"), up("Nr2"), up("Nuc"), U(ab["M"]),
                 up("ncont"), up("nshell"), dp("c"), dp("a"), dp("rho"), dp("fock"), dp("fockb"),
-                None, None, U(ab["nco"]), B(False), U(0), U(0), U(ab["iexch"]),
+                dp("rhoa") if self.open else None, dp("rhob") if self.open else None,
+                U(ab["nco"]), B(self.open), U(2 if self.open else 0), U(0), U(ab["iexch"]),
                 dp("e1"), dp("e2"), dp("e3"), dp("w1"), dp("w2"), dp("w3"), B(False), U(0), U(0)]
         L.g2g_parameter_init_(*args)
         self._grid_type = U(ab["grid_type"])
@@ -376,8 +403,14 @@ class G2G:
 
     @property
     def fock(self) -> np.ndarray:
-        """The caller-owned packed Fock the library accumulates into."""
+        """The caller-owned packed Fock the library accumulates into (alpha
+        in open shell)."""
         return self._arr["fock"]
+
+    @property
+    def fock_b(self) -> np.ndarray:
+        """Open shell: the caller-owned packed beta Fock."""
+        return self._arr["fockb"]
 
     def solve(self, computation_type: int = 0, forces=None) -> float:
         """g2g_solve_groups_(computation_type, energy, forces): ``forces`` is
@@ -425,7 +458,11 @@ class Batch:
         self.offsets = off
 
     def packed_density(self):
-        """The concatenated packed densities of the systems (host array)."""
+        """The concatenated packed densities of the systems (host array); for
+        open-shell systems all alpha blocks, then all beta blocks."""
+        if self.systems[0].open:
+            return np.concatenate([s.rho_a_packed for s in self.systems] +
+                                  [s.rho_b_packed for s in self.systems])
         return np.concatenate([s.rho_packed for s in self.systems])
 
     def fock_of(self, F, k: int):

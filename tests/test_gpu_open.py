@@ -1,0 +1,123 @@
+"""Open-shell XC on the GPU (g2g/cpu/iteration.cpp:311-622, spin-polarised PBE
+of g2g/pointxc/calc_ggaOS.h).
+
+Both spins' Fock matrices, the energy, the electron count and the forces are
+held to 1e-8 absolute against LIO's own CPU path (golden vectors) in FP64 and
+to 1e-6 in FP32 and split-BF16; the Fortran ABI reads rhoalpha/rhobeta and
+accumulates onto Fmat_vec and Fmat_vec2; a batch of open-shell molecules
+reproduces each one alone bitwise; open-shell LDA, which LIO's CPU path does
+not evaluate, is refused with a message.
+"""
+import numpy as np
+import pytest
+
+from golden_io import check_inputs, load, system_for
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+TOL_MIXED = 1e-6
+CASES = ["c0_pbe_open2", "mol10_s102_pbe_open2", "mol25_s103_pbe_open2"]
+
+
+def _gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _open(s, precision="fp64", forces=False):
+    from lio_amd import g2g
+    ctx = g2g.Context(s, precision=precision)
+    ctx.build_grid()
+    out = ctx.solve_open(fock=not forces, forces=forces)
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x6"])
+@pytest.mark.parametrize("name", CASES)
+def test_open_shell_matches_golden(name, precision):
+    _gpu()
+    gold = load(name)
+    s = system_for(name)
+    check_inputs(s, gold)
+    print("case", name, precision, flush=True)
+    E, Fa, Fb, _, info = _open(s, precision)
+    tol = TOL if precision == "fp64" else TOL_MIXED
+    dE = abs(E - gold["energy"])
+    dFa = float(np.max(np.abs(Fa - gold["fock"])))
+    dFb = float(np.max(np.abs(Fb - gold["fock_b"])))
+    dN = abs(info["electrons"] - gold["electrons"])
+    print("%s %s dE %.2e dFa %.2e dFb %.2e dN %.2e" % (name, precision, dE, dFa, dFb, dN),
+          flush=True)
+    assert dE <= tol and dFa <= tol and dFb <= tol and dN <= tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x6"])
+@pytest.mark.parametrize("name", ["c0_pbe_open2", "mol25_s103_pbe_open2"])
+def test_open_shell_forces(name, precision):
+    _gpu()
+    gold = load(name)
+    s = system_for(name)
+    E, _, _, f, _ = _open(s, precision, forces=True)
+    df = float(np.max(np.abs(f - gold["forces"])))
+    print("%s %s max|df| %.2e" % (name, precision, df), flush=True)
+    tol = TOL if precision == "fp64" else TOL_MIXED
+    assert abs(E - gold["energy_forces"]) <= tol and df <= tol
+
+
+def test_g2g_abi_open_shell():
+    """g2g_parameter_init_ with OPEN: the solve reads rhoalpha/rhobeta and adds
+    onto both Fock vectors; type 2 adds the forces."""
+    _gpu()
+    from lio_amd import g2g
+    name = "mol10_s102_pbe_open2"
+    gold = load(name)
+    s = system_for(name)
+    run = g2g.G2G()
+    run.setup(s)
+    run.fock[:] = 0.5
+    run.fock_b[:] = -0.5
+    E = run.solve(0)
+    assert abs(E - gold["energy"]) <= TOL
+    assert np.max(np.abs((run.fock - 0.5) - gold["fock"])) <= TOL
+    assert np.max(np.abs((run.fock_b + 0.5) - gold["fock_b"])) <= TOL
+    forces = np.zeros((s.natom, 3), order="F")
+    E2 = run.solve(2, forces)
+    assert abs(E2 - gold["energy_forces"]) <= TOL
+    assert np.max(np.abs(forces - gold["forces"])) <= TOL
+    run.deinit()
+
+
+def test_open_batch_equals_alone():
+    torch = _gpu()
+    from lio_amd import g2g, systems
+    sys_ = [systems.open_shell(m) for m in systems.molecule_batch(4, seed=12, atoms=(3, 12))]
+    b = g2g.Batch(sys_, precision="fp64")
+    rho = torch.from_numpy(b.packed_density()).cuda()
+    F = torch.zeros_like(rho)
+    b.solve_device(rho.data_ptr(), F.data_ptr())
+    E, N = b.energies()
+    torch.cuda.synchronize()
+    Fh = F.cpu().numpy()
+    total = int(b.offsets[-1])
+    for k, s in enumerate(sys_):
+        Ea, Fa, Fb, _, info = _open(s)
+        print("molecule", k, s.natom, "atoms", E[k], Ea, flush=True)
+        assert E[k] == Ea and N[k] == info["electrons"]
+        assert np.array_equal(b.fock_of(Fh[:total], k), Fa)
+        assert np.array_equal(b.fock_of(Fh[total:], k), Fb)
+    b.close()
+
+
+def test_open_shell_lda_is_refused():
+    _gpu()
+    from lio_amd import g2g, systems
+    s = systems.open_shell(systems.config0())
+    s.iexch = 2
+    ctx = g2g.Context(s)
+    with pytest.raises(RuntimeError, match="open-shell LDA"):
+        ctx.build_grid()
+    ctx.close()
