@@ -196,13 +196,20 @@ long long xcq_batch_arena_bytes(xcq_batch* b);
 /* Autotuner: times `reps` builds for every candidate schedule c < ncand --
  * Fock slice length slices[c] (points, multiple of 128) and, when variants is
  * not NULL, split-BF16 density kernel variants[c] (0: 64-function chunks, 1 and 2:
- * 128- and 64-function chunks with LDS-staged density pieces) -- on these buffers
+ * 128- and 64-function chunks with LDS-staged density pieces, 3 and 4:
+ * row-split 16-point blocks for groups above 64 functions, 5: 2 in the
+ * radial-Hessian layout, 4 value planes instead of 10) -- on these buffers
  * (rebuilding the work lists for each), keeps the fastest and returns the
  * per-candidate ms in ms_out; *best is its index.  A captured graph is
  * dropped. */
 int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps,
                    const int* slices, const int* variants, int ncand, double* ms_out,
                    int* best);
+/* Sets the schedule directly (a zero slice or a negative variant keeps that
+ * knob): for a context it takes effect at the next xcq_build_grid, a batch
+ * reloads at once.  No reference counterpart. */
+int xcq_set_schedule(xcq_context* ctx, int fock_slice, int density_variant);
+int xcq_batch_set_schedule(xcq_batch* b, int fock_slice, int density_variant);
 
 #ifdef __cplusplus
 }

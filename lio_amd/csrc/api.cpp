@@ -503,6 +503,25 @@ long long xcq_batch_arena_bytes(xcq_batch* b) {
   return b ? (long long)b->engine->arena_bytes() : -1;
 }
 
+static xcq::Tuning schedule(xcq::Tuning t, int fock_slice, int density_variant) {
+  if (fock_slice > 0) t.fock_slice = fock_slice;
+  if (density_variant >= 0) t.density_variant = density_variant;
+  return t;
+}
+
+int xcq_set_schedule(xcq_context* ctx, int fock_slice, int density_variant) {
+  return guarded([&] {
+    ctx->engine->set_tuning(schedule(ctx->engine->tuning(), fock_slice, density_variant));
+  });
+}
+
+int xcq_batch_set_schedule(xcq_batch* b, int fock_slice, int density_variant) {
+  return guarded([&] {
+    b->engine->set_tuning(schedule(b->engine->tuning(), fock_slice, density_variant));
+    load_batch(b);
+  });
+}
+
 int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, const int* slices,
                    const int* variants, int ncand, double* ms_out, int* best) {
   return guarded([&] {

@@ -85,6 +85,19 @@ struct DevGroup {
   int mol;             // molecule of the batch
   int pad2_;
   long long force_off; // force partials: one 3 x m_pad block per 64-point block
+  int shell_off;       // radial-Hessian layout: first shell in the shell table
+  int nsh_pad;         // its shells, padded to a multiple of 4
+  long long rad_off;   // radial planes R0, R1, R2: 3 x npts_pad x nsh_pad
+};
+
+// Per expanded function of a group in the radial-Hessian layout: its
+// centre, the factor relating its contraction to its shell's first function
+// (the d xx/yy/zz normalisation) and (shell, lx, ly, lz) packed as
+// shell | lx << 16 | ly << 20 | lz << 24.
+struct FnMeta {
+  double cx, cy, cz;
+  float scale;
+  int code;
 };
 
 // One molecule of a batch: its functions are [fn_base, fn_base + M) of the
@@ -555,17 +568,34 @@ __device__ inline void split3(const float (&x)[8], bf8& h, bf8& m, bf8& l) {
   mx = fmaxf(mx, __shfl_xor(mx, 32));
   int e;
   (void)frexpf(mx, &e);
+  // Adding and removing 1.5 * 2^(e+15) rounds a value of magnitude below 2^e
+  // to the nearest multiple of 2^(e-8) (ties to even): the sum lies in
+  // [2^(e+15), 2^(e+16)), whose spacing is 2^(e-8).  The same with
+  // 1.5 * 2^(e+7) and 1.5 * 2^(e-1) takes the remainders to multiples of
+  // 2^(e-16) and 2^(e-24).  Every piece then has at most 9 significant bits,
+  // so its BF16 form is the upper half of its FP32 bits.
+  const float c0 = ldexpf(1.5f, e + 15), c1 = ldexpf(1.5f, e + 7), c2 = ldexpf(1.5f, e - 1);
+  unsigned hb[8], mb[8], lb[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const float xs = ldexpf(x[q], 8 - e);  // |xs| < 256, exact
-    const float a0 = rintf(xs);
-    const float r1 = (xs - a0) * 256.f;    // exact, |r1| <= 128
-    const float a1 = rintf(r1);
-    const float a2 = rintf((r1 - a1) * 256.f);
-    h[q] = (__bf16)ldexpf(a0, e - 8);
-    m[q] = (__bf16)ldexpf(a1, e - 16);
-    l[q] = (__bf16)ldexpf(a2, e - 24);
+    const float vh = (x[q] + c0) - c0;
+    const float r = x[q] - vh;  // exact
+    const float vm = (r + c1) - c1;
+    const float vl = ((r - vm) + c2) - c2;
+    hb[q] = __builtin_bit_cast(unsigned, vh);
+    mb[q] = __builtin_bit_cast(unsigned, vm);
+    lb[q] = __builtin_bit_cast(unsigned, vl);
   }
+  u4 H, Mi, L;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    H[p] = (hb[2 * p + 1] & 0xffff0000u) | (hb[2 * p] >> 16);
+    Mi[p] = (mb[2 * p + 1] & 0xffff0000u) | (mb[2 * p] >> 16);
+    L[p] = (lb[2 * p + 1] & 0xffff0000u) | (lb[2 * p] >> 16);
+  }
+  h = __builtin_bit_cast(bf8, H);
+  m = __builtin_bit_cast(bf8, Mi);
+  l = __builtin_bit_cast(bf8, L);
 }
 
 // 16x16x32 BF16 MFMA (measured on the MI355X): lane l supplies
@@ -625,6 +655,73 @@ __global__ __launch_bounds__(256) void k_eval_pm(const GatherItem* __restrict__ 
   float* out = vals + g.val_off + e;
   const long long plane = (long long)g.m_pad * g.npts_pad;
   for (int q = 0; q < NV; ++q) out[q * plane] = (float)v[q];
+}
+
+// Radial sums of every shell of a group at every point (radial-Hessian
+// layout): R0 = sum_k c_k exp(-a_k r^2), R1 = -2 sum_k c_k a_k exp(..),
+// R2 = 4 sum_k c_k a_k^2 exp(..), with the contraction of the shell's first
+// function and primitives with a r^2 > 70 skipped (g2g/cpu/functions.cpp:
+// 56-82).  One (point, shell) per thread, shells contiguous per point.
+__global__ __launch_bounds__(256) void k_eval_radial(const GatherItem* __restrict__ items,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const int* __restrict__ shells,
+                                                     const DevFn* __restrict__ bas,
+                                                     const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz,
+                                                     float* __restrict__ rad) {
+  const GatherItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const long long nplane = (long long)g.npts_pad * g.nsh_pad;
+  const int e = it.start + threadIdx.x;
+  if (e >= nplane) return;
+  const int p = e / g.nsh_pad, sh = e % g.nsh_pad;
+  const int f = shells[g.shell_off + sh];
+  double t = 0.0, tg = 0.0, th = 0.0;
+  if (f >= 0) {
+    const DevFn& b = bas[f];
+    const int q = g.pt_off + p;
+    const double vx = px[q] - b.cx, vy = py[q] - b.cy, vz = pz[q] - b.cz;
+    const double r2 = vx * vx + vy * vy + vz * vz;
+    for (int k = 0; k < b.ncont; ++k) {
+      const double a = b.a[k];
+      if (a * r2 > 70.0) continue;
+      const double ex = exp(-a * r2) * b.c[k];
+      t += ex;
+      tg += ex * a;
+      th += ex * (a * a);
+    }
+  }
+  float* out = rad + g.rad_off + e;
+  out[0] = (float)t;
+  out[nplane] = (float)(-2.0 * tg);
+  out[2 * nplane] = (float)(4.0 * th);
+}
+
+// The six second derivatives (xx, yy, zz, xy, xz, yz) of function meta at a
+// point from its shell's radial sums and the offset v from its centre:
+// x^lx y^ly z^lz g(r) with g' terms in R1 and g'' terms in R2.
+__device__ inline void radial_hessian(const FnMeta& mt, float vx, float vy, float vz, float r0,
+                                      float r1, float r2, float (&h)[6]) {
+  const int lx = (mt.code >> 16) & 3, ly = (mt.code >> 20) & 3, lz = (mt.code >> 24) & 3;
+  auto ax = [](float v, int l, float& m, float& d, float& dd) {
+    m = l == 0 ? 1.f : (l == 1 ? v : v * v);
+    d = l == 0 ? 0.f : (l == 1 ? 1.f : 2.f * v);
+    dd = l == 2 ? 2.f : 0.f;
+  };
+  float Xm, Xd, Xdd, Ym, Yd, Ydd, Zm, Zd, Zdd;
+  ax(vx, lx, Xm, Xd, Xdd);
+  ax(vy, ly, Ym, Yd, Ydd);
+  ax(vz, lz, Zm, Zd, Zdd);
+  const float P = Xm * Ym * Zm;
+  const float Px = Xd * Ym * Zm, Py = Xm * Yd * Zm, Pz = Xm * Ym * Zd;
+  const float s = mt.scale;
+  h[0] = s * (Xdd * Ym * Zm * r0 + 2.f * Px * vx * r1 + P * (vx * vx * r2 + r1));
+  h[1] = s * (Xm * Ydd * Zm * r0 + 2.f * Py * vy * r1 + P * (vy * vy * r2 + r1));
+  h[2] = s * (Xm * Ym * Zdd * r0 + 2.f * Pz * vz * r1 + P * (vz * vz * r2 + r1));
+  h[3] = s * (Xd * Yd * Zm * r0 + (Px * vy + Py * vx) * r1 + P * vx * vy * r2);
+  h[4] = s * (Xd * Ym * Zd * r0 + (Px * vz + Pz * vx) * r1 + P * vx * vz * r2);
+  h[5] = s * (Xm * Yd * Zd * r0 + (Py * vz + Pz * vy) * r1 + P * vy * vz * r2);
 }
 
 // Density block P_g as three BF16 piece planes [piece][j][k] (true symmetric
@@ -797,7 +894,11 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
 // by each of the four waves; the wider chunk halves the re-reads of the B
 // operand.  B operand (split in registers), epilogue and reductions as in
 // k_density_bf.
-template <int NT, int ROWS>
+// RAD: radial-Hessian layout -- phi and its gradient are planes, the six
+// second derivatives are rebuilt from the shells' radial sums (FP32) and the
+// functions' angular factors in the epilogue, which then reads 4 planes
+// instead of 10.
+template <int NT, int ROWS, bool RAD = false>
 __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const DensityItem* __restrict__ items,
                                                      const DevGroup* __restrict__ groups,
                                                      const unsigned short* __restrict__ pg3,
@@ -806,7 +907,12 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
                                                      double* __restrict__ factor,
                                                      double* __restrict__ epart, int iexch,
                                                      int spin, double* __restrict__ factor_b,
-                                                     double* __restrict__ sums, long long sstride) {
+                                                     double* __restrict__ sums, long long sstride,
+                                                     const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz,
+                                                     const FnMeta* __restrict__ fmeta,
+                                                     const float* __restrict__ rad) {
   constexpr int NS = (NT == 4) ? 10 : 1;
   constexpr int RT = ROWS / 16;
   constexpr int UNITS = 3 * ROWS * 4;      // 16-byte units of one staged tile
@@ -824,6 +930,12 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
   const int pt = active ? p0 + lrow : it.p0;  // inactive waves still stage tiles
   const float* Vp = vals + g.val_off + (long long)pt * mp;  // this lane's point
   const unsigned short* Pp = pg3 + 3 * g.pg_off;
+  double ppx = 0.0, ppy = 0.0, ppz = 0.0;
+  if constexpr (RAD) {
+    ppx = px[g.pt_off + pt];
+    ppy = py[g.pt_off + pt];
+    ppz = pz[g.pt_off + pt];
+  }
   double s[NS];
   for (int k = 0; k < NS; ++k) s[k] = 0.0;
   // staging map: 16-byte unit c = threadIdx.x + 256 i of a tile is piece
@@ -887,8 +999,25 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       const float* vj = Vp + c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
       if constexpr (NT == 4) {
         fl4 v[10];
+        if constexpr (RAD) {
 #pragma unroll
-        for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+          for (int q = 0; q < 4; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+          const long long rplane = (long long)np * g.nsh_pad;
+          const float* R = rad + g.rad_off + (long long)pt * g.nsh_pad;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const FnMeta mt = fmeta[g.fn_off + c0 + 16 * rt + 4 * lk + i];
+            const int sh = mt.code & 0xffff;
+            float h[6];
+            radial_hessian(mt, (float)(ppx - mt.cx), (float)(ppy - mt.cy), (float)(ppz - mt.cz),
+                           R[sh], R[rplane + sh], R[2 * rplane + sh], h);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[4 + q][i] = h[q];
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
@@ -1709,6 +1838,12 @@ struct DeviceGrid {
   int* afn_off = nullptr;         // functions of each batch atom
   int* afn = nullptr;
   double* fout = nullptr;         // forces, natom x 3 (atom-major)
+  bool radial = false;            // radial-Hessian layout (split-BF16 variant 5)
+  int* shells = nullptr;          // first (batch) function of every group shell
+  FnMeta* fmeta = nullptr;        // per padded function of every group
+  float* rad = nullptr;           // radial planes
+  GatherItem* ritems = nullptr;   // radial evaluation blocks
+  int nrad = 0;
   std::vector<int> mol_atom;      // first batch atom of each molecule, then natom
   double* h_rho = nullptr;
   double* h_F = nullptr;
@@ -1815,7 +1950,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->nmol = nmol;
   dg->fock_slice = tuning_.fock_slice;
   dg->density_variant = tuning_.density_variant;
-  if (dg->density_variant < 0 || dg->density_variant > 4)
+
+  if (dg->density_variant < 0 || dg->density_variant > 5)
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
@@ -1829,13 +1965,16 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
                              "functional there and reads unset factors)");
   }
   if (dg->open && prec == Precision::BF16X6 && dg->density_variant != 1 &&
-      dg->density_variant != 2) {
+      dg->density_variant != 2 && dg->density_variant != 5) {
     delete dg;
-    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1 or 2");
+    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2 or 5");
   }
   const int nspin = dg->open ? 2 : 1;
-  // LDA keeps the gradient too: the XC forces need it
-  dg->nval = dg->gga ? 10 : 4;
+  // LDA keeps the gradient too: the XC forces need it.  The radial-Hessian
+  // layout keeps phi and its gradient as planes and rebuilds the second
+  // derivatives from per-shell radial sums.
+  dg->radial = prec == Precision::BF16X6 && tuning_.density_variant == 5;
+  dg->nval = (dg->gga && !dg->radial) ? 10 : 4;
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const bool bf = prec == Precision::BF16X6;
@@ -1900,7 +2039,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   std::vector<DevGroup> groups(G);
   std::vector<int> fnl;
   std::vector<double> hx, hy, hz, hw;
-  long long pt = 0, vo = 0, po = 0, fo = 0, go = 0;
+  long long pt = 0, vo = 0, po = 0, fo = 0, go = 0, ro = 0;
+  std::vector<int> shells;
+  std::vector<FnMeta> fmeta;
   int fock_base = 0;
   std::vector<std::vector<int2>> membership(M);
   for (int g = 0, k = 0, gk = 0; g < G; ++g, ++gk) {
@@ -1934,6 +2075,48 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       fnl.push_back(f);
       if (f >= 0) membership[f].push_back(make_int2(g, i));
     }
+    d.shell_off = (int)shells.size();
+    d.nsh_pad = 0;
+    d.rad_off = ro;
+    if (dg->radial) {
+      const Molecule& mol = *mols[k];
+      int first = -1;
+      for (int i = 0; i < d.m_pad; ++i) {
+        FnMeta mt{0.0, 0.0, 0.0, 0.f, 0};
+        if (i < m) {
+          const int fm = part.fn[part.fn_off[gk] + i], f = hmol[k].fn_base + fm;
+          const int comp = fm < mol.ns ? 0
+                           : fm < mol.ns + 3 * mol.np ? (fm - mol.ns) % 3
+                                                      : (fm - mol.ns - 3 * mol.np) % 6;
+          if (comp == 0) {
+            first = f;
+            shells.push_back(f);
+          }
+          if (first < 0) throw std::runtime_error("g2g: a group starts inside a shell");
+          const DevFn& b = bas[f];
+          const DevFn& b0 = bas[first];
+          const double ratio = b.c[0] / b0.c[0];
+          bool same = b.ncont == b0.ncont && b.cx == b0.cx && b.cy == b0.cy && b.cz == b0.cz;
+          for (int q = 0; same && q < b.ncont; ++q)
+            same = b.a[q] == b0.a[q] &&
+                   std::fabs(b.c[q] - ratio * b0.c[q]) <= 1e-12 * std::fabs(b.c[q]);
+          if (!same)
+            throw std::runtime_error("g2g: density variant 5 needs the functions of a shell to "
+                                     "share centre, exponents and contraction");
+          mt.cx = b.cx;
+          mt.cy = b.cy;
+          mt.cz = b.cz;
+          mt.scale = (float)ratio;
+          mt.code = ((int)shells.size() - 1 - d.shell_off) | (b.lx << 16) | (b.ly << 20) |
+                    (b.lz << 24);
+        }
+        fmeta.push_back(mt);
+      }
+      const int nsh = (int)shells.size() - d.shell_off;
+      d.nsh_pad = (nsh + 3) / 4 * 4;
+      for (int q = nsh; q < d.nsh_pad; ++q) shells.push_back(-1);
+      ro += 3LL * d.npts_pad * d.nsh_pad;
+    }
     for (int p = 0; !dpts && p < d.npts_pad; ++p) {
       const int src = part.pt_off[gk] + (p < n ? p : 0);  // padding repeats point 0 with w = 0
       hx.push_back(part.x[src]);
@@ -1957,7 +2140,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return groups[a].m_pad > groups[b].m_pad; });
   std::vector<EvalItem> eitems;
-  std::vector<GatherItem> gitems, pmitems;
+  std::vector<GatherItem> gitems, pmitems, ritems;
   // split-BF16 variant 3: groups of more than 64 functions run the row-split
   // kernel on 16-point blocks (listed first), the rest 64-point blocks
   const bool rowsplit = bf && (tuning_.density_variant == 3 || tuning_.density_variant == 4);
@@ -1970,6 +2153,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
     if (bf)
       for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += 256) pmitems.push_back(GatherItem{g, e0});
+    if (dg->radial)
+      for (int e0 = 0; e0 < d.nsh_pad * d.npts_pad; e0 += 256) ritems.push_back(GatherItem{g, e0});
     if (rowsplit && d.m_pad > kRowSplitMin)
       for (int p0 = 0; p0 < d.npts_pad; p0 += 16) ditems.push_back(DensityItem{g, p0});
     else
@@ -2002,6 +2187,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->neval = (int)eitems.size();
   dg->ngather = (int)gitems.size();
   dg->npm = (int)pmitems.size();
+  dg->nrad = (int)ritems.size();
   dg->ndensity = (int)ditems.size();
   dg->nforce = (int)citems.size();
   dg->ngroup_tiles = (int)gtiles.size();
@@ -2051,6 +2237,10 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->pg2 = c.take<char>(dg->open ? po * pbytes : 0);
     dg->factor_b = c.take<double>(dg->open ? pt : 0);
     dg->sums = c.take<double>(dg->open ? 10 * pt : 0);
+    dg->shells = c.take<int>(shells.size() + 1);
+    dg->fmeta = c.take<FnMeta>(fmeta.size() + 1);
+    dg->rad = c.take<float>(ro);
+    dg->ritems = c.take<GatherItem>(ritems.size());
   };
   Carver sizing{nullptr};
   plan(sizing);
@@ -2121,6 +2311,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->fn_mol, fn_mol.data(), fn_mol.size() * sizeof(int));
   up(dg->eidx, eidx.data(), eidx.size() * sizeof(int));
   up(dg->citems, citems.data(), citems.size() * sizeof(DensityItem));
+  up(dg->shells, shells.data(), shells.size() * sizeof(int));
+  up(dg->fmeta, fmeta.data(), fmeta.size() * sizeof(FnMeta));
+  up(dg->ritems, ritems.data(), ritems.size() * sizeof(GatherItem));
   up(dg->afn_off, afn_off.data(), afn_off.size() * sizeof(int));
   up(dg->afn, afn.data(), afn.size() * sizeof(int));
   if (dpts) {  // points come from the device rebuild: compact them into place
@@ -2161,7 +2354,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     } else {
       float* v = static_cast<float*>(dg->vals);
       if (dg->npm > 0) {
-        if (dg->gga)
+        if (dg->gga && !dg->radial)
           hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
         else
@@ -2170,6 +2363,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       }
       hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
                          dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->phiT, 1);
+      if (dg->nrad > 0)
+        hipLaunchKernelGGL(k_eval_radial, dim3(dg->nrad), block, 0, s, dg->ritems, dg->groups,
+                           dg->shells, dg->bas, dg->px, dg->py, dg->pz, dg->rad);
     }
     XCQ_HIP(hipGetLastError());
   }
@@ -2397,20 +2593,29 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
     } else if (g.density_variant == 2) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+    } else if (g.density_variant == 5) {
+      if (g.gga)
+        hipLaunchKernelGGL((k_density_bf2<4, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
+                           g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+      else
+        hipLaunchKernelGGL((k_density_bf2<1, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
+                           g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
     } else if (g.density_variant == 3 || g.density_variant == 4) {
       const int nr = g.nrowsplit, ns = g.ndensity - nr;
       const DensityItem* si = g.ditems + nr;
@@ -2425,7 +2630,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
       } else {
         if (nr > 0)
           hipLaunchKernelGGL((k_density_bf3<1, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
@@ -2433,7 +2638,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
       }
     } else {
       if (g.gga)

@@ -149,7 +149,10 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 //               in LDS; 3 = 2 for groups of up to 64 functions and, above,
 //               16-point blocks whose waves split the rows (6 row tiles a
 //               wave per pass) and share the B operand through LDS; 4 = 3
-//               with 4 row tiles a wave per pass.
+//               with 4 row tiles a wave per pass; 5 = 2 in the radial-Hessian
+//               layout (phi and its gradient as planes, the second
+//               derivatives rebuilt from per-shell radial sums: 4 value
+//               planes instead of 10 in HBM, more arithmetic per point).
 struct Tuning {
   int fock_slice = 1024;
   int density_variant = 2;

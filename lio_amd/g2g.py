@@ -115,6 +115,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_batch_arena_bytes.restype = C.c_longlong
     L.xcq_batch_tune.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.c_int, _dp, C.POINTER(C.c_int)]
+    L.xcq_set_schedule.argtypes = [vp, C.c_int, C.c_int]
+    L.xcq_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     _LIB = L
     return L
 
@@ -318,6 +320,10 @@ class Context:
         _check(self.L.xcq_collect(self.h, _ptr(F, C.c_double), C.byref(info)), "xcq_collect")
         return info.energy, F, dict(electrons=info.electrons)
 
+    def set_schedule(self, fock_slice=0, density_variant=-1):
+        """Kernel schedule for the next build_grid (0 / -1 keep a knob)."""
+        _check(self.L.xcq_set_schedule(self.h, fock_slice, density_variant), "xcq_set_schedule")
+
     def release_device(self):
         """Frees the device memory of this molecule; build_grid restores it."""
         _check(self.L.xcq_release_device(self.h), "xcq_release_device")
@@ -510,6 +516,11 @@ class Batch:
 
     def arena_bytes(self) -> int:
         return int(self.L.xcq_batch_arena_bytes(self.h))
+
+    def set_schedule(self, fock_slice=0, density_variant=-1):
+        """Sets the kernel schedule and reloads (0 / -1 keep a knob)."""
+        _check(self.L.xcq_batch_set_schedule(self.h, fock_slice, density_variant),
+               "xcq_batch_set_schedule")
 
     def tune(self, d_rho: int, d_fock: int, slices=(256, 512, 1024, 2048), variants=None,
              reps: int = 5):

@@ -85,8 +85,10 @@ def test_batch_graph_replays_match_eager():
 
 
 def test_batch_tuner_over_density_variants():
-    """Split-BF16: both density kernels give the same result to FP32-GEMM
-    round-off, and the tuner keeps one of them."""
+    """Split-BF16: every density kernel gives the same result to FP32-GEMM
+    round-off (the kernels add the K steps' products in different orders into
+    FP32 sums: 1e-7, a tenth of the 1e-6 bar against FP64), and the tuner
+    keeps one of them."""
     torch = _torch()
     from lio_amd import g2g
     sys_ = _systems()
@@ -94,13 +96,14 @@ def test_batch_tuner_over_density_variants():
     rho = torch.from_numpy(b.packed_density()).cuda()
     F = torch.zeros_like(rho)
     res = {}
-    for v in (0, 1):
+    for v in (0, 1, 2, 3, 5):
         b.tune(rho.data_ptr(), F.data_ptr(), slices=(512,), variants=(v,), reps=1)
         res[v] = _batch_solve(b, torch)
-    (E0, N0, F0), (E1, N1, F1) = res[0], res[1]
-    print("variants", E0, E1, flush=True)
-    assert np.max(np.abs(E0 - E1)) < 1e-9 and np.max(np.abs(N0 - N1)) < 1e-9
-    assert np.max(np.abs(F0 - F1)) < 1e-9
+    E0, N0, F0 = res[0]
+    for v, (E1, N1, F1) in res.items():
+        print("variant", v, E1, flush=True)
+        assert np.max(np.abs(E0 - E1)) < 1e-7 and np.max(np.abs(N0 - N1)) < 1e-7
+        assert np.max(np.abs(F0 - F1)) < 1e-7
     best, table = b.tune(rho.data_ptr(), F.data_ptr(), slices=(512, 512), variants=(0, 1), reps=2)
     assert best in table
     b.close()

@@ -68,6 +68,24 @@ def test_open_shell_forces(name, precision):
     assert abs(E - gold["energy_forces"]) <= tol and df <= tol
 
 
+def test_open_shell_radial_hessian_layout():
+    """Open shell on split-BF16 density variant 5 (radial-Hessian layout)."""
+    _gpu()
+    from lio_amd import g2g
+    name = "mol25_s103_pbe_open2"
+    gold = load(name)
+    s = system_for(name)
+    ctx = g2g.Context(s, precision="bf16x6")
+    ctx.set_schedule(density_variant=5)
+    ctx.build_grid()
+    E, Fa, Fb, _, info = ctx.solve_open()
+    ctx.close()
+    assert abs(E - gold["energy"]) <= TOL_MIXED
+    assert np.max(np.abs(Fa - gold["fock"])) <= TOL_MIXED
+    assert np.max(np.abs(Fb - gold["fock_b"])) <= TOL_MIXED
+    assert abs(info["electrons"] - gold["electrons"]) <= TOL_MIXED
+
+
 def test_g2g_abi_open_shell():
     """g2g_parameter_init_ with OPEN: the solve reads rhoalpha/rhobeta and adds
     onto both Fock vectors; type 2 adds the forces."""

@@ -75,3 +75,27 @@ def test_precision_switch_rebuilds_values():
         E64b, F64b, _ = ctx.solve()
         assert E64 == E64b and np.array_equal(F64, F64b)
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["c0_pbe", "mol25_s103_pbe", "c1_pbe"])
+def test_radial_hessian_layout_within_tolerance(name):
+    """Split-BF16 density variant 5 keeps phi and its gradient as planes and
+    rebuilds the second derivatives from per-shell radial sums: the same bar
+    against FP64, with a smaller value arena than the ten-plane layout."""
+    _gpu()
+    from lio_amd import g2g
+    gold = load(name)
+    s = system_for(name)
+    ctx = g2g.Context(s, precision="bf16x6")
+    info10 = ctx.build_grid()
+    ctx.set_schedule(density_variant=5)
+    info4 = ctx.build_grid()
+    E, F, info = ctx.solve()
+    ctx.close()
+    dE = E - gold["energy"]
+    dF = float(np.max(np.abs(F - gold["fock"])))
+    dN = info["electrons"] - gold["electrons"]
+    print("%s radial-Hessian dE %.2e dF %.2e dN %.2e arena %.2f -> %.2f GB" %
+          (name, dE, dF, dN, info10["arena_bytes"] / 1e9, info4["arena_bytes"] / 1e9), flush=True)
+    assert abs(dE) <= TOL and dF <= TOL and abs(dN) <= TOL_N["bf16x6"]
+    assert info4["arena_bytes"] < info10["arena_bytes"]
