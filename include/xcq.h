@@ -120,6 +120,13 @@ int xcq_solve(xcq_context* ctx, const double* rho_packed, double* fock_accum,
  * copies and no synchronisation: the work is queued on the context stream
  * (xcq_stream); energy and electrons are read back by xcq_sync_energy. */
 int xcq_solve_device(xcq_context* ctx, const double* d_rho_packed, double* d_fock);
+/* Reserves `bytes` of device memory once per process (device -1: the current
+ * one); the grid arenas of every context and batch on that device are then
+ * carved from it on the host, with no device allocation per grid, and a grid
+ * that does not fit fails with "out of memory".  Kept until the process ends.
+ * No reference counterpart: for serving loops that rebuild and evict grids of
+ * different sizes at every request. */
+int xcq_reserve_slab(int device, unsigned long long bytes);
 /* Energy and XC forces (g2g_solve_groups_ computation types 2/3,
  * g2g/cpu/iteration.cpp:213-268 for the force, g2g/partition.cpp:516-526 for
  * the accumulation): forces_accum is the caller's natom x 3 column-major

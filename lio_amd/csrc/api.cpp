@@ -308,6 +308,12 @@ int xcq_solve_open(xcq_context* ctx, const double* rho_a, const double* rho_b, d
   });
 }
 
+int xcq_reserve_slab(int device, unsigned long long bytes) {
+  return guarded([&] {
+    xcq::Engine::reserve_slab(device, (size_t)bytes);
+  });
+}
+
 int xcq_solve_device(xcq_context* ctx, const double* d_rho, double* d_fock) {
   return guarded([&] {
     if (ctx->grid_type < 0) throw std::runtime_error("g2g: solve before the grid was built");

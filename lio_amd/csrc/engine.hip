@@ -29,6 +29,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -57,7 +59,8 @@ constexpr int kDensPts = 64;     // points per k_density block (16 per wave)
 constexpr int kRowSplitMin = 64; // split-BF16 row-split density above this m_pad
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
 constexpr int kSliceDefault = 1024;  // points per Fock slice (split-K over blocks)
-constexpr int kEvalPts = 256;    // points per evaluation block
+constexpr int kEvalPts = 256;    // threads per evaluation block
+constexpr int kEvalSpan = 4096;  // points (or elements) one evaluation block covers
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -219,20 +222,21 @@ __global__ __launch_bounds__(256) void k_eval_basis(const EvalItem* __restrict__
                                                     T* __restrict__ vals, int to_phi) {
   const EvalItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
-  const int p = it.p0 + threadIdx.x;
-  if (p >= g.npts_pad) return;
   const int f = fnl[g.fn_off + it.k];
-  double v[NV];
-  if (f >= 0) {
-    const int q = g.pt_off + p;
-    eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
-  } else {
-    for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  }
-  // to_phi: the function-major phi plane of the split-BF16 path
-  T* out = vals + (to_phi ? g.phi_off : g.val_off) + (long long)it.k * g.npts_pad + p;
   const long long plane = (long long)g.m_pad * g.npts_pad;
-  for (int k = 0; k < NV; ++k) out[k * plane] = (T)v[k];
+  const int pe = min(it.p0 + kEvalSpan, g.npts_pad);
+  for (int p = it.p0 + threadIdx.x; p < pe; p += kEvalPts) {
+    double v[NV];
+    if (f >= 0) {
+      const int q = g.pt_off + p;
+      eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
+    } else {
+      for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    }
+    // to_phi: the function-major phi plane of the split-BF16 path
+    T* out = vals + (to_phi ? g.phi_off : g.val_off) + (long long)it.k * g.npts_pad + p;
+    for (int k = 0; k < NV; ++k) out[k * plane] = (T)v[k];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -641,20 +645,21 @@ __global__ __launch_bounds__(256) void k_eval_pm(const GatherItem* __restrict__ 
                                                  float* __restrict__ vals) {
   const GatherItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
-  const int e = it.start + threadIdx.x;
-  if (e >= g.npts_pad * g.m_pad) return;
-  const int p = e / g.m_pad, k = e % g.m_pad;
-  const int f = fnl[g.fn_off + k];
-  double v[NV];
-  if (f >= 0) {
-    const int q = g.pt_off + p;
-    eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
-  } else {
-    for (int q = 0; q < NV; ++q) v[q] = 0.0;
-  }
-  float* out = vals + g.val_off + e;
   const long long plane = (long long)g.m_pad * g.npts_pad;
-  for (int q = 0; q < NV; ++q) out[q * plane] = (float)v[q];
+  const int ee = (int)min((long long)it.start + kEvalSpan, plane);
+  for (int e = it.start + threadIdx.x; e < ee; e += kEvalPts) {
+    const int p = e / g.m_pad, k = e % g.m_pad;
+    const int f = fnl[g.fn_off + k];
+    double v[NV];
+    if (f >= 0) {
+      const int q = g.pt_off + p;
+      eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
+    } else {
+      for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    }
+    float* out = vals + g.val_off + e;
+    for (int q = 0; q < NV; ++q) out[q * plane] = (float)v[q];
+  }
 }
 
 // Radial sums of every shell of a group at every point (radial-Hessian
@@ -673,29 +678,30 @@ __global__ __launch_bounds__(256) void k_eval_radial(const GatherItem* __restric
   const GatherItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const long long nplane = (long long)g.npts_pad * g.nsh_pad;
-  const int e = it.start + threadIdx.x;
-  if (e >= nplane) return;
-  const int p = e / g.nsh_pad, sh = e % g.nsh_pad;
-  const int f = shells[g.shell_off + sh];
-  double t = 0.0, tg = 0.0, th = 0.0;
-  if (f >= 0) {
-    const DevFn& b = bas[f];
-    const int q = g.pt_off + p;
-    const double vx = px[q] - b.cx, vy = py[q] - b.cy, vz = pz[q] - b.cz;
-    const double r2 = vx * vx + vy * vy + vz * vz;
-    for (int k = 0; k < b.ncont; ++k) {
-      const double a = b.a[k];
-      if (a * r2 > 70.0) continue;
-      const double ex = exp(-a * r2) * b.c[k];
-      t += ex;
-      tg += ex * a;
-      th += ex * (a * a);
+  const int ee = (int)min((long long)it.start + kEvalSpan, nplane);
+  for (int e = it.start + threadIdx.x; e < ee; e += kEvalPts) {
+    const int p = e / g.nsh_pad, sh = e % g.nsh_pad;
+    const int f = shells[g.shell_off + sh];
+    double t = 0.0, tg = 0.0, th = 0.0;
+    if (f >= 0) {
+      const DevFn& b = bas[f];
+      const int q = g.pt_off + p;
+      const double vx = px[q] - b.cx, vy = py[q] - b.cy, vz = pz[q] - b.cz;
+      const double r2 = vx * vx + vy * vy + vz * vz;
+      for (int k = 0; k < b.ncont; ++k) {
+        const double a = b.a[k];
+        if (a * r2 > 70.0) continue;
+        const double ex = exp(-a * r2) * b.c[k];
+        t += ex;
+        tg += ex * a;
+        th += ex * (a * a);
+      }
     }
+    float* out = rad + g.rad_off + e;
+    out[0] = (float)t;
+    out[nplane] = (float)(-2.0 * tg);
+    out[2 * nplane] = (float)(4.0 * th);
   }
-  float* out = rad + g.rad_off + e;
-  out[0] = (float)t;
-  out[nplane] = (float)(-2.0 * tg);
-  out[2 * nplane] = (float)(4.0 * th);
 }
 
 // The six second derivatives (xx, yy, zz, xy, xz, yz) of function meta at a
@@ -1876,6 +1882,10 @@ Engine::Engine(int device) {
   hipStream_t s;
   XCQ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   stream_ = s;
+  hipMemPool_t pool;
+  XCQ_HIP(hipDeviceGetDefaultMemPool(&pool, device_));
+  uint64_t keep = ~0ull;  // freed blocks stay in the pool until the process ends
+  XCQ_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
   for (auto& e : events_) {
     hipEvent_t ev;
     XCQ_HIP(hipEventCreate(&ev));
@@ -1883,9 +1893,93 @@ Engine::Engine(int device) {
   }
 }
 
-static void release(DeviceGrid* g) {
+// Device memory comes from the device's stream-ordered pool (hipMallocAsync /
+// hipFreeAsync on the engine stream, release threshold at its maximum): a
+// freed arena stays in the pool for the next grid of any engine on the
+// device, and freeing does not wait for the whole device, which would stall
+// the builds of other molecules in flight on other streams.
+// Optional device slab: one allocation per device made up front
+// (Engine::reserve_slab), from which every engine on that device carves its
+// grid arenas with a first-fit free list on the host.  A serving loop that
+// rebuilds and evicts grids of different sizes at every request then makes
+// no device allocation at all.  A block is returned only when its engine's
+// stream is idle (after a synchronisation), so another engine may reuse it
+// at once.
+struct Slab {
+  std::mutex mu;
+  char* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length, coalesced
+  bool owns(const void* p) const {
+    return base && p >= base && static_cast<const char*>(p) < base + size;
+  }
+  char* take(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> l(mu);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > bytes) free_[off + bytes] = len - bytes;
+      return base + off;
+    }
+    return nullptr;
+  }
+  void give(char* p, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> l(mu);
+    size_t off = p - base, len = bytes;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && off + len == next->first) {
+      len += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        off = prev->first;
+        len += prev->second;
+        free_.erase(prev);
+      }
+    }
+    free_[off] = len;
+  }
+};
+static Slab g_slab[64];
+
+// Allocates `want` bytes (at least `need`) from the pool on stream s.  When
+// the pool's cached blocks do not fit the request it returns them to the
+// device and tries again, then falls back to `need`.  Returns the size
+// obtained.
+static size_t pool_alloc(void** p, size_t want, size_t need, hipStream_t s, int device) {
+  if (hipMallocAsync(p, want, s) == hipSuccess) return want;
+  (void)hipGetLastError();
+  hipMemPool_t pool;
+  XCQ_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+  XCQ_HIP(hipStreamSynchronize(s));
+  XCQ_HIP(hipMemPoolTrimTo(pool, 0));
+  if (hipMallocAsync(p, want, s) == hipSuccess) return want;
+  (void)hipGetLastError();
+  XCQ_HIP(hipMallocAsync(p, need, s));
+  return need;
+}
+
+// Returns an arena to the slab it came from (after the stream's work on it
+// has finished) or to the pool (stream-ordered).  Never throws: destructors
+// call it.
+static void free_arena(char* a, size_t cap, int device, hipStream_t s) {
+  if (!a) return;
+  Slab& sl = g_slab[device];
+  if (sl.owns(a)) {  // another engine may take the block at once
+    (void)hipStreamSynchronize(s);
+    sl.give(a, cap);
+  } else
+    (void)hipFreeAsync(a, s);
+}
+
+static void release(DeviceGrid* g, hipStream_t s, int device) {
   if (!g) return;
-  if (g->arena) (void)hipFree(g->arena);
+  free_arena(g->arena, g->arena_cap, device, s);
   if (g->h_rho) (void)hipHostFree(g->h_rho);
   if (g->h_F) (void)hipHostFree(g->h_F);
   if (g->h_e) (void)hipHostFree(g->h_e);
@@ -1896,8 +1990,9 @@ Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
   drop_graph();
-  release(grid_);
-  if (stage_) (void)hipFree(stage_);
+  release(grid_, (hipStream_t)stream_, device_);
+  if (stage_) (void)hipFreeAsync(stage_, (hipStream_t)stream_);
+  if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
   for (auto& e : events_)
     if (e) (void)hipEventDestroy((hipEvent_t)e);
   if (stream_) (void)hipStreamDestroy((hipStream_t)stream_);
@@ -1935,8 +2030,10 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   // of other molecules in flight on other streams
   struct Old {
     DeviceGrid* g;
-    ~Old() { release(g); }
-  } old{grid_};
+    hipStream_t s;
+    int device;
+    ~Old() { release(g, s, device); }
+  } old{grid_, (hipStream_t)stream_, device_};
   grid_ = nullptr;
   const int nmol = (int)mols.size();
   if (nmol == 0 || parts.size() != mols.size())
@@ -2149,12 +2246,14 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     const DevGroup& d = groups[g];
     for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) citems.push_back(DensityItem{g, p0});
     for (int k = 0; k < d.m_pad; ++k)
-      for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalPts) eitems.push_back(EvalItem{g, k, p0, 0});
+      for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalSpan) eitems.push_back(EvalItem{g, k, p0, 0});
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
     if (bf)
-      for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += 256) pmitems.push_back(GatherItem{g, e0});
+      for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += kEvalSpan)
+        pmitems.push_back(GatherItem{g, e0});
     if (dg->radial)
-      for (int e0 = 0; e0 < d.nsh_pad * d.npts_pad; e0 += 256) ritems.push_back(GatherItem{g, e0});
+      for (int e0 = 0; e0 < d.nsh_pad * d.npts_pad; e0 += kEvalSpan)
+        ritems.push_back(GatherItem{g, e0});
     if (rowsplit && d.m_pad > kRowSplitMin)
       for (int p0 = 0; p0 < d.npts_pad; p0 += 16) ditems.push_back(DensityItem{g, p0});
     else
@@ -2251,11 +2350,21 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     old.g->arena = nullptr;
   } else {
     const size_t reclaim = old.g && old.g->arena ? old.g->arena_cap : 0;
-    // a grown grid gets 25 % headroom so that the next steps reuse it
-    const size_t want = reclaim ? dg->arena_bytes + dg->arena_bytes / 4 : dg->arena_bytes;
+    // headroom (12.5 %, 25 % for a grid that outgrew its arena) so that the
+    // grids of the next steps of a moving molecule fit the same arena
+    const size_t want = dg->arena_bytes + (reclaim ? dg->arena_bytes / 4 : dg->arena_bytes / 8);
     size_t free_b = 0, total_b = 0;
+    const bool slab = g_slab[device_].base != nullptr;
     XCQ_HIP(hipMemGetInfo(&free_b, &total_b));
-    if (dg->arena_bytes > free_b + reclaim) {
+    {  // memory freed into the pool is available again
+      hipMemPool_t pool;
+      uint64_t reserved = 0, used = 0;
+      XCQ_HIP(hipDeviceGetDefaultMemPool(&pool, device_));
+      XCQ_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved));
+      XCQ_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used));
+      if (reserved > used) free_b += reserved - used;
+    }
+    if (!slab && dg->arena_bytes > free_b + reclaim) {
       const size_t need = dg->arena_bytes;
       delete dg;
       throw std::runtime_error("g2g: grid needs " + std::to_string(need >> 20) +
@@ -2263,12 +2372,25 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
                                " MiB free");
     }
     if (reclaim) {
-      (void)hipFree(old.g->arena);
+      free_arena(old.g->arena, old.g->arena_cap, device_, (hipStream_t)stream_);
       old.g->arena = nullptr;
     }
-    const size_t alloc = want <= free_b + reclaim ? want : dg->arena_bytes;
-    XCQ_HIP(hipMalloc(&dg->arena, alloc));
-    dg->arena_cap = alloc;
+    Slab& sl = g_slab[device_];
+    if (sl.base) {  // carve from the slab: exact size, no device call
+      dg->arena = sl.take(dg->arena_bytes);
+      if (!dg->arena) {
+        const size_t need = dg->arena_bytes;
+        delete dg;
+        throw std::runtime_error("g2g: device slab full: grid needs " +
+                                 std::to_string(need >> 20) + " MiB (out of memory)");
+      }
+      dg->arena_cap = dg->arena_bytes;
+    } else {
+      const size_t alloc = want <= free_b + reclaim ? want : dg->arena_bytes;
+      void* a = nullptr;
+      dg->arena_cap = pool_alloc(&a, alloc, dg->arena_bytes, (hipStream_t)stream_, device_);
+      dg->arena = static_cast<char*>(a);
+    }
   }
   Carver carve{dg->arena};
   plan(carve);
@@ -2453,12 +2575,11 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   // the staging buffer is kept (grow-only) across rebuilds, for the same
   // reason as the arena
   if (stage_cap_ < plan.used + 256) {
-    if (stage_) (void)hipFree(stage_);
+    if (stage_) XCQ_HIP(hipFreeAsync(stage_, (hipStream_t)stream_));
     stage_ = nullptr;
     stage_cap_ = 0;
     const size_t want = plan.used + 256 + (plan.used + 256) / 4;
-    XCQ_HIP(hipMalloc(&stage_, want));
-    stage_cap_ = want;
+    stage_cap_ = pool_alloc(&stage_, want, plan.used + 256, (hipStream_t)stream_, device_);
   }
   char* base = static_cast<char*>(stage_);
   Carver c{base};
@@ -2767,13 +2888,16 @@ void Engine::capture_graph(const double* d_rho, double* d_fock) {
 
 void Engine::unload() {
   XCQ_HIP(hipSetDevice(device_));
-  XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
+  hipStream_t s = (hipStream_t)stream_;
+  XCQ_HIP(hipStreamSynchronize(s));
   drop_graph();
-  release(grid_);
+  release(grid_, s, device_);
   grid_ = nullptr;
-  if (stage_) (void)hipFree(stage_);
+  if (stage_) (void)hipFreeAsync(stage_, s);
   stage_ = nullptr;
   stage_cap_ = 0;
+  // the frees have run: the memory is back in the pool for any stream
+  XCQ_HIP(hipStreamSynchronize(s));
 }
 
 void Engine::replay_graph() {
@@ -2911,5 +3035,28 @@ SolveResult Engine::solve_open(const double* rho_a, const double* rho_b, double*
 }
 
 bool Engine::open_shell() const { return grid_ && grid_->open; }
+
+void Engine::reserve_slab(int device, size_t bytes) {
+  if (device < 0) XCQ_HIP(hipGetDevice(&device));
+  if (device >= 64) throw std::runtime_error("g2g: device out of range");
+  Slab& sl = g_slab[device];
+  std::lock_guard<std::mutex> l(sl.mu);
+  if (sl.base) {
+    if (sl.size >= bytes) return;  // a second server in the same process
+    throw std::runtime_error("g2g: the device slab is already reserved with " +
+                             std::to_string(sl.size >> 20) + " MiB");
+  }
+  int cur = 0;
+  XCQ_HIP(hipGetDevice(&cur));
+  XCQ_HIP(hipSetDevice(device));
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, bytes);
+  (void)hipSetDevice(cur);
+  XCQ_HIP(e);
+  sl.base = static_cast<char*>(p);
+  sl.size = bytes;
+  sl.free_.clear();
+  sl.free_[0] = bytes;
+}
 
 }  // namespace xcq

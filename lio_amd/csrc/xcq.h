@@ -209,6 +209,10 @@ class Engine {
   SolveResult solve_open(const double* rho_a, const double* rho_b, double* fock_a, double* fock_b,
                          bool want_energy, double* forces_accum = nullptr, int forces_ld = 0);
   bool open_shell() const;
+  // Reserves `bytes` of device memory once for the process; from then on the
+  // grid arenas of every engine on that device are carved from it on the
+  // host (no device allocation per grid).  Kept until the process ends.
+  static void reserve_slab(int device, size_t bytes);  // device -1: the current one
   // Queues a build from a packed host density (copied to pinned memory at
   // once) with its copy-back; idle() tells when it has finished and
   // collect() waits for it and adds F onto fock_accum (may be null).

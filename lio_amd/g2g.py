@@ -116,9 +116,17 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_batch_tune.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.c_int, _dp, C.POINTER(C.c_int)]
     L.xcq_set_schedule.argtypes = [vp, C.c_int, C.c_int]
+    L.xcq_reserve_slab.argtypes = [C.c_int, C.c_ulonglong]
     L.xcq_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     _LIB = L
     return L
+
+
+def reserve_slab(nbytes: int, device: int = -1):
+    """Reserves one device slab for all grid arenas of this process (see
+    xcq_reserve_slab); once per process and device."""
+    L = load_library()
+    _check(L.xcq_reserve_slab(device, nbytes), "xcq_reserve_slab")
 
 
 def _check(rc: int, what: str):

@@ -9,9 +9,10 @@ step.  The loop does continuous batching on one GPU:
   device between its steps, and are rebuilt in place (host staging, device
   Becke weights, device basis evaluation) when the step moves the atoms;
 * requests are admitted as soon as a slot frees up and their replica is not
-  already being served; one host thread queues each admitted build on its
-  replica's stream without waiting and polls for completions, so the builds
-  of all admitted requests are in flight together and overlap on the device;
+  already being served; each admitted request runs on a worker thread (its
+  host staging, its grid rebuild and its build on the replica's stream), so
+  the host work of several requests and the device work of all admitted
+  requests overlap;
 * admission is bounded by an HBM budget: idle replicas' arenas are released,
   least recently used first, when a new one would not fit;
 * latency is arrival -> result on the host, reported as p50/p99 together with
@@ -23,7 +24,9 @@ same numbers alone and in the mix.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import dataclasses
+import threading
 import time
 
 import numpy as np
@@ -44,6 +47,9 @@ class Request:
     energy: float = 0.0
     electrons: float = 0.0
     fock: np.ndarray | None = None
+    stage_ms: float = 0.0   # host staging of the new grid
+    load_ms: float = 0.0    # device rebuild (allocation, uploads, weights, basis values)
+    solve_ms: float = 0.0   # the build itself, queued to collected
 
 
 class Replica:
@@ -59,11 +65,29 @@ class Replica:
         self.rho_sigma = rho_sigma
         self.trace0 = float(np.trace(system.P))
         self._step = None  # cached walk position (step, positions, density)
+        self._lock = threading.Lock()
+        self._ahead = None  # (step, future) of a state computed ahead
 
     def state(self, step: int) -> systems.System:
         """The system at MD step ``step`` (step 0 is the initial one).  The walk
         is a fixed seeded sequence, so a step is the same whoever asks; asking
         for the steps in order advances it by one step each time."""
+        with self._lock:
+            return self._state(step)
+
+    def prefetch(self, step: int, pool: concurrent.futures.Executor):
+        """Starts computing state(step) on a worker thread (the next MD step
+        of a replica is known as soon as its current one is taken)."""
+        self._ahead = (step, pool.submit(self.state, step))
+
+    def take(self, step: int) -> systems.System:
+        """state(step), from the prefetched computation when there is one."""
+        ahead, self._ahead = self._ahead, None
+        if ahead is not None and ahead[0] == step:
+            return ahead[1].result()
+        return self.state(step)
+
+    def _state(self, step: int) -> systems.System:
         if self._step is None or step < self._step:
             self._rng = np.random.default_rng((self.seed, 0))
             self._step, self._pos, self._P = 0, self.base.pos.copy(), self.base.P.copy()
@@ -81,18 +105,22 @@ class Replica:
 class Server:
     """Continuous-batching XC Fock server over a set of replicas (one GPU).
 
-    One host thread drives everything: it admits a request when a slot is
-    free, its replica has no build in flight and its arena fits the HBM
-    budget; it rebuilds the replica's grid for the new geometry and queues
-    the build on the replica's own stream without waiting; it then polls the
-    streams and collects finished builds.  Builds of different replicas are
-    in flight together on their streams and overlap on the device.
+    The main thread admits a request when a slot is free, its replica has no
+    build in flight and its arena fits the HBM budget, and hands it to a
+    worker thread, which rebuilds the replica's grid for the new geometry
+    (host staging, device Becke weights and basis values), queues the build
+    on the replica's own stream and waits for it.  The library releases the
+    interpreter lock in every call, so the workers' host staging and their
+    device builds overlap, and builds of different replicas are in flight
+    together on their streams.
     """
 
     def __init__(self, replicas: list[Replica], precision: str = "bf16x6", slots: int = 8,
-                 hbm_budget: int | None = None, grid_mode: str = "device"):
+                 hbm_budget: int | None = None, grid_mode: str = "device",
+                 density_variant: int | None = None, slab: bool = False):
         self.replicas = replicas
         self.precision = precision
+        self.density_variant = density_variant
         self.grid_mode = grid_mode
         self.slots = slots
         self.budget = hbm_budget
@@ -102,16 +130,26 @@ class Server:
         self.last_use = [0.0] * len(replicas)
         self.busy = [False] * len(replicas)
         self.evictions = 0
+        self.retries = 0
+        if slab:  # one device slab of the whole budget: no allocation per rebuild
+            if hbm_budget is None:
+                raise ValueError("a slab needs an HBM budget")
+            g2g.reserve_slab(hbm_budget)  # builds that found the device full and were queued again
+        self.workers = concurrent.futures.ThreadPoolExecutor(max_workers=slots)
+        # the next MD step's inputs of each replica are computed ahead on
+        # other threads (numpy releases the interpreter lock in its loops)
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=4)
+        self.last_step = [0] * len(replicas)
 
     # -- device memory -------------------------------------------------------
     def _estimate(self, r: int) -> int:
         """Arena bytes a replica needs: its last grid's size, or for a replica
-        never built a bound from its size (points ~ 7000 per atom, a tenth of
-        its functions significant per point, 10 FP64 planes)."""
+        never built a bound from its size (points ~ 7000 per atom, a quarter
+        of its functions significant per point, 10 FP64 planes)."""
         if self.arena[r]:
             return int(1.1 * self.arena[r])
         s = self.replicas[r].base
-        return int(7000 * s.natom * max(1, s.M // 10) * 10 * 8 * 1.3)
+        return int(7000 * s.natom * max(1, s.M // 4) * 10 * 8 * 1.3)
 
     def _resident_bytes(self) -> int:
         return sum(a for a, res in zip(self.arena, self.resident) if res)
@@ -137,24 +175,46 @@ class Server:
         return True
 
     # -- one request ---------------------------------------------------------
-    def _start(self, req: Request):
-        r = req.replica
-        s = self.replicas[r].state(req.step)
-        ctx = self.ctx[r]
-        if ctx is None:
-            ctx = g2g.Context(precision=self.precision, grid_mode=self.grid_mode)
-            self.ctx[r] = ctx
+    @staticmethod
+    def _build(req: Request, ctx: g2g.Context, s: systems.System):
+        """Worker thread: rebuild the grid for the step's geometry and run
+        the build; returns (arena bytes, energy, F, info), or None when the
+        device ran out of memory (the replica then holds no arena)."""
         req.start = time.perf_counter()
         ctx.set_system(s)
-        info = ctx.build_grid()
-        self.arena[r] = int(info["arena_bytes"])
+        try:
+            info = ctx.build_grid()
+        except RuntimeError as e:
+            if "out of memory" in str(e) or "MiB free" in str(e):
+                return None
+            raise
+        t1 = time.perf_counter()
+        ctx.solve_async(s.rho_packed)
+        E, F, inf = ctx.collect()
+        req.solve_ms = 1e3 * (time.perf_counter() - t1)
+        req.stage_ms = 1e3 * info["grid_seconds"]
+        req.load_ms = 1e3 * info["upload_seconds"]
+        return int(info["arena_bytes"]), E, F, inf
+
+    def _start(self, req: Request):
+        r = req.replica
+        s = self.replicas[r].take(req.step)
+        if req.step < self.last_step[r]:
+            self.replicas[r].prefetch(req.step + 1, self.pool)
+        if self.ctx[r] is None:
+            self.ctx[r] = g2g.Context(precision=self.precision, grid_mode=self.grid_mode)
+            if self.density_variant is not None:
+                self.ctx[r].set_schedule(density_variant=self.density_variant)
+        if not self.resident[r]:
+            self.arena[r] = self.arena[r] or self._estimate(r)
         self.resident[r] = True
         self.busy[r] = True
-        ctx.solve_async(s.rho_packed)
+        return self.workers.submit(self._build, req, self.ctx[r], s)
 
-    def _finish(self, req: Request, keep_fock: bool):
+    def _finish(self, req: Request, result, keep_fock: bool):
         r = req.replica
-        E, F, info = self.ctx[r].collect()
+        arena, E, F, info = result
+        self.arena[r] = arena
         req.energy, req.electrons = E, info["electrons"]
         req.fock = F if keep_fock else None
         req.done = time.perf_counter()
@@ -171,8 +231,9 @@ class Server:
         for i, (t, r) in enumerate(arrivals):
             step[r] += 1
             reqs.append(Request(i, r, step[r], t))
+        self.last_step = list(step)
         pending: list[Request] = []
-        inflight: list[Request] = []
+        inflight: dict = {}
         done: list[Request] = []
         t0 = time.perf_counter()
         nxt = 0
@@ -185,12 +246,23 @@ class Server:
                 pending.append(reqs[nxt])
                 nxt += 1
             progressed = False
-            for req in list(inflight):  # collect finished builds
-                if self.ctx[req.replica].done():
-                    self._finish(req, keep_fock)
-                    inflight.remove(req)
-                    done.append(req)
-                    progressed = True
+            for fut in [f for f in inflight if f.done()]:  # collect finished builds
+                req = inflight.pop(fut)
+                res = fut.result()
+                progressed = True
+                if res is None:  # out of device memory: back in the queue, a larger estimate
+                    r = req.replica
+                    self.busy[r] = False
+                    self.resident[r] = False
+                    if not inflight and not any(self.resident):
+                        raise RuntimeError("replica %d does not fit the device alone" % r)
+                    self.arena[r] = int(1.5 * max(self.arena[r], self._estimate(r)))
+                    self.retries += 1
+                    pending.append(req)
+                    pending.sort(key=lambda q: q.rid)
+                    continue
+                self._finish(req, res, keep_fock)
+                done.append(req)
             for req in list(pending):   # admit: slot, replica idle, in order, memory
                 if len(inflight) >= self.slots:
                     break
@@ -200,8 +272,7 @@ class Server:
                 if not self._make_room(r):
                     continue
                 pending.remove(req)
-                self._start(req)
-                inflight.append(req)
+                inflight[self._start(req)] = req
                 progressed = True
             if not progressed:
                 time.sleep(0.0002)
@@ -215,9 +286,13 @@ class Server:
                                "mean": float(lat.mean())},
                 "service_ms": {"p50": float(np.percentile(svc, 50)),
                                "p99": float(np.percentile(svc, 99))},
-                "evictions": self.evictions}
+                "phase_ms_p50": {k: float(np.median([getattr(q, k + "_ms") for q in done]))
+                                 for k in ("stage", "load", "solve")},
+                "evictions": self.evictions, "retries": self.retries}
 
     def close(self):
+        self.workers.shutdown(wait=True)
+        self.pool.shutdown(wait=True)
         for c in self.ctx:
             if c is not None:
                 c.close()

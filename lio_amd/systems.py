@@ -25,6 +25,7 @@ with the Obara-Saika recursion for Cartesian Gaussians.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import functools
 import json
 import math
@@ -316,16 +317,13 @@ def overlap(pos: np.ndarray, b: Basis, normalize: bool = True) -> np.ndarray:
                         v = v + (j - 1) * inv2p * s[(i, j - 2)]
                 s[(i, j)] = v
         li = pl[:, d]
-        out = np.zeros_like(Sxyz)
-        for i in range(3):
-            for j in range(3):
-                mask = (li[:, None] == i) & (li[None, :] == j)
-                out = np.where(mask, s[(i, j)], out)
-        Sxyz *= out
+        pick = 3 * li[:, None] + li[None, :]
+        Sxyz *= np.choose(pick, [s[(i, j)] for i in range(3) for j in range(3)])
     Sp = Sxyz * pc[:, None] * pc[None, :]
-    T = np.zeros((b.M, len(pa)))
-    T[owner, np.arange(len(pa))] = 1.0
-    S = T @ Sp @ T.T
+    # contract primitives to functions: each function's primitives are
+    # consecutive, so both sums are segment sums
+    starts = np.flatnonzero(np.r_[True, owner[1:] != owner[:-1]])
+    S = np.add.reduceat(np.add.reduceat(Sp, starts, axis=0), starts, axis=1)
     return 0.5 * (S + S.T)
 
 
@@ -575,4 +573,7 @@ def molecule_batch(n: int = 64, seed: int = 3, atoms=(3, 60)) -> list[System]:
     molecule k is grown with seed 1000 * seed + k."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(atoms[0], atoms[1] + 1, size=n)
-    return [random_molecule(int(sizes[k]), 1000 * seed + k) for k in range(n)]
+    # threads: the overlap and density set-up is numpy array work, which runs
+    # outside the interpreter lock; each molecule depends on its seed only
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        return list(ex.map(lambda k: random_molecule(int(sizes[k]), 1000 * seed + k), range(n)))
