@@ -70,6 +70,10 @@ typedef struct {
   double eval_ms;             /* basis evaluation kernel (HIP events) */
   unsigned long long arena_bytes;
   int density_tiles, fock_items;
+  /* host wall time (ms) of the load's phases: point weights (staging copies,
+   * Becke weights, group counts; device-built grids only), planning (work
+   * lists and arena), uploads, basis evaluation until the stream is idle */
+  double load_ms[4];
 } xcq_grid_info;
 
 typedef struct {

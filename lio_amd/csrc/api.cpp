@@ -252,6 +252,7 @@ int xcq_query_grid(xcq_context* ctx, xcq_grid_info* o) {
     o->arena_bytes = ctx->engine->arena_bytes();
     o->density_tiles = (int)(pad / 16);
     o->fock_items = 0;
+    ctx->engine->load_phases(o->load_ms);
   });
 }
 

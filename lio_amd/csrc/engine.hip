@@ -1628,21 +1628,17 @@ struct StGroup {
 };
 struct StMol {
   int atom_base, natom;  // atoms in the batch-wide atom arrays
-  long long rij_off;     // natom x natom distances
+  long long rij_off;     // natom x natom distances and size terms
 };
 
-// Becke cell function with the atom-size adjustment (g2g/cpu/weight.cpp:
-// 13-112, the same expression order as the host partition).
-__device__ inline double becke_cell(double dj, double dk, double rjk, double rmj, double rmk) {
+// Size term of the Becke cell for atoms j, k: chi = rm_j / rm_k,
+// chi <- (chi - 1) / (chi + 1), aw = chi / (chi^2 - 1) (g2g/cpu/weight.cpp:
+// 13-112, the same operations in the same order as the host partition).
+static double becke_size_term(double rmj, double rmk) {
 #pragma clang fp contract(off)
-  double u = (dj - dk) / rjk;
   double chi = rmj / rmk;
   chi = (chi - 1.0) / (chi + 1.0);
-  u += (chi / (chi * chi - 1.0)) * (1.0 - u * u);
-  u = 1.5 * u - 0.5 * (u * u * u);
-  u = 1.5 * u - 0.5 * (u * u * u);
-  u = 1.5 * u - 0.5 * (u * u * u);
-  return 0.5 * (1.0 - u);
+  return chi / (chi * chi - 1.0);
 }
 
 __device__ inline double dist3(double x, double y, double z, double ax, double ay, double az) {
@@ -1651,66 +1647,104 @@ __device__ inline double dist3(double x, double y, double z, double ax, double a
   return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// One staged point per thread: weight *= P_own / sum_j P_j over the group's
-// nuclei, the own atom's cell taken separately when it is not among them.
-// Distances are recomputed instead of stored per point (the host keeps them
-// in an array), with identical values.
+// Becke cell function with the pair's size term aw = chi / (chi^2 - 1)
+// taken from a table (the host computes it with the same operations), so a
+// factor costs one division instead of three.
+__device__ inline double becke_cell_w(double dj, double dk, double rjk, double aw) {
+#pragma clang fp contract(off)
+  double u = (dj - dk) / rjk;
+  u += aw * (1.0 - u * u);
+  u = 1.5 * u - 0.5 * (u * u * u);
+  u = 1.5 * u - 0.5 * (u * u * u);
+  u = 1.5 * u - 0.5 * (u * u * u);
+  return 0.5 * (1.0 - u);
+}
+
+// One staged point per thread, one block per 256 points of one staged
+// group: weight *= P_own / sum_j P_j over the group's nuclei, the own atom's
+// cell taken separately when it is not among them.  The group is uniform in
+// the block, so its nuclei, their coordinates and the pair tables are
+// wave-uniform loads.  Distances are recomputed instead of stored per point
+// (the host keeps them in an array), with identical values.
+//
+// Each P_j is first tested against the point's nearest nucleus n: when
+// s(mu_jn) is exactly zero the product is zero whatever the order of its
+// other factors (all lie in [0, 1]), and a wave runs the k-term product only
+// for nuclei whose cells reach one of its points.  Zero terms added to the
+// sum change nothing, and a zero own cell gives weight 0 however far the sum
+// got, so the weight is bitwise the plain double loop's
+// (g2g/cpu/weight.cpp:13-112 without the early exit).
 __global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ groups,
-                                               const int* __restrict__ pgroup,
+                                               const int2* __restrict__ items,
                                                const StMol* __restrict__ mols,
                                                const int* __restrict__ gatoms,
                                                const int* __restrict__ owner,
                                                const double* __restrict__ ax,
                                                const double* __restrict__ ay,
                                                const double* __restrict__ az,
-                                               const double* __restrict__ arm,
+                                               const double* __restrict__ aw,
                                                const double* __restrict__ rij,
                                                const double* __restrict__ sx,
                                                const double* __restrict__ sy,
                                                const double* __restrict__ sz,
-                                               double* __restrict__ sw, int npoints) {
+                                               double* __restrict__ sw) {
 #pragma clang fp contract(off)
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= npoints) return;
-  const StGroup G = groups[pgroup[p]];
+  const int2 it = items[blockIdx.x];
+  const int gi = __builtin_amdgcn_readfirstlane(it.x);
+  const int p = __builtin_amdgcn_readfirstlane(it.y) + (int)threadIdx.x;
+  const StGroup G = groups[gi];
+  if (p >= G.pt_off + G.npts) return;
   const StMol m = mols[G.mol];
   const int* at = gatoms + G.atom_off;
   const int k = G.natoms, own = owner[p], na = m.natom;
   const double* A = ax + m.atom_base;
   const double* B = ay + m.atom_base;
   const double* C = az + m.atom_base;
-  const double* R = arm + m.atom_base;
+  const double* W = aw + m.rij_off;
   const double* D = rij + m.rij_off;
   const double x = sx[p], y = sy[p], z = sz[p];
+  int an = -1;
+  double dn = 0.0;
   bool own_in = false;
+  for (int q = 0; q < k; ++q) {
+    const int aq = at[q];
+    const double dq = dist3(x, y, z, A[aq], B[aq], C[aq]);
+    own_in |= (aq == own);
+    if (an < 0 || dq < dn) {
+      dn = dq;
+      an = aq;
+    }
+  }
   double total = 0.0, mine = 0.0;
   for (int j = 0; j < k; ++j) {
     const int aj = at[j];
-    own_in |= (aj == own);
     const double dj = dist3(x, y, z, A[aj], B[aj], C[aj]);
+    const double* Dj = D + (long long)aj * na;
+    const double* Wj = W + (long long)aj * na;
+    if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
     double prod = 1.0;
     for (int q = 0; q < k; ++q) {
       const int ak = at[q];
       if (ak == aj) continue;
-      prod *= becke_cell(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), D[(long long)aj * na + ak], R[aj],
-                         R[ak]);
+      prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
       if (prod == 0.0) break;
     }
-    if (aj == own) {
-      mine = prod;
-      if (mine == 0.0) break;
-    }
+    if (aj == own) mine = prod;
     total += prod;
   }
-  if (!own_in) {  // (the j loop only breaks at the own atom, so own_in is complete)
+  if (!own_in && k > 0) {
     const double down = dist3(x, y, z, A[own], B[own], C[own]);
+    const double* Do = D + (long long)own * na;
+    const double* Wo = W + (long long)own * na;
     mine = 1.0;
-    for (int q = 0; q < k; ++q) {
-      const int ak = at[q];
-      mine *= becke_cell(down, dist3(x, y, z, A[ak], B[ak], C[ak]), D[(long long)own * na + ak],
-                         R[own], R[ak]);
-      if (mine == 0.0) break;
-    }
+    if (becke_cell_w(down, dn, Do[an], Wo[an]) == 0.0)
+      mine = 0.0;
+    else
+      for (int q = 0; q < k; ++q) {
+        const int ak = at[q];
+        mine *= becke_cell_w(down, dist3(x, y, z, A[ak], B[ak], C[ak]), Do[ak], Wo[ak]);
+        if (mine == 0.0) break;
+      }
   }
   sw[p] *= (total == 0.0 ? 0.0 : mine / total);
 }
@@ -2023,6 +2057,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
                              const DevicePoints* dpts) {
   XCQ_HIP(hipSetDevice(device_));
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
+  const double t_plan = wall();
+  load_ms_[0] = load_ms_[1] = load_ms_[2] = load_ms_[3] = 0.0;
   drop_graph();
   // the previous grid's arena and pinned buffers are reused when large enough
   // (a rebuilt grid of a moved molecule is about the same size): freeing
@@ -2394,6 +2430,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   Carver carve{dg->arena};
   plan(carve);
+  const double t_up = wall();
   if (old.g && old.g->h_rho && old.g->pinned_packed >= (size_t)(nspin * packed) &&
       old.g->pinned_mols >= nmol) {
     dg->h_rho = old.g->h_rho;
@@ -2454,6 +2491,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   // basis values of every group, kept for all solves on this grid
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
+  const double t_eval = wall();
   XCQ_HIP(hipEventRecord(ev[6], s));
   if (dg->neval > 0) {
     const dim3 grid(dg->neval), block(kEvalPts);
@@ -2497,20 +2535,26 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   XCQ_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
   dg->eval_ms = ms;
   grid_ = dg;
+  const double t_end = wall();
+  load_ms_[1] = 1e3 * (t_up - t_plan);
+  load_ms_[2] = 1e3 * (t_eval - t_up);
+  load_ms_[3] = 1e3 * (t_end - t_eval);
 }
 
 std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*>& mols,
                                               const std::vector<const StagedPartition*>& staged,
                                               const Options& opt, Precision prec) {
   XCQ_HIP(hipSetDevice(device_));
+  const double t_start = wall();
   const int nmol = (int)mols.size();
   if (nmol == 0 || staged.size() != mols.size())
     throw std::runtime_error("g2g: a staged batch needs one staging per molecule");
   // batch-wide staging tables
   std::vector<StGroup> sg;
   std::vector<StMol> sm(nmol);
-  std::vector<int> pgroup, owner, gatoms;
-  std::vector<double> sx, sy, sz, sw, ax, ay, az, arm, rij;
+  std::vector<int> owner, gatoms;
+  std::vector<int2> bitems;  // (staged group, first point) per Becke block
+  std::vector<double> sx, sy, sz, sw, ax, ay, az, aw, rij;
   for (int k = 0; k < nmol; ++k) {
     const Molecule& mol = *mols[k];
     const StagedPartition& st = *staged[k];
@@ -2520,8 +2564,9 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
     ax.insert(ax.end(), mol.x.begin(), mol.x.end());
     ay.insert(ay.end(), mol.y.begin(), mol.y.end());
     az.insert(az.end(), mol.z.begin(), mol.z.end());
-    arm.insert(arm.end(), mol.rm.begin(), mol.rm.end());
     rij.insert(rij.end(), st.rij.begin(), st.rij.end());
+    for (int a = 0; a < mol.natom; ++a)
+      for (int b = 0; b < mol.natom; ++b) aw.push_back(becke_size_term(mol.rm[a], mol.rm[b]));
     const GridPartition& gp = st.groups;
     for (int g = 0; g < gp.ngroups; ++g) {
       StGroup G;
@@ -2539,7 +2584,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
       sz.insert(sz.end(), gp.z.begin() + b, gp.z.begin() + b + G.npts);
       sw.insert(sw.end(), gp.w.begin() + b, gp.w.begin() + b + G.npts);
       owner.insert(owner.end(), st.owner.begin() + b, st.owner.begin() + b + G.npts);
-      pgroup.insert(pgroup.end(), G.npts, (int)sg.size());
+      for (int o = 0; o < G.npts; o += 256) bitems.push_back(make_int2((int)sg.size(), G.pt_off + o));
       sg.push_back(G);
     }
   }
@@ -2550,21 +2595,22 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   struct Stage {
     StGroup* groups;
     StMol* mols;
-    int *pgroup, *owner, *gatoms, *count;
-    double *ax, *ay, *az, *arm, *rij, *x, *y, *z, *w;
+    int2* bitems;
+    int *owner, *gatoms, *count;
+    double *ax, *ay, *az, *aw, *rij, *x, *y, *z, *w;
   } d{};
   Carver plan{nullptr};
   auto carve = [&](Carver& c) {
     d.groups = c.take<StGroup>(nst + 1);
     d.mols = c.take<StMol>(nmol);
-    d.pgroup = c.take<int>(npts + 1);
+    d.bitems = c.take<int2>(bitems.size() + 1);
     d.owner = c.take<int>(npts + 1);
     d.gatoms = c.take<int>(gatoms.size() + 1);
     d.count = c.take<int>(nst + 1);
     d.ax = c.take<double>(ax.size());
     d.ay = c.take<double>(ay.size());
     d.az = c.take<double>(az.size());
-    d.arm = c.take<double>(arm.size());
+    d.aw = c.take<double>(aw.size());
     d.rij = c.take<double>(rij.size());
     d.x = c.take<double>(npts);
     d.y = c.take<double>(npts);
@@ -2590,22 +2636,22 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   };
   up(d.groups, sg.data(), sg.size() * sizeof(StGroup));
   up(d.mols, sm.data(), sm.size() * sizeof(StMol));
-  up(d.pgroup, pgroup.data(), pgroup.size() * sizeof(int));
+  up(d.bitems, bitems.data(), bitems.size() * sizeof(int2));
   up(d.owner, owner.data(), owner.size() * sizeof(int));
   up(d.gatoms, gatoms.data(), gatoms.size() * sizeof(int));
   up(d.ax, ax.data(), ax.size() * sizeof(double));
   up(d.ay, ay.data(), ay.size() * sizeof(double));
   up(d.az, az.data(), az.size() * sizeof(double));
-  up(d.arm, arm.data(), arm.size() * sizeof(double));
+  up(d.aw, aw.data(), aw.size() * sizeof(double));
   up(d.rij, rij.data(), rij.size() * sizeof(double));
   up(d.x, sx.data(), npts * sizeof(double));
   up(d.y, sy.data(), npts * sizeof(double));
   up(d.z, sz.data(), npts * sizeof(double));
   up(d.w, sw.data(), npts * sizeof(double));
-  if (npts > 0)
-    hipLaunchKernelGGL(k_becke, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s, d.groups,
-                       d.pgroup, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.arm, d.rij, d.x,
-                       d.y, d.z, d.w, (int)npts);
+  if (!bitems.empty())
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)bitems.size()), dim3(256), 0, s, d.groups,
+                       d.bitems, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.aw, d.rij, d.x,
+                       d.y, d.z, d.w);
   const int drop = opt.remove_zero_weights ? 1 : 0;
   if (nst > 0)
     hipLaunchKernelGGL(k_group_count, dim3(nst), dim3(256), 0, s, d.groups, d.w, drop, d.count);
@@ -2614,6 +2660,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   if (nst > 0) XCQ_HIP(hipMemcpyAsync(count.data(), d.count, nst * sizeof(int),
                                       hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipStreamSynchronize(s));
+  const double weights_ms = 1e3 * (wall() - t_start);
   // the final partitions: groups keeping at least min_points_per_cube points
   std::vector<GridPartition> parts(nmol);
   DevicePoints dp;
@@ -2651,6 +2698,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   std::vector<const GridPartition*> pp;
   for (auto& p : parts) pp.push_back(&p);
   load_batch_impl(mols, pp, prec, &dp);
+  load_ms_[0] = weights_ms;
   return parts;
 }
 

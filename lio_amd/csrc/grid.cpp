@@ -98,7 +98,10 @@ inline double cell(double dj, double dk, double rjk, double rmj, double rmk) {
 }
 
 // Multiplies the staged weights by the Becke partition over the group's own
-// atoms and drops points whose weight became exactly zero.
+// atoms and drops points whose weight became exactly zero.  P_j is tested
+// first against the point's nearest atom n: a zero s(mu_jn) makes the product
+// zero whatever the order of its other factors (all in [0, 1]); otherwise the
+// product runs over all atoms in the plain order.
 void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& rij,
            const std::vector<int>& atoms, Staging& st) {
   const int n = (int)st.x.size(), k = (int)atoms.size(), na = mol.natom;
@@ -113,22 +116,28 @@ void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& r
       d = big.data();
     }
     bool own_in = false;
+    int qn = -1;
     for (int q = 0; q < k; ++q) {
       const int a = atoms[q];
       const double dx = st.x[p] - mol.x[a], dy = st.y[p] - mol.y[a], dz = st.z[p] - mol.z[a];
       d[q] = std::sqrt(dx * dx + dy * dy + dz * dz);
       own_in |= (a == own);
+      if (qn < 0 || d[q] < d[qn]) qn = q;
     }
+    const int an = qn < 0 ? -1 : atoms[qn];
     double total = 0.0, mine = 0.0;
     for (int j = 0; j < k; ++j) {
       const int aj = atoms[j];
       double prod = 1.0;
-      for (int q = 0; q < k; ++q) {
-        const int ak = atoms[q];
-        if (ak == aj) continue;
-        prod *= cell(d[j], d[q], rij[(size_t)aj * na + ak], mol.rm[aj], mol.rm[ak]);
-        if (prod == 0.0) break;
-      }
+      if (j != qn && cell(d[j], d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0)
+        prod = 0.0;
+      else
+        for (int q = 0; q < k; ++q) {
+          const int ak = atoms[q];
+          if (ak == aj) continue;
+          prod *= cell(d[j], d[q], rij[(size_t)aj * na + ak], mol.rm[aj], mol.rm[ak]);
+          if (prod == 0.0) break;
+        }
       if (aj == own) {
         mine = prod;
         if (mine == 0.0) break;
@@ -139,11 +148,14 @@ void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& r
       const double dx = st.x[p] - mol.x[own], dy = st.y[p] - mol.y[own], dz = st.z[p] - mol.z[own];
       const double down = std::sqrt(dx * dx + dy * dy + dz * dz);
       mine = 1.0;
-      for (int q = 0; q < k; ++q) {
-        const int ak = atoms[q];
-        mine *= cell(down, d[q], rij[(size_t)own * na + ak], mol.rm[own], mol.rm[ak]);
-        if (mine == 0.0) break;
-      }
+      if (k > 0 && cell(down, d[qn], rij[(size_t)own * na + an], mol.rm[own], mol.rm[an]) == 0.0)
+        mine = 0.0;
+      else
+        for (int q = 0; q < k; ++q) {
+          const int ak = atoms[q];
+          mine *= cell(down, d[q], rij[(size_t)own * na + ak], mol.rm[own], mol.rm[ak]);
+          if (mine == 0.0) break;
+        }
     }
     st.w[p] *= (total == 0.0 ? 0.0 : mine / total);
   }

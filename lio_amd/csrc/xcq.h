@@ -246,6 +246,11 @@ class Engine {
   size_t arena_bytes() const;
   // Duration (ms, HIP events) of the basis evaluation at the last load.
   double eval_ms() const;
+  // Host wall time (ms) of the last load's phases: point weights, planning,
+  // uploads, evaluation.
+  void load_phases(double ms[4]) const {
+    for (int k = 0; k < 4; ++k) ms[k] = load_ms_[k];
+  }
   void* stream() const { return stream_; }
   const DeviceGrid* grid() const { return grid_; }
 
@@ -257,6 +262,7 @@ class Engine {
   void* graph_ = nullptr;
   void* stage_ = nullptr;     // device grid rebuild staging (kept, grow-only)
   size_t stage_cap_ = 0;
+  double load_ms_[4] = {};
   Tuning tuning_;
   void drop_graph();
   void load_batch_impl(const std::vector<const Molecule*>& mols,

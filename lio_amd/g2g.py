@@ -52,7 +52,7 @@ class XcqGridInfo(C.Structure):
                 ("flops", C.c_double), ("bytes_values", C.c_double),
                 ("grid_seconds", C.c_double), ("upload_seconds", C.c_double),
                 ("eval_ms", C.c_double), ("arena_bytes", C.c_ulonglong), ("density_tiles", C.c_int),
-                ("fock_items", C.c_int)]
+                ("fock_items", C.c_int), ("load_ms", C.c_double * 4)]
 
 
 class XcqSolveInfo(C.Structure):
@@ -235,7 +235,9 @@ class Context:
     def grid_info(self) -> dict:
         gi = XcqGridInfo()
         _check(self.L.xcq_query_grid(self.h, C.byref(gi)), "xcq_query_grid")
-        return {k: getattr(gi, k) for k, _ in XcqGridInfo._fields_}
+        out = {k: getattr(gi, k) for k, _ in XcqGridInfo._fields_}
+        out["load_ms"] = list(out["load_ms"])
+        return out
 
     def groups(self):
         n = C.c_int(0)

@@ -47,9 +47,11 @@ class Request:
     energy: float = 0.0
     electrons: float = 0.0
     fock: np.ndarray | None = None
+    set_ms: float = 0.0     # problem arrays handed to the context
     stage_ms: float = 0.0   # host staging of the new grid
     load_ms: float = 0.0    # device rebuild (allocation, uploads, weights, basis values)
     solve_ms: float = 0.0   # the build itself, queued to collected
+    load_phases: list | None = None  # weights, planning, uploads, evaluation (ms)
 
 
 class Replica:
@@ -182,6 +184,7 @@ class Server:
         device ran out of memory (the replica then holds no arena)."""
         req.start = time.perf_counter()
         ctx.set_system(s)
+        req.set_ms = 1e3 * (time.perf_counter() - req.start)
         try:
             info = ctx.build_grid()
         except RuntimeError as e:
@@ -194,6 +197,7 @@ class Server:
         req.solve_ms = 1e3 * (time.perf_counter() - t1)
         req.stage_ms = 1e3 * info["grid_seconds"]
         req.load_ms = 1e3 * info["upload_seconds"]
+        req.load_phases = info["load_ms"]
         return int(info["arena_bytes"]), E, F, inf
 
     def _start(self, req: Request):
@@ -287,7 +291,9 @@ class Server:
                 "service_ms": {"p50": float(np.percentile(svc, 50)),
                                "p99": float(np.percentile(svc, 99))},
                 "phase_ms_p50": {k: float(np.median([getattr(q, k + "_ms") for q in done]))
-                                 for k in ("stage", "load", "solve")},
+                                 for k in ("set", "stage", "load", "solve")},
+                "load_phase_ms_p50": [float(np.median([q.load_phases[i] for q in done]))
+                                      for i in range(4)],
                 "evictions": self.evictions, "retries": self.retries}
 
     def close(self):
