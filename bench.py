@@ -21,10 +21,11 @@ Prints ONE JSON line:
   batch          64 independent molecules of 3-60 atoms resident together,
                  every phase one launch over all of them, replayed as one
                  hipGraph: molecules/s in both modes
-  serve          the continuous-batching replica server (reduced replica
-                 set, Poisson arrivals, every request an MD step with a new
-                 geometry and density): p50/p99 latency and builds/s in both
-                 modes
+  serve          the continuous-batching replica server: 128 QM/MM replicas
+                 of 10-200 atoms, every request an MD step with a new
+                 geometry and density (grid rebuilt on the device), grid
+                 arenas from one device slab under an HBM budget: builds/s
+                 saturated and p50/p99 latency paced, in both modes
   roofline       the dominant mixed kernel (split-BF16 density) from HIP
                  events recorded on the engine's stream
   cpu_baseline   LIO's own CPU g2g path (oracle/_ref, when built) or this
@@ -181,16 +182,42 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     return out
 
 
-def bench_serve(precision, nrep=32, nreq=96, rate=20.0, seed=4):
-    """Reduced replica set through the server: every request is one MD step."""
-    from lio_amd import serve
-    reps = serve.replica_set(nrep, seed=seed, atoms=(10, 60))
-    srv = serve.Server(reps, precision=precision, slots=8, grid_mode="device")
-    out = srv.run(serve.poisson_arrivals(nreq, rate, nrep, seed=seed))
-    srv.close()
-    return {"replicas": nrep, "atoms": "10-60", "requests": nreq, "rate_hz": rate,
-            "builds_per_s": out["builds_per_s"], "latency_ms": out["latency_ms"],
-            "service_ms": out["service_ms"], "evictions": out["evictions"]}
+def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
+    """Replica server at its production shape: nrep QM/MM replicas of
+    ``atoms`` atoms, every request one MD step (new geometry, new density,
+    grid rebuilt on the device).  Per mode two runs over the same replicas:
+    ``saturated`` (all requests queued at once: the server's throughput and
+    the latency of a backlog) and ``paced`` (Poisson arrivals at 60 % of the
+    saturated rate: p50/p99 of a loaded but stable server).  Grid arenas come
+    from one device slab sized to the free memory (no allocation per
+    request); admission keeps the resident replicas within it."""
+    import torch
+    from lio_amd import g2g, serve
+    t0 = time.perf_counter()
+    reps = serve.replica_set(nrep, seed=seed, atoms=atoms)
+    log("serve: %d replicas built in %.1f s" % (nrep, time.perf_counter() - t0))
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(0.85 * free)
+    g2g.reserve_slab(budget)
+    out = {"replicas": nrep, "atoms": "%d-%d" % atoms, "slots": slots,
+           "hbm_budget_gb": budget / 1e9, "requests": nreq}
+    keys = ("builds_per_s", "latency_ms", "service_ms", "phase_ms_p50", "load_phase_ms_p50",
+            "evictions", "retries")
+    for name, prec in (("fp64", "fp64"), ("mixed", MIXED)):
+        srv = serve.Server(reps, precision=prec, slots=slots, hbm_budget=budget,
+                           grid_mode="device", slab=True)
+        sat = srv.run(serve.poisson_arrivals(nreq, 1e4, nrep, seed=seed), max_seconds=300)
+        rate = 0.6 * sat["builds_per_s"]
+        paced = srv.run(serve.poisson_arrivals(nreq, rate, nrep, seed=seed + 1), max_seconds=300)
+        srv.close()
+        out[name] = {"saturated": {k: sat[k] for k in keys},
+                     "paced": dict(rate_hz=rate, **{k: paced[k] for k in keys})}
+        log("serve %s: %.1f builds/s saturated, paced p50 %.0f ms p99 %.0f ms" % (
+            name, sat["builds_per_s"], paced["latency_ms"]["p50"], paced["latency_ms"]["p99"]))
+    out["speedup_saturated"] = (out["mixed"]["saturated"]["builds_per_s"]
+                                / out["fp64"]["saturated"]["builds_per_s"])
+    return out
 
 
 def main():
@@ -293,7 +320,7 @@ def main():
         log("batch:", json.dumps(result["batch"]))
     if not args.no_serve:
         log("serving replicas ...")
-        result["serve"] = {"fp64": bench_serve("fp64"), "mixed": bench_serve(MIXED)}
+        result["serve"] = bench_serve()
         log("serve:", json.dumps(result["serve"]))
     if args.copy_probe:
         # 1 GiB read + 1 GiB write by one elementwise kernel; a 2 GiB write in

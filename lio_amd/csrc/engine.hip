@@ -3097,6 +3097,13 @@ void Engine::reserve_slab(int device, size_t bytes) {
   int cur = 0;
   XCQ_HIP(hipGetDevice(&cur));
   XCQ_HIP(hipSetDevice(device));
+  {  // blocks earlier grids returned to the stream-ordered pool go back first
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      (void)hipDeviceSynchronize();
+      (void)hipMemPoolTrimTo(pool, 0);
+    }
+  }
   void* p = nullptr;
   const hipError_t e = hipMalloc(&p, bytes);
   (void)hipSetDevice(cur);

@@ -133,6 +133,7 @@ class Server:
         self.busy = [False] * len(replicas)
         self.evictions = 0
         self.retries = 0
+        self.next_step = [0] * len(replicas)  # MD steps served so far, per replica
         if slab:  # one device slab of the whole budget: no allocation per rebuild
             if hbm_budget is None:
                 raise ValueError("a slab needs an HBM budget")
@@ -228,14 +229,17 @@ class Server:
     def run(self, arrivals: list[tuple[float, int]], keep_fock: bool = False,
             max_seconds: float = 600.0) -> dict:
         """Serves requests arriving at (time offset s, replica) in order; a
-        replica's requests are its consecutive MD steps.  Returns the completed
-        requests and the latency / throughput figures."""
-        step = [0] * len(self.replicas)
+        replica's requests are its consecutive MD steps, continuing from the
+        previous run on this server.  Returns the completed requests and the
+        latency / throughput figures."""
+        step = list(self.next_step)
+        ev0, rt0 = self.evictions, self.retries
         reqs = []
         for i, (t, r) in enumerate(arrivals):
             step[r] += 1
             reqs.append(Request(i, r, step[r], t))
         self.last_step = list(step)
+        self.next_step = list(step)
         pending: list[Request] = []
         inflight: dict = {}
         done: list[Request] = []
@@ -294,7 +298,7 @@ class Server:
                                  for k in ("set", "stage", "load", "solve")},
                 "load_phase_ms_p50": [float(np.median([q.load_phases[i] for q in done]))
                                       for i in range(4)],
-                "evictions": self.evictions, "retries": self.retries}
+                "evictions": self.evictions - ev0, "retries": self.retries - rt0}
 
     def close(self):
         self.workers.shutdown(wait=True)
