@@ -82,3 +82,29 @@ def test_host_partition_rejects_bad_input():
     s.iexch = 12
     with pytest.raises(RuntimeError, match="Iexch"):
         g2g.host_partition(s)
+
+
+def test_links_like_lioamber(tmp_path):
+    """A program that references every symbol lioamber's Fortran calls into
+    g2g (the list of the test above, Fortran's trailing underscore included)
+    links against libg2g.so with --no-undefined, as liblio-g2g.so does
+    (lioamber/Makefile.options:48,53); gfortran is absent, so the calls are
+    declared from C with the same linkage names."""
+    names = sorted({"g2g_init_", "g2g_set_options_", "g2g_parameter_init_",
+                    "g2g_reload_atom_positions_", "g2g_new_grid_", "g2g_solve_groups_",
+                    "g2g_deinit_", "g2g_timer_start_", "g2g_timer_stop_", "g2g_timer_pause_",
+                    "g2g_timer_sum_start_", "g2g_timer_sum_stop_", "g2g_timer_sum_pause_",
+                    "g2g_timer_clear_", "g2g_timer_summary_", "aint_query_gpu_level_",
+                    "aint_parameter_init_", "aint_deinit_", "aint_new_step_", "aint_qmmm_init_",
+                    "aint_coulomb_init_", "aint_qmmm_forces_", "aint_qmmm_fock_",
+                    "aint_coulomb_forces_", "aint_coulomb_fock_"})
+    src = tmp_path / "lio_calls.c"
+    src.write_text("".join("extern void %s(void);\n" % n for n in names)
+                   + "void* lio_table[] = {" + ", ".join("(void*)%s" % n for n in names) + "};\n"
+                   + "int main(void) { return lio_table[0] == 0; }\n")
+    lib_dir = os.path.dirname(g2g.LIB_PATH)
+    out = tmp_path / "lio_calls"
+    r = subprocess.run(["gcc", str(src), "-o", str(out), "-L" + lib_dir, "-lg2g",
+                        "-Wl,--no-undefined", "-Wl,--allow-shlib-undefined"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
