@@ -229,6 +229,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=96,
+                    help="requests per serving run (two runs per mode)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB read + 1 GiB write elementwise kernel "
                          "(calibrates HBM byte counters)")
@@ -320,7 +322,7 @@ def main():
         log("batch:", json.dumps(result["batch"]))
     if not args.no_serve:
         log("serving replicas ...")
-        result["serve"] = bench_serve()
+        result["serve"] = bench_serve(nreq=args.serve_requests)
         log("serve:", json.dumps(result["serve"]))
     if args.copy_probe:
         # 1 GiB read + 1 GiB write by one elementwise kernel; a 2 GiB write in

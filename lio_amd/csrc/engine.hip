@@ -633,9 +633,14 @@ __device__ inline void load8(const float* p, float (&x)[8]) {
 }
 
 // Point-major FP32 basis planes [value][point][function] (functions
-// contiguous) for the split-BF16 path: one (point, function) per thread.
+// contiguous) for the split-BF16 path.  A block covers 64 points and 16
+// functions of one group: each wave evaluates one function at a time for its
+// 64 points (the function record is wave-uniform), the values are transposed
+// through LDS, and each thread stores 4 consecutive functions of one point
+// per plane in one 16-byte store.
+constexpr int kPmPts = 64, kPmFns = 16;
 template <int NV>
-__global__ __launch_bounds__(256) void k_eval_pm(const GatherItem* __restrict__ items,
+__global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
                                                  const int* __restrict__ fnl,
                                                  const DevFn* __restrict__ bas,
@@ -643,23 +648,32 @@ __global__ __launch_bounds__(256) void k_eval_pm(const GatherItem* __restrict__ 
                                                  const double* __restrict__ py,
                                                  const double* __restrict__ pz,
                                                  float* __restrict__ vals) {
-  const GatherItem it = items[blockIdx.x];
+  __shared__ float sT[NV][kPmFns][kPmPts + 1];
+  const EvalItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
-  const long long plane = (long long)g.m_pad * g.npts_pad;
-  const int ee = (int)min((long long)it.start + kEvalSpan, plane);
-  for (int e = it.start + threadIdx.x; e < ee; e += kEvalPts) {
-    const int p = e / g.m_pad, k = e % g.m_pad;
-    const int f = fnl[g.fn_off + k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = it.p0 + lane;
+  const int q = g.pt_off + (p < g.npts_pad ? p : it.p0);
+  const double x = px[q], y = py[q], z = pz[q];
+  for (int kk = wave; kk < kPmFns; kk += 4) {
+    const int f = fnl[g.fn_off + it.k + kk];
     double v[NV];
     if (f >= 0) {
-      const int q = g.pt_off + p;
-      eval_basis<NV>(bas[f], px[q], py[q], pz[q], v);
+      eval_basis<NV>(bas[f], x, y, z, v);
     } else {
-      for (int q = 0; q < NV; ++q) v[q] = 0.0;
+      for (int c = 0; c < NV; ++c) v[c] = 0.0;
     }
-    float* out = vals + g.val_off + e;
-    for (int q = 0; q < NV; ++q) out[q * plane] = (float)v[q];
+    for (int c = 0; c < NV; ++c) sT[c][kk][lane] = (float)v[c];
   }
+  __syncthreads();
+  const int wp = threadIdx.x >> 2, wf = 4 * (threadIdx.x & 3);
+  const int pp = it.p0 + wp;
+  if (pp >= g.npts_pad) return;
+  const long long plane = (long long)g.m_pad * g.npts_pad;
+  float* out = vals + g.val_off + (long long)pp * g.m_pad + it.k + wf;
+  for (int c = 0; c < NV; ++c)
+    *reinterpret_cast<fl4*>(out + c * plane) =
+        fl4{sT[c][wf][wp], sT[c][wf + 1][wp], sT[c][wf + 2][wp], sT[c][wf + 3][wp]};
 }
 
 // Radial sums of every shell of a group at every point (radial-Hessian
@@ -1848,7 +1862,7 @@ struct DeviceGrid {
   DevGroup* groups = nullptr;
   EvalItem* eitems = nullptr;
   GatherItem* gitems = nullptr;
-  GatherItem* pmitems = nullptr;  // point-major evaluation blocks (split-BF16)
+  EvalItem* pmitems = nullptr;  // point-major evaluation blocks (split-BF16)
   int npm = 0;
   float* phiT = nullptr;          // function-major phi (split-BF16 Fock operand)
   DensityItem* ditems = nullptr;
@@ -2273,7 +2287,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return groups[a].m_pad > groups[b].m_pad; });
   std::vector<EvalItem> eitems;
-  std::vector<GatherItem> gitems, pmitems, ritems;
+  std::vector<GatherItem> gitems, ritems;
+  std::vector<EvalItem> pmitems;
   // split-BF16 variant 3: groups of more than 64 functions run the row-split
   // kernel on 16-point blocks (listed first), the rest 64-point blocks
   const bool rowsplit = bf && (tuning_.density_variant == 3 || tuning_.density_variant == 4);
@@ -2285,8 +2300,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalSpan) eitems.push_back(EvalItem{g, k, p0, 0});
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
     if (bf)
-      for (int e0 = 0; e0 < d.m_pad * d.npts_pad; e0 += kEvalSpan)
-        pmitems.push_back(GatherItem{g, e0});
+      for (int p0 = 0; p0 < d.npts_pad; p0 += kPmPts)
+        for (int k0 = 0; k0 < d.m_pad; k0 += kPmFns) pmitems.push_back(EvalItem{g, k0, p0, 0});
     if (dg->radial)
       for (int e0 = 0; e0 < d.nsh_pad * d.npts_pad; e0 += kEvalSpan)
         ritems.push_back(GatherItem{g, e0});
@@ -2339,7 +2354,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->groups = c.take<DevGroup>(G);
     dg->eitems = c.take<EvalItem>(eitems.size());
     dg->gitems = c.take<GatherItem>(gitems.size());
-    dg->pmitems = c.take<GatherItem>(pmitems.size());
+    dg->pmitems = c.take<EvalItem>(pmitems.size());
     dg->phiT = c.take<float>(fo);
     dg->ditems = c.take<DensityItem>(ditems.size());
     dg->fitems = c.take<FockItem>(fitems.size());
@@ -2455,7 +2470,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->groups, groups.data(), groups.size() * sizeof(DevGroup));
   up(dg->eitems, eitems.data(), eitems.size() * sizeof(EvalItem));
   up(dg->gitems, gitems.data(), gitems.size() * sizeof(GatherItem));
-  up(dg->pmitems, pmitems.data(), pmitems.size() * sizeof(GatherItem));
+  up(dg->pmitems, pmitems.data(), pmitems.size() * sizeof(EvalItem));
   up(dg->ditems, ditems.data(), ditems.size() * sizeof(DensityItem));
   up(dg->fitems, fitems.data(), fitems.size() * sizeof(FockItem));
   up(dg->gtiles, gtiles.data(), gtiles.size() * sizeof(int2));
