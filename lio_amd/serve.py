@@ -14,7 +14,9 @@ step.  The loop does continuous batching on one GPU:
   the host work of several requests and the device work of all admitted
   requests overlap;
 * admission is bounded by an HBM budget: idle replicas' arenas are released,
-  least recently used first, when a new one would not fit;
+  least recently used first, when a new one would not fit; with ``slab`` the
+  budget is reserved once as a device slab and every arena is carved from it
+  on the host, so a request makes no device allocation;
 * latency is arrival -> result on the host, reported as p50/p99 together with
   Fock builds per second.
 
@@ -132,12 +134,12 @@ class Server:
         self.last_use = [0.0] * len(replicas)
         self.busy = [False] * len(replicas)
         self.evictions = 0
-        self.retries = 0
+        self.retries = 0  # builds that found the device full and were queued again
         self.next_step = [0] * len(replicas)  # MD steps served so far, per replica
         if slab:  # one device slab of the whole budget: no allocation per rebuild
             if hbm_budget is None:
                 raise ValueError("a slab needs an HBM budget")
-            g2g.reserve_slab(hbm_budget)  # builds that found the device full and were queued again
+            g2g.reserve_slab(hbm_budget)
         self.workers = concurrent.futures.ThreadPoolExecutor(max_workers=slots)
         # the next MD step's inputs of each replica are computed ahead on
         # other threads (numpy releases the interpreter lock in its loops)
