@@ -1,7 +1,8 @@
 """The device slab (xcq_reserve_slab): grids carved from one up-front
 reservation give bitwise the results of grids allocated one by one, blocks
-return to the slab when a grid is released or outgrown, and a grid larger
-than the free slab fails with "out of memory" and leaves the slab usable.
+return to the slab when a grid is released or outgrown, a grid larger than
+the free slab fails with "out of memory" and leaves the slab usable, and the
+replica server on the slab serves every step as computed alone.
 
 The slab lasts until the process ends, so the slab side runs in a child
 process; the parent computes the same builds with per-grid allocations."""
@@ -58,6 +59,15 @@ ctx = g2g.Context(s, precision="fp64", grid_mode="device")
 ctx.build_grid()
 out["after"] = ctx.solve(s.rho_packed)[0]
 ctx.close()
+# the replica server on the slab: a budget of about two replicas forces
+# evictions, every served step is compared with the same step alone
+srv = serve.Server(reps, precision="bf16x6", slots=3, hbm_budget=small, grid_mode="device",
+                   slab=True)
+srv.budget = 2 * max(srv._estimate(k) for k in range(len(reps)))
+res = srv.run(serve.poisson_arrivals(12, 200.0, len(reps), seed=3), max_seconds=120.0)
+srv.close()
+out["served"] = [[q.replica, q.step, q.energy, q.electrons] for q in res["requests"]]
+out["evictions"] = res["evictions"]
 print("RESULT " + json.dumps(out), flush=True)
 """
 
@@ -104,3 +114,12 @@ def test_slab_results_equal_per_grid_allocation():
     ctx.build_grid()
     assert child["after"] == ctx.solve(s.rho_packed)[0]
     ctx.close()
+    assert len(child["served"]) == 12
+    print("served on the slab with %d evictions" % child["evictions"], flush=True)
+    for r, step, E, N in child["served"]:
+        s = reps[r].state(step)
+        ctx = g2g.Context(s, precision="bf16x6", grid_mode="device")
+        ctx.build_grid()
+        E1, _, info = ctx.solve(s.rho_packed)
+        ctx.close()
+        assert E == E1 and N == info["electrons"]
