@@ -60,3 +60,37 @@ def test_device_grid_batch_matches_host_batch():
     (E0, N0, F0), (E1, N1, F1) = out
     assert np.max(np.abs(E0 - E1)) < 1e-10 and np.max(np.abs(N0 - N1)) < 1e-10
     assert np.max(np.abs(F0 - F1)) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x6"])
+def test_device_grid_forces_and_open_shell(precision):
+    """Forces (computation type 2) and the open-shell build on a device-built
+    grid equal those on the host-built grid (the same groups, points and
+    functions; weights to 1e-12 relative): to FP64 round-off in FP64, and in
+    split-BF16 to 1e-7, a tenth of the mixed-precision bar, since a weight
+    differing in its last bit can round a factor-scaled value to the
+    neighbouring FP32 number."""
+    _gpu()
+    from lio_amd import g2g, systems
+    s = system_for("mol25_s103_pbe")
+    so = systems.open_shell(s)
+    out = {}
+    for mode in ("host", "device"):
+        print("forces", precision, mode, flush=True)
+        ctx = g2g.Context(s, precision=precision, grid_mode=mode)
+        ctx.build_grid()
+        F = np.zeros(s.M * (s.M + 1) // 2)
+        E, f, info = ctx.solve_forces(fock=F)
+        ctx.close()
+        print("open shell", precision, mode, flush=True)
+        co = g2g.Context(so, precision=precision, grid_mode=mode)
+        co.build_grid()
+        Eo, Fa, Fb, fo, io = co.solve_open(forces=True)
+        co.close()
+        out[mode] = (E, F, f, info["electrons"], Eo, Fa, Fb, fo, io["electrons"])
+    h, d = out["host"], out["device"]
+    tol = 1e-10 if precision == "fp64" else 1e-7
+    for a, b in zip(h, d):
+        diff = float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+        print("host vs device %.2e" % diff, flush=True)
+        assert diff < tol
