@@ -477,6 +477,7 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
   const int np = g.npts_pad, mp = g.m_pad;
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
+  const bool diag = it.ti == it.tj;
   const int pb = it.slice * slen + wave * (slen / 4);
   const int pe = min(pb + slen / 4, np);
   const T* V = vals + g.val_off;
@@ -499,8 +500,12 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
     for (int r = 0; r < 4; ++r) {
       A[r] = (r < nri) ? *reinterpret_cast<const V4*>(V + (long long)(i0 + 16 * r + lrow) * np + p)
                        : V4{0, 0, 0, 0};
-      B[r] = (r < nrj) ? *reinterpret_cast<const V4*>(V + (long long)(j0 + 16 * r + lrow) * np + p) * f
-                       : V4{0, 0, 0, 0};
+      if (diag)  // the same function rows on both sides: one load
+        B[r] = A[r] * f;
+      else
+        B[r] = (r < nrj)
+                   ? *reinterpret_cast<const V4*>(V + (long long)(j0 + 16 * r + lrow) * np + p) * f
+                   : V4{0, 0, 0, 0};
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s)
@@ -1262,6 +1267,7 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
   const int np = g.npts_pad, mp = g.m_pad;
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
+  const bool diag = it.ti == it.tj;
   const int pb = it.slice * slen + wave * (slen / 4);
   const int pe = min(pb + slen / 4, np);
   const float* Phi = phiT + g.phi_off;
@@ -1298,7 +1304,7 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
       }
       split3(x, ah[r], am[r], al[r]);
       if (ok && r < nrj) {
-        load8(Phi + (long long)(j0 + 16 * r + lrow) * np + p, x);
+        if (!diag) load8(Phi + (long long)(j0 + 16 * r + lrow) * np + p, x);  // else: same rows
 #pragma unroll
         for (int q = 0; q < 8; ++q) x[q] = (float)((double)x[q] * f[q]);
       } else {
