@@ -18,6 +18,9 @@ Prints ONE JSON line:
   fp64           the FP64 build: builds/s, phases, grid points/s
   mixed          speedup over FP64 and the errors against it
   latency_ms     p50/p99 of single synchronised mixed builds
+  next_rows      the same workload with XC forces (computation type 2) and
+                 as an open-shell triplet: builds/s in both modes and the
+                 mixed mode's errors against FP64
   batch          64 independent molecules of 3-60 atoms resident together,
                  every phase one launch over all of them, replayed as one
                  hipGraph: molecules/s in both modes
@@ -182,6 +185,71 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     return out
 
 
+def bench_next_rows(s, steps=20, warmup=3):
+    """The rows built beyond the Fock build, on the same (H2O)32 workload with
+    everything resident in HBM: energy + Fock + XC forces (computation type 2)
+    and the open-shell build (both spins' Fock matrices from a triplet made
+    out of the closed-shell density), in both modes; builds/s and the mixed
+    mode's errors against FP64."""
+    import numpy as np
+    import torch
+    from lio_amd import g2g, systems
+
+    def timed(b, launch):
+        stream = torch.cuda.ExternalStream(b.stream())
+        for _ in range(warmup):
+            launch()
+        b.energies()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            launch()
+        end.record(stream)
+        b.energies()
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / steps
+
+    so = systems.open_shell(s)
+    res = {}
+    for prec in ("fp64", MIXED):
+        b = g2g.Batch([s], precision=prec)
+        rho = torch.from_numpy(b.packed_density()).cuda()
+        F = torch.zeros_like(rho)
+        f = torch.zeros(3 * s.natom, dtype=torch.float64, device="cuda")
+        ms_f = timed(b, lambda: b.solve_forces(rho.data_ptr(), F.data_ptr(), f.data_ptr()))
+        f.zero_()
+        b.solve_forces(rho.data_ptr(), F.data_ptr(), f.data_ptr())
+        E, _ = b.energies()
+        forces = f.cpu().numpy().copy()
+        b.close()
+        bo = g2g.Batch([so], precision=prec)
+        rho2 = torch.from_numpy(bo.packed_density()).cuda()
+        F2 = torch.zeros_like(rho2)
+        ms_o = timed(bo, lambda: bo.solve_device(rho2.data_ptr(), F2.data_ptr()))
+        F2.zero_()
+        bo.solve_device(rho2.data_ptr(), F2.data_ptr())
+        Eo, No = bo.energies()
+        fock_o = F2.cpu().numpy().copy()
+        bo.close()
+        res[prec] = dict(ms_f=ms_f, E=float(E[0]), forces=forces, ms_o=ms_o, Eo=float(Eo[0]),
+                         No=float(No[0]), fock_o=fock_o)
+        del rho, F, f, rho2, F2
+        torch.cuda.empty_cache()
+    a, m = res["fp64"], res[MIXED]
+    return {
+        "forces": {"what": "energy + Fock + XC forces per build (computation type 2)",
+                   "fp64_builds_per_s": 1e3 / a["ms_f"], "mixed_builds_per_s": 1e3 / m["ms_f"],
+                   "speedup": a["ms_f"] / m["ms_f"],
+                   "dExc_vs_fp64": m["E"] - a["E"],
+                   "max_dforce_vs_fp64": float(np.max(np.abs(m["forces"] - a["forces"])))},
+        "open_shell": {"what": "alpha and beta densities in, both Fock matrices out (triplet)",
+                       "fp64_builds_per_s": 1e3 / a["ms_o"], "mixed_builds_per_s": 1e3 / m["ms_o"],
+                       "speedup": a["ms_o"] / m["ms_o"],
+                       "dExc_vs_fp64": m["Eo"] - a["Eo"], "dN_vs_fp64": m["No"] - a["No"],
+                       "max_dF_vs_fp64": float(np.max(np.abs(m["fock_o"] - a["fock_o"])))},
+    }
+
+
 def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
     """Replica server at its production shape: nrep QM/MM replicas of
     ``atoms`` atoms, every request one MD step (new geometry, new density,
@@ -229,6 +297,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-next-rows", action="store_true",
+                    help="skip the forces and open-shell lines")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests per serving run (two runs per mode)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -312,6 +382,10 @@ def main():
                      "mfma_tflops_executed": mfma_tflops,
                      "mfma_frac": mfma_tflops / PEAK_BF16_TFLOPS},
     }
+    if not args.no_next_rows:
+        log("forces and open shell ...")
+        result["next_rows"] = bench_next_rows(s)
+        log("next rows:", json.dumps(result["next_rows"]))
     if not args.no_batch:
         log("batch of 64 molecules ...")
         bb = {p: bench_batch(p) for p in ("fp64", MIXED)}
