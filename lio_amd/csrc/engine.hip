@@ -1434,13 +1434,17 @@ __global__ __launch_bounds__(256, 2) void k_force(const DensityItem* __restrict_
 }
 
 // Split-BF16 values (point-major FP32 planes, P_g as three BF16 pieces): Y on
-// the six-product BF16 MFMA chain as in the density kernels.
+// the six-product BF16 MFMA chain as in the density kernels, the P_g pieces of
+// each chunk and K step staged once per block in LDS (double-buffered).
 __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restrict__ items,
                                                      const DevGroup* __restrict__ groups,
                                                      const unsigned short* __restrict__ pg3,
                                                      const float* __restrict__ vals,
                                                      const double* __restrict__ factor,
                                                      double* __restrict__ gpart) {
+  constexpr int ROWS = kFnChunk;
+  constexpr int UNITS = 3 * ROWS * 4;  // 16-byte units of one staged tile
+  __shared__ u4 sA[2][UNITS];          // [buf][piece][row][k octet]
   __shared__ double sG[4][kFnChunk][3];
   const DensityItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
@@ -1455,29 +1459,52 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
   const unsigned short* Pp = pg3 + 3 * g.pg_off;
   const double f = active ? factor[g.pt_off + p0 + lrow] : 0.0;
   double* out = gpart + g.force_off + (long long)(it.p0 / kDensPts) * mp * 3;
+  // staging map as in the density kernel: unit c = threadIdx.x + 256 i is
+  // piece c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4
+  auto fetch = [&](int c0, int k0, u4 (&r)[UNITS / 256]) {
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) {
+      const int c = threadIdx.x + 256 * i;
+      const int sp = c / (4 * ROWS), row = (c >> 2) & (ROWS - 1), oc = c & 3;
+      const int j = c0 + row;
+      r[i] = j < mp ? *reinterpret_cast<const u4*>(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc)
+                    : u4{0, 0, 0, 0};
+    }
+  };
   for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
     const int nrt = min(4, (mp - c0) / 16);
     f4 acc[4];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) acc[rt] = f4{0, 0, 0, 0};
-    if (active) {
-      for (int k0 = 0; k0 < mp; k0 += 32) {
-        const int kk = k0 + 8 * lk;
+    u4 nxt[UNITS / 256];
+    fetch(c0, 0, nxt);
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) sA[0][threadIdx.x + 256 * i] = nxt[i];
+    __syncthreads();
+    for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
+      const bool more = k0 + 32 < mp;
+      if (more) fetch(c0, k0 + 32, nxt);
+      if (active) {
         bf8 bh, bm, bl;
         float x[8];
-        load8(Vp + kk, x);
+        load8(Vp + k0 + 8 * lk, x);
         split3(x, bh, bm, bl);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
           if (rt < nrt) {
-            const unsigned short* pa = Pp + (long long)(c0 + 16 * rt + lrow) * mp + kk;
-            const bf8 ah = *reinterpret_cast<const bf8*>(pa);
-            const bf8 am = *reinterpret_cast<const bf8*>(pa + psz);
-            const bf8 al = *reinterpret_cast<const bf8*>(pa + 2 * psz);
+            const int u = (16 * rt + lrow) * 4 + lk;
+            const bf8 ah = __builtin_bit_cast(bf8, sA[buf][u]);
+            const bf8 am = __builtin_bit_cast(bf8, sA[buf][ROWS * 4 + u]);
+            const bf8 al = __builtin_bit_cast(bf8, sA[buf][2 * ROWS * 4 + u]);
             acc[rt] += mma6(ah, am, al, bh, bm, bl);
           }
         }
       }
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < UNITS / 256; ++i) sA[buf ^ 1][threadIdx.x + 256 * i] = nxt[i];
+      }
+      __syncthreads();
     }
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) {
