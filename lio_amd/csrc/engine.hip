@@ -1371,6 +1371,38 @@ __device__ inline void force_lanes(double (&c)[3], int lrow, double* slot) {
   }
 }
 
+// Sums w[3 i + a] (four functions i, three components a) over the 16 lanes
+// of a row (the 16 points of a wave's tile) in a fixed order: two halving
+// exchanges (xor 8, then 4) leave each lane three partial sums of one
+// function, a butterfly (xor 2, 1) completes them, and the lanes with
+// lrow % 4 == 0 store function i = lrow / 4 into slot jj[i].
+__device__ inline void force_rows(const double (&w)[12], int lrow, double (*slot)[3],
+                                  const int (&jj)[4]) {
+  const bool h8 = (lrow & 8) != 0, h4 = (lrow & 4) != 0;
+  double u[6], v[3];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const double keep = h8 ? w[6 + t] : w[t], send = h8 ? w[t] : w[6 + t];
+    u[t] = keep + __shfl_xor(send, 8);
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const double keep = h4 ? u[3 + t] : u[t], send = h4 ? u[t] : u[3 + t];
+    v[t] = keep + __shfl_xor(send, 4);
+  }
+#pragma unroll
+  for (int o = 2; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) v[t] += __shfl_xor(v[t], o);
+  }
+  if ((lrow & 3) == 0) {
+    double* dst = slot[jj[(h8 ? 2 : 0) + (h4 ? 1 : 0)]];
+    dst[0] = v[0];
+    dst[1] = v[1];
+    dst[2] = v[2];
+  }
+}
+
 // FP64 / FP32 values (planes [value][function][point]).
 template <class T>
 __global__ __launch_bounds__(256, 2) void k_force(const DensityItem* __restrict__ items,
@@ -1514,13 +1546,17 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
 #pragma unroll
         for (int a = 0; a < 3; ++a) v[a] = *reinterpret_cast<const fl4*>(vj + (a + 1) * plane);
       }
+      double w[12];
+      int jj[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int jj = 16 * rt + 4 * lk + i;
+        jj[i] = 16 * rt + 4 * lk + i;
         const double y = f * (double)acc[rt][i];
-        double c[3] = {y * (double)v[0][i], y * (double)v[1][i], y * (double)v[2][i]};
-        force_lanes(c, lrow, sG[wave][jj]);
+        w[3 * i] = y * (double)v[0][i];
+        w[3 * i + 1] = y * (double)v[1][i];
+        w[3 * i + 2] = y * (double)v[2][i];
       }
+      force_rows(w, lrow, sG[wave], jj);
     }
     __syncthreads();
     if (threadIdx.x < 3 * kFnChunk) {
