@@ -913,12 +913,15 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
 
 // Density + functional on split-BF16 MFMA, wide form: 64 points per block
 // (16 per wave) and 128-function row chunks (8 row tiles per wave), with the
-// P_g pieces of a chunk and K step (128 rows x 32 columns x 3 pieces, 24 KB)
+// P_g pieces of a chunk and K step (ROWS rows x 32 columns x 3 pieces)
 // staged once per block in LDS (double-buffered: the next step's tile is
-// fetched into registers while this step's MFMAs run) instead of being read
-// by each of the four waves; the wider chunk halves the re-reads of the B
-// operand.  B operand (split in registers), epilogue and reductions as in
-// k_density_bf.
+// copied from global memory straight into LDS while this step's MFMAs run)
+// instead of being read by each of the four waves; the wider chunk halves the
+// re-reads of the B operand.  The B operand of the next K step (FP32 values
+// of the wave's points) is loaded while this step's products run and split
+// in registers; the ten row-dot sums of each point live in LDS between row
+// chunks, which keeps the kernel at two waves per SIMD without spills.
+// Epilogue and reductions as in k_density_bf.
 // RAD: radial-Hessian layout -- phi and its gradient are planes, the six
 // second derivatives are rebuilt from the shells' radial sums (FP32) and the
 // functions' angular factors in the epilogue, which then reads 4 planes
@@ -961,18 +964,22 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
     ppy = py[g.pt_off + pt];
     ppz = pz[g.pt_off + pt];
   }
-  double s[NS];
-  for (int k = 0; k < NS; ++k) s[k] = 0.0;
+  __shared__ double sS[NS][256];
+  for (int k = 0; k < NS; ++k) sS[k][threadIdx.x] = 0.0;
   // staging map: 16-byte unit c = threadIdx.x + 256 i of a tile is piece
-  // c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4
-  auto fetch = [&](int c0, int k0, u4 (&r)[UNITS / 256]) {
+  // c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4, copied from global
+  // memory straight into LDS (no staging registers): lane l of wave w writes
+  // unit 256 i + 64 w + l.  Rows past the group's functions are never read
+  // (their address is clamped to the last row).
+  auto dma = [&](int c0, int k0, int b) {
 #pragma unroll
     for (int i = 0; i < UNITS / 256; ++i) {
       const int c = threadIdx.x + 256 * i;
       const int sp = c / (4 * ROWS), row = (c >> 2) & (ROWS - 1), oc = c & 3;
-      const int j = c0 + row;
-      r[i] = j < mp ? *reinterpret_cast<const u4*>(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc)
-                    : u4{0, 0, 0, 0};
+      const int j = min(c0 + row, mp - 1);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc),
+          (__attribute__((address_space(3))) void*)&sA[b][256 * i + 64 * wave], 16, 0, 0);
     }
   };
   for (int c0 = 0; c0 < mp; c0 += ROWS) {
@@ -982,21 +989,21 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[rt][t] = f4{0, 0, 0, 0};
-    u4 nxt[UNITS / 256];
-    fetch(c0, 0, nxt);
+    float xb[NT][8];
 #pragma unroll
-    for (int i = 0; i < UNITS / 256; ++i) sA[0][threadIdx.x + 256 * i] = nxt[i];
+    for (int t = 0; t < NT; ++t) load8(Vp + t * plane + 8 * lk, xb[t]);
+    dma(c0, 0, 0);
     __syncthreads();
     for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
       const bool more = k0 + 32 < mp;
-      if (more) fetch(c0, k0 + 32, nxt);
+      if (more) dma(c0, k0 + 32, buf ^ 1);
       const int kk = k0 + 8 * lk;
       bf8 bh[NT], bm[NT], bl[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        float x[8];
-        load8(Vp + t * plane + kk, x);
-        split3(x, bh[t], bm[t], bl[t]);
+      for (int t = 0; t < NT; ++t) split3(xb[t], bh[t], bm[t], bl[t]);
+      if (more) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) load8(Vp + t * plane + kk + 32, xb[t]);
       }
       if (active) {
 #pragma unroll
@@ -1011,13 +1018,12 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
           }
         }
       }
-      if (more) {
-#pragma unroll
-        for (int i = 0; i < UNITS / 256; ++i) sA[buf ^ 1][threadIdx.x + 256 * i] = nxt[i];
-      }
       __syncthreads();
     }
     if (!active) continue;
+    double s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = sS[k][threadIdx.x];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
       if (rt >= nrt) continue;
@@ -1065,9 +1071,14 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
         for (int i = 0; i < 4; ++i) s[0] += (double)v0[i] * (double)acc[rt][0][i];
       }
     }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sS[k][threadIdx.x] = s[k];
   }
   double e_w = 0.0, n_w = 0.0;
   if (active) {
+    double s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = sS[k][threadIdx.x];
     for (int k = 0; k < NS; ++k) {  // the 4 lane groups holding the same point
       s[k] += __shfl_xor(s[k], 16);
       s[k] += __shfl_xor(s[k], 32);
