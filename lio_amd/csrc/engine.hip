@@ -1478,7 +1478,8 @@ __global__ __launch_bounds__(256, 2) void k_force(const DensityItem* __restrict_
 
 // Split-BF16 values (point-major FP32 planes, P_g as three BF16 pieces): Y on
 // the six-product BF16 MFMA chain as in the density kernels, the P_g pieces of
-// each chunk and K step staged once per block in LDS (double-buffered).
+// each chunk and K step copied once per block straight into LDS
+// (double-buffered), the next K step's B operand loaded during this step.
 __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restrict__ items,
                                                      const DevGroup* __restrict__ groups,
                                                      const unsigned short* __restrict__ pg3,
@@ -1503,15 +1504,18 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
   const double f = active ? factor[g.pt_off + p0 + lrow] : 0.0;
   double* out = gpart + g.force_off + (long long)(it.p0 / kDensPts) * mp * 3;
   // staging map as in the density kernel: unit c = threadIdx.x + 256 i is
-  // piece c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4
-  auto fetch = [&](int c0, int k0, u4 (&r)[UNITS / 256]) {
+  // piece c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4, copied from global
+  // memory straight into LDS (lane l of wave w writes unit 256 i + 64 w + l;
+  // rows past the group's functions are clamped and never read)
+  auto dma = [&](int c0, int k0, int b) {
 #pragma unroll
     for (int i = 0; i < UNITS / 256; ++i) {
       const int c = threadIdx.x + 256 * i;
       const int sp = c / (4 * ROWS), row = (c >> 2) & (ROWS - 1), oc = c & 3;
-      const int j = c0 + row;
-      r[i] = j < mp ? *reinterpret_cast<const u4*>(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc)
-                    : u4{0, 0, 0, 0};
+      const int j = min(c0 + row, mp - 1);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc),
+          (__attribute__((address_space(3))) void*)&sA[b][256 * i + 64 * wave], 16, 0, 0);
     }
   };
   for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
@@ -1519,19 +1523,17 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
     f4 acc[4];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) acc[rt] = f4{0, 0, 0, 0};
-    u4 nxt[UNITS / 256];
-    fetch(c0, 0, nxt);
-#pragma unroll
-    for (int i = 0; i < UNITS / 256; ++i) sA[0][threadIdx.x + 256 * i] = nxt[i];
+    float xb[8];  // the B operand of the current K step, loaded one step ahead
+    load8(Vp + 8 * lk, xb);
+    dma(c0, 0, 0);
     __syncthreads();
     for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
       const bool more = k0 + 32 < mp;
-      if (more) fetch(c0, k0 + 32, nxt);
+      if (more) dma(c0, k0 + 32, buf ^ 1);
       if (active) {
         bf8 bh, bm, bl;
-        float x[8];
-        load8(Vp + k0 + 8 * lk, x);
-        split3(x, bh, bm, bl);
+        split3(xb, bh, bm, bl);
+        if (more) load8(Vp + k0 + 32 + 8 * lk, xb);
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
           if (rt < nrt) {
@@ -1542,10 +1544,6 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
             acc[rt] += mma6(ah, am, al, bh, bm, bl);
           }
         }
-      }
-      if (more) {
-#pragma unroll
-        for (int i = 0; i < UNITS / 256; ++i) sA[buf ^ 1][threadIdx.x + 256 * i] = nxt[i];
       }
       __syncthreads();
     }
