@@ -1759,8 +1759,8 @@ __device__ inline double becke_cell_w(double dj, double dk, double rjk, double a
 // wave-uniform loads.  Distances are recomputed instead of stored per point
 // (the host keeps them in an array), with identical values.
 //
-// Each P_j is first tested against the point's three nearest nuclei n: when
-// an s(mu_jn) is exactly zero the product is zero whatever the order of its
+// Each P_j is first tested against the point's nearest nucleus n: when
+// s(mu_jn) is exactly zero the product is zero whatever the order of its
 // other factors (all lie in [0, 1]), and a wave runs the k-term product only
 // for nuclei whose cells reach one of its points.  Zero terms added to the
 // sum change nothing, and a zero own cell gives weight 0 however far the sum
@@ -1795,23 +1795,16 @@ __global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ group
   const double* W = aw + m.rij_off;
   const double* D = rij + m.rij_off;
   const double x = sx[p], y = sy[p], z = sz[p];
-  // the three nearest nuclei (first minimum on ties), the likeliest to cut
-  int an = -1, an2 = -1, an3 = -1;
-  double dn = 0.0, dn2 = 0.0, dn3 = 0.0;
+  int an = -1;
+  double dn = 0.0;
   bool own_in = false;
   for (int q = 0; q < k; ++q) {
     const int aq = at[q];
     const double dq = dist3(x, y, z, A[aq], B[aq], C[aq]);
     own_in |= (aq == own);
     if (an < 0 || dq < dn) {
-      an3 = an2, dn3 = dn2;
-      an2 = an, dn2 = dn;
-      an = aq, dn = dq;
-    } else if (an2 < 0 || dq < dn2) {
-      an3 = an2, dn3 = dn2;
-      an2 = aq, dn2 = dq;
-    } else if (an3 < 0 || dq < dn3) {
-      an3 = aq, dn3 = dq;
+      dn = dq;
+      an = aq;
     }
   }
   double total = 0.0, mine = 0.0;
@@ -1821,8 +1814,6 @@ __global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ group
     const double* Dj = D + (long long)aj * na;
     const double* Wj = W + (long long)aj * na;
     if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
-    if (an2 >= 0 && aj != an2 && becke_cell_w(dj, dn2, Dj[an2], Wj[an2]) == 0.0) continue;
-    if (an3 >= 0 && aj != an3 && becke_cell_w(dj, dn3, Dj[an3], Wj[an3]) == 0.0) continue;
     double prod = 1.0;
     for (int q = 0; q < k; ++q) {
       const int ak = at[q];
