@@ -1264,7 +1264,10 @@ __global__ __launch_bounds__(256, 2) void k_density_bf3(const DensityItem* __res
 // 512-point slice, 64 x 64 tile).  K steps of 32 points: a lane's operand is
 // 8 consecutive points of one function row of the function-major phi plane,
 // times the factor for the B side (rounded to FP32 from the FP64 product),
-// split in registers.  Each wave sums its 128 points in FP32 in the matrix
+// split in registers.  The next K step's rows and factors are loaded while
+// this step's products run: the B side is split first, then each A row just
+// before its products, so the raw values of both steps fit in registers
+// (253 VGPRs, two waves per SIMD).  Each wave sums its 128 points in FP32 in the matrix
 // unit; the four partial tiles are added in FP64 in wave order.
 __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__ items,
                                                   const DevGroup* __restrict__ groups,
@@ -1288,9 +1291,11 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
-  for (int p = pb + 8 * lk; p - 8 * lk < pe; p += 32) {
+  // raw values of the current K step, loaded one step ahead
+  float xa[4][8], xb[4][8];
+  double f[8];
+  auto load_fb = [&](int p) {
     const bool ok = p < pe;
-    double f[8];
     if (ok) {
       const d4 f0 = *reinterpret_cast<const d4*>(fac + p);
       const d4 f1 = *reinterpret_cast<const d4*>(fac + p + 4);
@@ -1303,33 +1308,55 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
 #pragma unroll
       for (int q = 0; q < 8; ++q) f[q] = 0.0;
     }
-    bf8 ah[4], am[4], al[4], bh[4], bm[4], bl[4];
+    if (!diag) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (ok && r < nrj) {
+          load8(Phi + (long long)(j0 + 16 * r + lrow) * np + p, xb[r]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) xb[r][q] = 0.f;
+        }
+      }
+    }
+  };
+  auto load_a = [&](int p, int r) {
+    if (p < pe && r < nri) {
+      load8(Phi + (long long)(i0 + 16 * r + lrow) * np + p, xa[r]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xa[r][q] = 0.f;
+    }
+  };
+  load_fb(pb + 8 * lk);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) load_a(pb + 8 * lk, r);
+  for (int p = pb + 8 * lk; p - 8 * lk < pe; p += 32) {
+    const bool ok = p < pe;
+    const bool next = p + 32 - 8 * lk < pe;
+    bf8 bh[4], bm[4], bl[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       float x[8];
-      if (ok && r < nri) {
-        load8(Phi + (long long)(i0 + 16 * r + lrow) * np + p, x);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) x[q] = 0.f;
-      }
-      split3(x, ah[r], am[r], al[r]);
       if (ok && r < nrj) {
-        if (!diag) load8(Phi + (long long)(j0 + 16 * r + lrow) * np + p, x);  // else: same rows
 #pragma unroll
-        for (int q = 0; q < 8; ++q) x[q] = (float)((double)x[q] * f[q]);
+        for (int q = 0; q < 8; ++q) x[q] = (float)((double)(diag ? xa[r][q] : xb[r][q]) * f[q]);
       } else {
 #pragma unroll
         for (int q = 0; q < 8; ++q) x[q] = 0.f;
       }
       split3(x, bh[r], bm[r], bl[r]);
     }
+    if (next) load_fb(p + 32);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
+      bf8 ah, am, al;
+      split3(xa[r], ah, am, al);
+      if (next) load_a(p + 32, r);
       if (r < nri) {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-          if (c < nrj) acc[r][c] += mma6(ah[r], am[r], al[r], bh[c], bm[c], bl[c]);
+          if (c < nrj) acc[r][c] += mma6(ah, am, al, bh[c], bm[c], bl[c]);
       }
     }
   }
