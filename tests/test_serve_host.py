@@ -27,3 +27,30 @@ def test_poisson_arrivals_reproducible():
     t = np.array([x for x, _ in a])
     assert np.all(np.diff(t) > 0) and 1.0 < t[-1] < 6.0
     assert {r for _, r in a} <= set(range(8))
+
+
+def test_slab_needs_a_budget():
+    """A server on a device slab must be given the HBM budget to reserve; the
+    check comes before any device call."""
+    import pytest
+    reps = [serve.Replica(systems.random_molecule(3, 5), seed=1)]
+    with pytest.raises(ValueError):
+        serve.Server(reps, slab=True)
+
+
+def test_prefetched_state_is_the_state():
+    """A step's inputs computed ahead on a worker thread are the same as the
+    step computed on demand."""
+    import concurrent.futures
+    s = systems.random_molecule(6, 11)
+    r1, r2 = serve.Replica(s, seed=9), serve.Replica(s, seed=9)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=2) as pool:
+        r1.prefetch(3, pool)
+        a = r1.take(3)
+    b = r2.state(3)
+    assert np.array_equal(a.pos, b.pos) and np.array_equal(a.rho_packed, b.rho_packed)
+    # a prefetch for another step is ignored
+    with concurrent.futures.ThreadPoolExecutor(max_workers=1) as pool:
+        r1.prefetch(7, pool)
+        c = r1.take(4)
+    assert np.array_equal(c.pos, r2.state(4).pos)
