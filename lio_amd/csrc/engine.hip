@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <cstdio>
 #include <cstring>
@@ -2060,11 +2061,15 @@ Engine::Engine(int device) {
 // at once.
 struct Slab {
   std::mutex mu;
-  char* base = nullptr;
+  // set once (size first, then base with release order) and never changed,
+  // so engines on other threads may test it without the lock
+  std::atomic<char*> base{nullptr};
   size_t size = 0;
   std::map<size_t, size_t> free_;  // offset -> length, coalesced
+  bool active() const { return base.load(std::memory_order_acquire) != nullptr; }
   bool owns(const void* p) const {
-    return base && p >= base && static_cast<const char*>(p) < base + size;
+    const char* b = base.load(std::memory_order_acquire);
+    return b && p >= b && static_cast<const char*>(p) < b + size;
   }
   char* take(size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
@@ -2074,14 +2079,14 @@ struct Slab {
       const size_t off = it->first, len = it->second;
       free_.erase(it);
       if (len > bytes) free_[off + bytes] = len - bytes;
-      return base + off;
+      return base.load(std::memory_order_relaxed) + off;
     }
     return nullptr;
   }
   void give(char* p, size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
     std::lock_guard<std::mutex> l(mu);
-    size_t off = p - base, len = bytes;
+    size_t off = p - base.load(std::memory_order_relaxed), len = bytes;
     auto next = free_.lower_bound(off);
     if (next != free_.end() && off + len == next->first) {
       len += next->second;
@@ -2510,7 +2515,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     // grids of the next steps of a moving molecule fit the same arena
     const size_t want = dg->arena_bytes + (reclaim ? dg->arena_bytes / 4 : dg->arena_bytes / 8);
     size_t free_b = 0, total_b = 0;
-    const bool slab = g_slab[device_].base != nullptr;
+    const bool slab = g_slab[device_].active();
     XCQ_HIP(hipMemGetInfo(&free_b, &total_b));
     {  // memory freed into the pool is available again
       hipMemPool_t pool;
@@ -2532,7 +2537,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       old.g->arena = nullptr;
     }
     Slab& sl = g_slab[device_];
-    if (sl.base) {  // carve from the slab: exact size, no device call
+    if (sl.active()) {  // carve from the slab: exact size, no device call
       dg->arena = sl.take(dg->arena_bytes);
       if (!dg->arena) {
         const size_t need = dg->arena_bytes;
@@ -3209,7 +3214,7 @@ void Engine::reserve_slab(int device, size_t bytes) {
   if (device >= 64) throw std::runtime_error("g2g: device out of range");
   Slab& sl = g_slab[device];
   std::lock_guard<std::mutex> l(sl.mu);
-  if (sl.base) {
+  if (sl.active()) {
     if (sl.size >= bytes) return;  // a second server in the same process
     throw std::runtime_error("g2g: the device slab is already reserved with " +
                              std::to_string(sl.size >> 20) + " MiB");
@@ -3228,10 +3233,10 @@ void Engine::reserve_slab(int device, size_t bytes) {
   const hipError_t e = hipMalloc(&p, bytes);
   (void)hipSetDevice(cur);
   XCQ_HIP(e);
-  sl.base = static_cast<char*>(p);
-  sl.size = bytes;
   sl.free_.clear();
   sl.free_[0] = bytes;
+  sl.size = bytes;
+  sl.base.store(static_cast<char*>(p), std::memory_order_release);
 }
 
 }  // namespace xcq
