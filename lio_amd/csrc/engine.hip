@@ -60,6 +60,7 @@ constexpr int kDensPts = 64;     // points per k_density block (16 per wave)
 constexpr int kRowSplitMin = 64; // split-BF16 row-split density above this m_pad
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
 constexpr int kSliceDefault = 1024;  // points per Fock slice (split-K over blocks)
+constexpr int kPackMapMax = 16384;  // groups per molecule the packing kernel maps in LDS
 constexpr int kEvalPts = 256;    // threads per evaluation block
 constexpr int kEvalSpan = 4096;  // points (or elements) one evaluation block covers
 
@@ -111,6 +112,7 @@ struct DevMol {
   int M, fn_base;
   long long rho_off;
   int eb, ee;
+  int g0, ng;  // its groups [g0, g0 + ng) in the batch
 };
 
 struct EvalItem {
@@ -1679,8 +1681,42 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
 // Packed F: entry (I <= J) sums F_g(li, lj) over the groups holding both
 // functions, in ascending group order.  memb holds, per function, its
 // (group, local index) pairs sorted by group.  I (blockIdx.y) and J are
-// batch function indices; J runs over the rest of I's molecule.
+// batch function indices; J runs over the rest of I's molecule.  The block
+// first maps each group of I's molecule to I's local index in it (-1 where
+// I is absent) in LDS; each thread then walks J's list alone, in ascending
+// group order, so the sum is the same as a merge of the two lists.
 __global__ void k_fock_pack(const int* __restrict__ memb_off, const int2* __restrict__ memb,
+                            const DevGroup* __restrict__ groups, const DevMol* __restrict__ mols,
+                            const int* __restrict__ fn_mol, const double* __restrict__ fg,
+                            double* __restrict__ F) {
+  extern __shared__ int smap[];
+  const int I = blockIdx.y;
+  const DevMol mol = mols[fn_mol[I]];
+  for (int q = threadIdx.x; q < mol.ng; q += blockDim.x) smap[q] = -1;
+  __syncthreads();
+  for (int a = memb_off[I] + threadIdx.x; a < memb_off[I + 1]; a += blockDim.x) {
+    const int2 u = memb[a];
+    smap[u.x - mol.g0] = u.y;
+  }
+  __syncthreads();
+  const int J = mol.fn_base + blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= mol.fn_base + mol.M || J < I) return;
+  double s = 0.0;
+  for (int b = memb_off[J]; b < memb_off[J + 1]; ++b) {
+    const int2 v = memb[b];
+    const int ui = smap[v.x - mol.g0];
+    if (ui >= 0) {
+      const DevGroup g = groups[v.x];
+      const int li = min(ui, v.y), lj = max(ui, v.y);
+      s += fg[g.pg_off + (long long)li * g.m_pad + lj];
+    }
+  }
+  F[mol.rho_off + packed_index(I - mol.fn_base, J - mol.fn_base, mol.M)] = s;
+}
+
+// The same by merging the two sorted lists, for molecules with more groups
+// than the LDS map holds.
+__global__ void k_fock_pack_merge(const int* __restrict__ memb_off, const int2* __restrict__ memb,
                             const DevGroup* __restrict__ groups, const DevMol* __restrict__ mols,
                             const int* __restrict__ fn_mol, const double* __restrict__ fg,
                             double* __restrict__ F) {
@@ -1945,7 +1981,7 @@ double wall() {
 // ---------------------------------------------------------------------------
 struct DeviceGrid {
   int M = 0, natom = 0, iexch = 9, nval = 10;  // M, natom: whole batch
-  int nmol = 1, max_M = 0;
+  int nmol = 1, max_M = 0, max_ng = 0;
   int fock_slice = kSliceDefault;
   int density_variant = 2;
   int nrowsplit = 0;  // leading density items on the row-split kernel
@@ -2250,6 +2286,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     hmol[k].M = mols[k]->M;
     hmol[k].fn_base = M;
     hmol[k].rho_off = packed;
+    hmol[k].g0 = G;
+    hmol[k].ng = parts[k]->ngroups;
+    dg->max_ng = std::max(dg->max_ng, parts[k]->ngroups);
     M += mols[k]->M;
     G += parts[k]->ngroups;
     packed += (long long)mols[k]->M * (mols[k]->M + 1) / 2;
@@ -3017,8 +3056,14 @@ static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double
   for (int sp = 0; d_fock && sp < nspin; ++sp) {
     launch_fock(g, s, sp == 1 ? g.factor_b : g.factor);
     if (sp == 0) mark(ev, 3, s);
-    hipLaunchKernelGGL(k_fock_pack, dim3((g.max_M + 127) / 128, g.M), dim3(128), 0, s, g.memb_off,
-                       g.memb, g.groups, g.mols, g.fn_mol, g.fg, d_fock + sp * g.packed);
+    if (g.max_ng <= kPackMapMax)
+      hipLaunchKernelGGL(k_fock_pack, dim3((g.max_M + 127) / 128, g.M), dim3(128),
+                         (size_t)g.max_ng * sizeof(int), s, g.memb_off, g.memb, g.groups, g.mols,
+                         g.fn_mol, g.fg, d_fock + sp * g.packed);
+    else
+      hipLaunchKernelGGL(k_fock_pack_merge, dim3((g.max_M + 127) / 128, g.M), dim3(128), 0, s,
+                         g.memb_off, g.memb, g.groups, g.mols, g.fn_mol, g.fg,
+                         d_fock + sp * g.packed);
   }
   if (!d_fock) mark(ev, 3, s);
   hipLaunchKernelGGL(k_energy, dim3(g.nmol), dim3(1024), 0, s, g.epart, g.mols, g.eidx, g.eout);
