@@ -645,7 +645,8 @@ __device__ inline void load8(const float* p, float (&x)[8]) {
 // functions of one group: each wave evaluates one function at a time for its
 // 64 points (the function record is wave-uniform), the values are transposed
 // through LDS, and each thread stores 4 consecutive functions of one point
-// per plane in one 16-byte store.
+// per plane in one 16-byte store.  The tile's phi values also go to the
+// function-major phi plane the Fock kernel reads (rows of 64 points).
 constexpr int kPmPts = 64, kPmFns = 16;
 template <int NV>
 __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ items,
@@ -655,7 +656,8 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
                                                  const double* __restrict__ px,
                                                  const double* __restrict__ py,
                                                  const double* __restrict__ pz,
-                                                 float* __restrict__ vals) {
+                                                 float* __restrict__ vals,
+                                                 float* __restrict__ phiT) {
   __shared__ float sT[NV][kPmFns][kPmPts + 1];
   const EvalItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
@@ -674,6 +676,11 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
     for (int c = 0; c < NV; ++c) sT[c][kk][lane] = (float)v[c];
   }
   __syncthreads();
+  for (int e = threadIdx.x; e < kPmFns * kPmPts; e += 256) {
+    const int kk = e / kPmPts, pl = e % kPmPts;
+    if (it.p0 + pl < g.npts_pad)
+      phiT[g.phi_off + (long long)(it.k + kk) * g.npts_pad + it.p0 + pl] = sT[0][kk][pl];
+  }
   const int wp = threadIdx.x >> 2, wf = 4 * (threadIdx.x & 3);
   const int pp = it.p0 + wp;
   if (pp >= g.npts_pad) return;
@@ -2680,13 +2687,11 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       if (dg->npm > 0) {
         if (dg->gga && !dg->radial)
           hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
-                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
         else
           hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
-                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v);
+                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
       }
-      hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
-                         dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->phiT, 1);
       if (dg->nrad > 0)
         hipLaunchKernelGGL(k_eval_radial, dim3(dg->nrad), block, 0, s, dg->ritems, dg->groups,
                            dg->shells, dg->bas, dg->px, dg->py, dg->pz, dg->rad);
