@@ -53,7 +53,7 @@ BF16_PRODUCTS = 6          # BF16 MFMA products per split-BF16 tile product
 DENSITY_KERNEL = "k_density_bf2<4,64>"
 # HBM bytes per launch of DENSITY_KERNEL on (H2O)32 (rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 in KiB, copy-kernel calibrated)
-DENSITY_TRAFFIC = (3282082 * 2.0 + 5712 * 1.0) * 1024
+DENSITY_TRAFFIC = (3284730 * 2.0 + 5712 * 1.0) * 1024
 
 
 def log(*a):
