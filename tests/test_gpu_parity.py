@@ -1,7 +1,8 @@
 """HIP path vs the CPU oracle and LIO's golden vectors (FP64), on the GPU.
 
-Every solve goes through the C ABI of libg2g.so.  Tolerances: the north-star
-bar is |dExc| <= 1e-6 Ha, max|dF_ij| <= 1e-6 and |dN| <= 1e-6; the FP64 path
+Every solve goes through the C ABI of libg2g.so.  Tolerances: the accuracy
+bar of the reduced-precision modes is |dExc| <= 1e-6 Ha, max|dF_ij| <= 1e-6
+and |dN| <= 1e-6; the FP64 path
 is held to 1e-8 absolute (relative 1e-10 on energies of ~100 Ha), which only
 rounding differences in summation order can explain.
 """
