@@ -950,7 +950,8 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
                                                      const double* __restrict__ py,
                                                      const double* __restrict__ pz,
                                                      const FnMeta* __restrict__ fmeta,
-                                                     const float* __restrict__ rad) {
+                                                     const float* __restrict__ rad,
+                                                     float* __restrict__ ybuf) {
   constexpr int NS = (NT == 4) ? 10 : 1;
   constexpr int RT = ROWS / 16;
   constexpr int UNITS = 3 * ROWS * 4;      // 16-byte units of one staged tile
@@ -1031,6 +1032,13 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       __syncthreads();
     }
     if (!active) continue;
+    if (ybuf) {  // Y = P_g phi of this point for the force pass (exact FP32 sums)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+        if (rt < nrt)
+          *reinterpret_cast<fl4*>(ybuf + g.phi_off + (long long)pt * mp + c0 + 16 * rt + 4 * lk) =
+              fl4{acc[rt][0][0], acc[rt][0][1], acc[rt][0][2], acc[rt][0][3]};
+    }
     double s[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) s[k] = sS[k][threadIdx.x];
@@ -1615,6 +1623,64 @@ __global__ __launch_bounds__(256, 2) void k_force_bf(const DensityItem* __restri
   }
 }
 
+// Split-BF16 forces from the Y = P_g phi the density kernel left in ybuf
+// (point-major, exact FP32 sums): the same per-function sums as k_force_bf,
+// in the same lane and wave order, without recomputing the product.
+__global__ __launch_bounds__(256) void k_force_y(const DensityItem* __restrict__ items,
+                                                 const DevGroup* __restrict__ groups,
+                                                 const float* __restrict__ ybuf,
+                                                 const float* __restrict__ vals,
+                                                 const double* __restrict__ factor,
+                                                 double* __restrict__ gpart) {
+  __shared__ double sG[4][kFnChunk][3];
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;  // uniform over the wave
+  const int np = g.npts_pad, mp = g.m_pad;
+  const long long plane = (long long)mp * np;
+  const long long prow = (long long)(active ? p0 + lrow : 0) * mp;
+  const float* Vp = vals + g.val_off + prow;
+  const float* Yp = ybuf + g.phi_off + prow;
+  const double f = active ? factor[g.pt_off + p0 + lrow] : 0.0;
+  double* out = gpart + g.force_off + (long long)(it.p0 / kDensPts) * mp * 3;
+  for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
+    const int nrt = min(4, (mp - c0) / 16);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      fl4 y = fl4{0, 0, 0, 0};
+      fl4 v[3] = {fl4{0, 0, 0, 0}, fl4{0, 0, 0, 0}, fl4{0, 0, 0, 0}};
+      if (active && rt < nrt) {
+        const int j = c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
+        y = *reinterpret_cast<const fl4*>(Yp + j);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = *reinterpret_cast<const fl4*>(Vp + j + (a + 1) * plane);
+      }
+      double w[12];
+      int jj[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        jj[i] = 16 * rt + 4 * lk + i;
+        const double yy = f * (double)y[i];
+        w[3 * i] = yy * (double)v[0][i];
+        w[3 * i + 1] = yy * (double)v[1][i];
+        w[3 * i + 2] = yy * (double)v[2][i];
+      }
+      force_rows(w, lrow, sG[wave], jj);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * kFnChunk) {
+      const int jj = threadIdx.x / 3, a = threadIdx.x % 3;
+      if (c0 + jj < mp)
+        out[(long long)(c0 + jj) * 3 + a] =
+            ((sG[0][jj][a] + sG[1][jj][a]) + sG[2][jj][a]) + sG[3][jj][a];
+    }
+    __syncthreads();
+  }
+}
+
 // gsum[fn_off + j][a] = sum over the group's blocks (in order) of gpart; one
 // block per group.
 // accumulate: add onto gsum (the second spin of an open shell).
@@ -1633,27 +1699,42 @@ __global__ __launch_bounds__(256) void k_force_group(const DevGroup* __restrict_
   }
 }
 
+// Per batch function I, the sum of its group sums over the groups holding it
+// (ascending group order); one thread per function.
+__global__ __launch_bounds__(256) void k_force_fn(const int* __restrict__ memb_off,
+                                                  const int2* __restrict__ memb,
+                                                  const DevGroup* __restrict__ groups,
+                                                  const double* __restrict__ gsum, int M,
+                                                  double* __restrict__ fsum) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= M) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int k = memb_off[I]; k < memb_off[I + 1]; ++k) {
+    const int2 u = memb[k];
+    const double* gs = gsum + 3LL * (groups[u.x].fn_off + u.y);
+    s[0] += gs[0];
+    s[1] += gs[1];
+    s[2] += gs[2];
+  }
+  fsum[3LL * I] = s[0];
+  fsum[3LL * I + 1] = s[1];
+  fsum[3LL * I + 2] = s[2];
+}
+
 // Force on batch atom A (atom-major out[3 A + a]) = -2 sum over A's functions
-// (ascending) of the group sums of the groups holding them (ascending).
+// (ascending) of their function sums.
 __global__ __launch_bounds__(64) void k_force_atom(const int* __restrict__ afn_off,
                                                    const int* __restrict__ afn,
-                                                   const int* __restrict__ memb_off,
-                                                   const int2* __restrict__ memb,
-                                                   const DevGroup* __restrict__ groups,
-                                                   const double* __restrict__ gsum, int natom,
+                                                   const double* __restrict__ fsum, int natom,
                                                    double* __restrict__ out) {
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
   if (A >= natom) return;
   double s[3] = {0.0, 0.0, 0.0};
   for (int q = afn_off[A]; q < afn_off[A + 1]; ++q) {
-    const int I = afn[q];
-    for (int k = memb_off[I]; k < memb_off[I + 1]; ++k) {
-      const int2 u = memb[k];
-      const double* gs = gsum + 3LL * (groups[u.x].fn_off + u.y);
-      s[0] += gs[0];
-      s[1] += gs[1];
-      s[2] += gs[2];
-    }
+    const double* fs = fsum + 3LL * afn[q];
+    s[0] += fs[0];
+    s[1] += fs[1];
+    s[2] += fs[2];
   }
   out[3 * A + 0] = -2.0 * s[0];
   out[3 * A + 1] = -2.0 * s[1];
@@ -2014,6 +2095,7 @@ struct DeviceGrid {
   EvalItem* pmitems = nullptr;  // point-major evaluation blocks (split-BF16)
   int npm = 0;
   float* phiT = nullptr;          // function-major phi (split-BF16 Fock operand)
+  float* ybuf = nullptr;          // Y = P_g phi per point (split-BF16 forces), point-major
   DensityItem* ditems = nullptr;
   FockItem* fitems = nullptr;
   int2* gtiles = nullptr;
@@ -2038,6 +2120,7 @@ struct DeviceGrid {
   DensityItem* citems = nullptr;  // force blocks
   double* gpart = nullptr;        // per block and function (3)
   double* gsum = nullptr;         // per group and function (3)
+  double* fsum = nullptr;         // per batch function (3): its groups' sums
   int* afn_off = nullptr;         // functions of each batch atom
   int* afn = nullptr;
   double* fout = nullptr;         // forces, natom x 3 (atom-major)
@@ -2512,6 +2595,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->gitems = c.take<GatherItem>(gitems.size());
     dg->pmitems = c.take<EvalItem>(pmitems.size());
     dg->phiT = c.take<float>(fo);
+    dg->ybuf = c.take<float>(fo);
     dg->ditems = c.take<DensityItem>(ditems.size());
     dg->fitems = c.take<FockItem>(fitems.size());
     dg->gtiles = c.take<int2>(gtiles.size());
@@ -2537,6 +2621,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->citems = c.take<DensityItem>(citems.size());
     dg->gpart = c.take<double>(go);
     dg->gsum = c.take<double>(3 * fnl.size());
+    dg->fsum = c.take<double>(3 * (size_t)M + 3);
     dg->afn_off = c.take<int>(afn_off.size());
     dg->afn = c.take<int>(afn.size());
     dg->fout = c.take<double>(3 * (size_t)dg->natom);
@@ -2917,8 +3002,9 @@ static void launch_density(DeviceGrid& g, hipStream_t s, const double* d_rho, in
   XCQ_HIP(hipGetLastError());
 }
 
+// yb: where the density kernel leaves Y = P_g phi for the force pass (or null).
 static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho, int spin,
-                              hipEvent_t* ev) {
+                              hipEvent_t* ev, float* yb = nullptr) {
   unsigned short* pg3 = static_cast<unsigned short*>(spin == 2 ? g.pg2 : g.pg);
   const double* rho = spin == 2 ? d_rho + g.packed : d_rho;
   const float* vals = static_cast<const float*>(g.vals);
@@ -2931,29 +3017,29 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
     } else if (g.density_variant == 2) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
     } else if (g.density_variant == 5) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
                            g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
                            g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
     } else if (g.density_variant == 3 || g.density_variant == 4) {
       const int nr = g.nrowsplit, ns = g.ndensity - nr;
       const DensityItem* si = g.ditems + nr;
@@ -2968,7 +3054,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr);
       } else {
         if (nr > 0)
           hipLaunchKernelGGL((k_density_bf3<1, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
@@ -2976,7 +3062,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr);
       }
     } else {
       if (g.gga)
@@ -3010,6 +3096,13 @@ static void launch_fock(DeviceGrid& g, hipStream_t s, const double* factor) {
 // XC forces after the density launches (P_g blocks and factors in place):
 // per-block partials and group sums per spin, then per-atom sums into
 // d_forces (natom x 3, atom-major).
+// Closed-shell split-BF16 builds whose density kernel is k_density_bf2
+// (variants 1, 2, 5) leave Y for the forces in ybuf.
+static bool ystore_forces(const DeviceGrid& g) {
+  return g.prec == Precision::BF16X6 && !g.open && g.ybuf &&
+         (g.density_variant == 1 || g.density_variant == 2 || g.density_variant == 5);
+}
+
 static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
   const int nspin = g.open ? 2 : 1;
   for (int sp = 0; sp < nspin; ++sp) {
@@ -3024,6 +3117,9 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
         hipLaunchKernelGGL(k_force<float>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
                            static_cast<const float*>(pg), static_cast<const float*>(g.vals),
                            factor, g.gpart);
+      else if (ystore_forces(g))
+        hipLaunchKernelGGL(k_force_y, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups, g.ybuf,
+                           static_cast<const float*>(g.vals), factor, g.gpart);
       else
         hipLaunchKernelGGL(k_force_bf, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
                            static_cast<const unsigned short*>(pg),
@@ -3033,9 +3129,12 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
       hipLaunchKernelGGL(k_force_group, dim3(g.ngroups), dim3(256), 0, s, g.groups, g.gpart,
                          g.gsum, sp);
   }
+  if (g.M > 0)
+    hipLaunchKernelGGL(k_force_fn, dim3((g.M + 255) / 256), dim3(256), 0, s, g.memb_off, g.memb,
+                       g.groups, g.gsum, g.M, g.fsum);
   if (g.natom > 0)
     hipLaunchKernelGGL(k_force_atom, dim3((g.natom + 63) / 64), dim3(64), 0, s, g.afn_off, g.afn,
-                       g.memb_off, g.memb, g.groups, g.gsum, g.natom, d_forces);
+                       g.fsum, g.natom, d_forces);
   XCQ_HIP(hipGetLastError());
 }
 
@@ -3048,6 +3147,8 @@ static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double
                        hipEvent_t* ev, double* d_forces = nullptr) {
   mark(ev, 0, s);
   const int nspin = g.open ? 2 : 1;
+  // closed-shell split-BF16 forces take Y = P_g phi from the density pass
+  const bool ystore = d_forces && ystore_forces(g);
   for (int sp = 0; sp < nspin; ++sp) {
     const int spin = g.open ? sp + 1 : 0;
     if (g.prec == Precision::FP64)
@@ -3055,7 +3156,7 @@ static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double
     else if (g.prec == Precision::FP32)
       launch_density<float>(g, s, d_rho, spin, ev);
     else
-      launch_density_bf(g, s, d_rho, spin, ev);
+      launch_density_bf(g, s, d_rho, spin, ev, ystore ? g.ybuf : nullptr);
   }
   mark(ev, 2, s);
   for (int sp = 0; d_fock && sp < nspin; ++sp) {
