@@ -1684,19 +1684,22 @@ __global__ __launch_bounds__(256) void k_force_y(const DensityItem* __restrict__
 // gsum[fn_off + j][a] = sum over the group's blocks (in order) of gpart; one
 // block per group.
 // accumulate: add onto gsum (the second spin of an open shell).
+// One thread per (group, element), blockIdx.x the group; the loads of
+// several blocks are in flight together, the sum stays in block order.
 __global__ __launch_bounds__(256) void k_force_group(const DevGroup* __restrict__ groups,
                                                      const double* __restrict__ gpart,
                                                      double* __restrict__ gsum, int accumulate) {
   const DevGroup g = groups[blockIdx.x];
+  const int e = blockIdx.y * 256 + threadIdx.x;
+  if (e >= 3 * g.m_pad) return;
   const int nblk = (g.npts_pad + kDensPts - 1) / kDensPts;
   const long long stride = 3LL * g.m_pad;
-  const double* src = gpart + g.force_off;
-  for (int e = threadIdx.x; e < 3 * g.m_pad; e += 256) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += src[b * stride + e];
-    double* dst = gsum + 3LL * g.fn_off + e;
-    *dst = accumulate ? *dst + s : s;
-  }
+  const double* src = gpart + g.force_off + e;
+  double s = 0.0;
+#pragma unroll 8
+  for (int b = 0; b < nblk; ++b) s += src[b * stride];
+  double* dst = gsum + 3LL * g.fn_off + e;
+  *dst = accumulate ? *dst + s : s;
 }
 
 // Per batch function I, the sum of its group sums over the groups holding it
@@ -2069,7 +2072,7 @@ double wall() {
 // ---------------------------------------------------------------------------
 struct DeviceGrid {
   int M = 0, natom = 0, iexch = 9, nval = 10;  // M, natom: whole batch
-  int nmol = 1, max_M = 0, max_ng = 0;
+  int nmol = 1, max_M = 0, max_ng = 0, max_mpad = 0;
   int fock_slice = kSliceDefault;
   int density_variant = 2;
   int nrowsplit = 0;  // leading density items on the row-split kernel
@@ -2444,6 +2447,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     d.npts_pad = (n + kPtAlign - 1) / kPtAlign * kPtAlign;
     d.m = m;
     d.m_pad = (m + fn_align - 1) / fn_align * fn_align;
+    dg->max_mpad = std::max(dg->max_mpad, d.m_pad);
     d.pt_off = (int)pt;
     d.fn_off = (int)fnl.size();
     d.ntile = (d.m_pad + kFnChunk - 1) / kFnChunk;
@@ -3126,8 +3130,8 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
                            static_cast<const float*>(g.vals), factor, g.gpart);
     }
     if (g.ngroups > 0)
-      hipLaunchKernelGGL(k_force_group, dim3(g.ngroups), dim3(256), 0, s, g.groups, g.gpart,
-                         g.gsum, sp);
+      hipLaunchKernelGGL(k_force_group, dim3(g.ngroups, (3 * g.max_mpad + 255) / 256), dim3(256),
+                         0, s, g.groups, g.gpart, g.gsum, sp);
   }
   if (g.M > 0)
     hipLaunchKernelGGL(k_force_fn, dim3((g.M + 255) / 256), dim3(256), 0, s, g.memb_off, g.memb,
