@@ -58,8 +58,10 @@ void g2g_new_grid_(const unsigned* grid_type);
 
 /* Replaces g2g_solve_groups_ (g2g/init.cpp:332-391).  computation_type:
  * 0 Fock (energy too when energy_all_iterations), 1 energy only,
- * 2 energy + forces, 3 forces.  Forces and open shell are not provided by
- * this build and stop the process with a message. */
+ * 2 energy + forces (forces are added to `forces`, natom x 3 with leading
+ * dimension max_atoms), 3 forces only.  Open shell (OPEN in
+ * g2g_parameter_init_) reads rhoalpha/rhobeta and adds to fock_vec and
+ * fockb_vec.  *energy is assigned; fock_vec and forces are accumulated. */
 void g2g_solve_groups_(const unsigned* computation_type, double* energy, double* forces);
 
 /* Replaces g2g_deinit_ (g2g/init.cpp:221-236). */

@@ -95,14 +95,17 @@ struct DevGroup {
   long long rad_off;   // radial planes R0, R1, R2: 3 x npts_pad x nsh_pad
 };
 
-// Per expanded function of a group in the radial-Hessian layout: its
-// centre, the factor relating its contraction to its shell's first function
-// (the d xx/yy/zz normalisation) and (shell, lx, ly, lz) packed as
-// shell | lx << 16 | ly << 20 | lz << 24.
+// Per expanded function of a group in the radial-Hessian and in-kernel
+// evaluation layouts: its centre, the factor relating its contraction to its
+// shell's first function (the d xx/yy/zz normalisation) and (shell, lx, ly,
+// lz) packed as shell | lx << 16 | ly << 20 | lz << 24, shell counted from
+// the group's first shell.  Padding functions carry scale 0 and the last
+// real function's shell.
 struct FnMeta {
   double cx, cy, cz;
-  float scale;
+  double scale;
   int code;
+  int pad_;
 };
 
 // One molecule of a batch: its functions are [fn_base, fn_base + M) of the
@@ -142,23 +145,30 @@ __device__ inline Axis axis(double v, int l) {
 // its derivatives (g2g/cpu/functions.cpp:56-354): NV = 1 value, 4 adds the
 // gradient (LDA with forces), 10 also the Hessian (xx, yy, zz, xy, xz, yz).  Primitives with
 // a r^2 > 70 are skipped as in the reference.
+// Radial sums of a contraction at squared distance r2: t = sum c e, tg =
+// sum c a e, th = sum c a^2 e with e = exp(-a r2), primitives with a r2 > 70
+// skipped (g2g/cpu/functions.cpp:56-82).  NV > 1 adds tg, NV > 4 th.
 template <int NV>
-__device__ inline void eval_basis(const DevFn& b, double px, double py, double pz, double* o) {
-  const double vx = px - b.cx, vy = py - b.cy, vz = pz - b.cz;
-  const double r2 = vx * vx + vy * vy + vz * vz;
-  double t = 0.0, tg = 0.0, th = 0.0;
+__device__ inline void radial_sums(const DevFn& b, double r2, double& t, double& tg, double& th) {
+  t = tg = th = 0.0;
   const int nc = b.ncont;
   for (int k = 0; k < nc; ++k) {
     const double a = b.a[k];
     if (a * r2 > 70.0) continue;
     const double e = exp(-a * r2) * b.c[k];
     t += e;
-    if (NV > 1) {
-      tg += e * a;
-      th += e * (a * a);
-    }
+    if (NV > 1) tg += e * a;
+    if (NV > 4) th += e * (a * a);
   }
-  const Axis X = axis(vx, b.lx), Y = axis(vy, b.ly), Z = axis(vz, b.lz);
+}
+
+// Values of x^lx y^ly z^lz g(r) from the radial sums of g at offset v from
+// the centre: NV = 1 the value, 4 adds the gradient, 10 the Hessian (xx, yy,
+// zz, xy, xz, yz); the operations of g2g/cpu/functions.cpp:83-354.
+template <int NV>
+__device__ inline void basis_from_radial(int lx, int ly, int lz, double vx, double vy, double vz,
+                                         double t, double tg, double th, double* o) {
+  const Axis X = axis(vx, lx), Y = axis(vy, ly), Z = axis(vz, lz);
   const double P = X.m * Y.m * Z.m;
   o[0] = P * t;
   if (NV == 1) return;
@@ -176,6 +186,15 @@ __device__ inline void eval_basis(const DevFn& b, double px, double py, double p
   o[7] = Pxy * t + Px * gy + Py * gx + P * (4.0 * vx * vy * th);
   o[8] = Pxz * t + Px * gz + Pz * gx + P * (4.0 * vx * vz * th);
   o[9] = Pyz * t + Py * gz + Pz * gy + P * (4.0 * vy * vz * th);
+}
+
+template <int NV>
+__device__ inline void eval_basis(const DevFn& b, double px, double py, double pz, double* o) {
+  const double vx = px - b.cx, vy = py - b.cy, vz = pz - b.cz;
+  const double r2 = vx * vx + vy * vy + vz * vz;
+  double t, tg, th;
+  radial_sums<NV>(b, r2, t, tg, th);
+  basis_from_radial<NV>(b.lx, b.ly, b.lz, vx, vy, vz, t, tg, th, o);
 }
 
 // 16x16x4 MFMA in FP64 or FP32.  Operands: lane l supplies A[l%16][l/16] and
@@ -759,15 +778,18 @@ __device__ inline void radial_hessian(const FnMeta& mt, float vx, float vy, floa
   h[5] = s * (Xm * Yd * Zd * r0 + (Py * vz + Pz * vy) * r1 + P * vy * vz * r2);
 }
 
-// Density block P_g as three BF16 piece planes [piece][j][k] (true symmetric
+// Density block P_g as NPC BF16 piece planes [piece][j][k] (true symmetric
 // P), split in blocks of 32 consecutive k of one row exactly as split3 does
-// (the 32 threads of a block are 32 consecutive lanes of one wave).
+// (the 32 threads of a block are 32 consecutive lanes of one wave); a fourth
+// piece carries the next 8 bits (units of 2^(e-32)) for the products that
+// need P to 2^-32 of the block maximum.
+template <int NPC>
 __global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __restrict__ items,
                                                            const DevGroup* __restrict__ groups,
                                                            const DevMol* __restrict__ mols,
                                                            const int* __restrict__ fnl,
                                                            const double* __restrict__ rho,
-                                                           unsigned short* __restrict__ pg3) {
+                                                           unsigned short* __restrict__ pgs) {
   const GatherItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int e = it.start + threadIdx.x;
@@ -786,12 +808,14 @@ __global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __r
   const double a0 = rint(xs);
   const double r1 = (xs - a0) * 256.0;
   const double a1 = rint(r1);
-  const double a2 = rint((r1 - a1) * 256.0);
+  const double r2 = (r1 - a1) * 256.0;
+  const double a2 = rint(r2);
   if (!in) return;
-  __bf16* out = reinterpret_cast<__bf16*>(pg3) + 3 * g.pg_off + e;
+  __bf16* out = reinterpret_cast<__bf16*>(pgs) + NPC * g.pg_off + e;
   out[0] = (__bf16)(float)ldexp(a0, ex - 8);
   out[psz] = (__bf16)(float)ldexp(a1, ex - 16);
   out[2 * psz] = (__bf16)(float)ldexp(a2, ex - 24);
+  if (NPC == 4) out[3 * psz] = (__bf16)(float)ldexp(rint((r2 - a2) * 256.0), ex - 32);
 }
 
 // Density + functional on split-BF16 MFMA: same blocking as k_density (64
@@ -1398,6 +1422,444 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// split-BF16 density with the basis evaluated in the kernel (density variant 6)
+// ---------------------------------------------------------------------------
+// No value planes are read from HBM.  A block of 64 points (16 per wave)
+// evaluates phi, its gradient and its Hessian from per-shell radial sums
+// (sum c e, sum c a e, sum c a^2 e with e = exp(-a r^2)) that each wave
+// computes in FP64 for its own points and keeps in LDS: once per K step for
+// the K operand and once per row tile for the row-dot epilogue.  The group
+// metadata says which shell every function belongs to (FnMeta) and which
+// shells a K step or row tile touches (kbsh, tsh).
+//
+// Precision.  Y_0 = P_g phi gives rho and its gradient, hence Exc and N.
+// Its K operand is split from FP64 values into four BF16 pieces (block
+// fixed point, units down to 2^-32 of the block maximum), P_g has four
+// pieces too, and the tile product is the ten piece products of weight
+// >= 2^-24 relative; the epilogue multiplies by FP64 values.  Single-precision
+// operands instead leave a rounding error that does not average out: on a
+// Becke/Lebedev grid all points of a radial shell see the same value of the
+// atom's own s functions, so their rounding repeats over every point of the
+// shell (and over every atom of the same element) and moved N by 1e-6 at
+// (H2O)32.  Y_a = P_g d_a phi (a = x, y, z) only enter the second-derivative
+// terms of the potential; they keep three-piece operands (the FP64 gradient
+// rounded to FP32) and the six-product chain.
+//
+// Schedule per K step: a single LDS buffer holds the step's P_g tile (four
+// pieces, copied straight from global memory); the tile and the radial sums
+// of the next step are produced while the block waits at the barrier that
+// ends this step, so each step runs [radial sums of K shells, DMA in flight]
+// -> barrier -> [K operand, MFMAs] -> barrier.  Two blocks share a CU.
+
+constexpr int kEvShK = 32;  // shells one K step (32 functions) touches at most
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block fixed-point split of 8 FP64 values (a lane's 8 K values of one
+// point; the same 4-lane block as split3) into four BF16 pieces in units
+// 2^(e-8), 2^(e-16), 2^(e-24), 2^(e-32), 2^(e-1) <= max |x| < 2^e.  Adding
+// and removing 1.5 * 2^(e+44) rounds a double of magnitude below 2^e to the
+// nearest multiple of 2^(e-8) (the sum lies in [2^(e+44), 2^(e+45)), whose
+// spacing is 2^(e-8)); the remainders are exact.  Every piece is an integer
+// of magnitude at most 256 times its unit, exact in FP32, and its BF16 form
+// is the upper half of the FP32 bits.  All 64 lanes must call it.
+__device__ inline void split4d(const double (&x)[8], bf8 (&pc)[4]) {
+  double mx = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) mx = fmax(mx, fabs(x[q]));
+  mx = fmax(mx, __shfl_xor(mx, 16));
+  mx = fmax(mx, __shfl_xor(mx, 32));
+  int e;
+  (void)frexp(mx, &e);
+  const double c0 = ldexp(1.5, e + 44), c1 = ldexp(1.5, e + 36), c2 = ldexp(1.5, e + 28),
+               c3 = ldexp(1.5, e + 20);
+  unsigned b[4][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double v0 = (x[q] + c0) - c0;
+    double r = x[q] - v0;
+    const double v1 = (r + c1) - c1;
+    r -= v1;
+    const double v2 = (r + c2) - c2;
+    r -= v2;
+    const double v3 = (r + c3) - c3;
+    b[0][q] = __builtin_bit_cast(unsigned, (float)v0);
+    b[1][q] = __builtin_bit_cast(unsigned, (float)v1);
+    b[2][q] = __builtin_bit_cast(unsigned, (float)v2);
+    b[3][q] = __builtin_bit_cast(unsigned, (float)v3);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    u4 u;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) u[p] = (b[k][2 * p + 1] & 0xffff0000u) | (b[k][2 * p] >> 16);
+    pc[k] = __builtin_bit_cast(bf8, u);
+  }
+}
+
+// The same split from an FP32 pair (hi + lo, |lo| <= ulp(hi) / 2: a double
+// rounded to FP32 and its remainder): the first three pieces come from hi
+// as in split3, the fourth from hi's remainder plus lo (their FP32 sum is
+// off by at most 2^-48 of the block maximum, far below the piece's unit
+// 2^(e-32)).  Same pieces as split4d of hi + lo up to that rounding.
+__device__ inline void split4f(const float (&hi)[8], const float (&lo)[8], bf8 (&pc)[4]) {
+  float mx = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) mx = fmaxf(mx, fabsf(hi[q]));
+  mx = fmaxf(mx, __shfl_xor(mx, 16));
+  mx = fmaxf(mx, __shfl_xor(mx, 32));
+  int e;
+  (void)frexpf(mx, &e);
+  const float c0 = ldexpf(1.5f, e + 15), c1 = ldexpf(1.5f, e + 7), c2 = ldexpf(1.5f, e - 1),
+              c3 = ldexpf(1.5f, e - 9);
+  unsigned b[4][8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const float v0 = (hi[q] + c0) - c0;
+    float r = hi[q] - v0;
+    const float v1 = (r + c1) - c1;
+    r -= v1;
+    const float v2 = (r + c2) - c2;
+    r = (r - v2) + lo[q];
+    const float v3 = (r + c3) - c3;
+    b[0][q] = __builtin_bit_cast(unsigned, v0);
+    b[1][q] = __builtin_bit_cast(unsigned, v1);
+    b[2][q] = __builtin_bit_cast(unsigned, v2);
+    b[3][q] = __builtin_bit_cast(unsigned, v3);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    u4 u;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) u[p] = (b[k][2 * p + 1] & 0xffff0000u) | (b[k][2 * p] >> 16);
+    pc[k] = __builtin_bit_cast(bf8, u);
+  }
+}
+
+// The ten piece products of weight >= 2^-24 of four-piece operands, as two
+// chains from zero, largest first: hi = the products of weight 1 and 2^-8,
+// lo = those of weight 2^-16 and 2^-24 (each chain keeps 24 bits of its own
+// scale, so the small products are not rounded away against the large ones).
+__device__ inline void mma10(const bf8 (&a)[4], const bf8 (&b)[4], f4& hi, f4& lo) {
+  hi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], f4{0, 0, 0, 0}, 0, 0, 0);
+  hi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], hi, 0, 0, 0);
+  hi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], hi, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], f4{0, 0, 0, 0}, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[3], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[2], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[1], lo, 0, 0, 0);
+  lo = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3], b[0], lo, 0, 0, 0);
+}
+
+// Radial sums of shells [r.x, r.x + r.y) of group g (at most ST / 16) at
+// the wave's 16 points into the wave's LDS table rw: t at [s][p], tg at
+// ST + [s][p], th (NV = 10) at 2 ST + [s][p] (16 doubles per shell row).
+// Lane (lrow, lk) takes shells lk, lk + 4, ... at point lrow.
+template <int NV, int ST>
+__device__ inline void radial_table(const DevGroup& g, int2 r, const int* __restrict__ shells,
+                                    const DevFn* __restrict__ bas, double ppx, double ppy,
+                                    double ppz, int lrow, int lk, double* rw) {
+  for (int sh = lk; sh < r.y; sh += 4) {
+    const DevFn& b = bas[shells[g.shell_off + r.x + sh]];
+    const double vx = ppx - b.cx, vy = ppy - b.cy, vz = ppz - b.cz;
+    double t, tg, th;
+    radial_sums<NV>(b, vx * vx + vy * vy + vz * vz, t, tg, th);
+    rw[sh * 16 + lrow] = t;
+    if (NV > 1) rw[ST + sh * 16 + lrow] = tg;
+    if (NV > 4) rw[2 * ST + sh * 16 + lrow] = th;
+  }
+}
+
+// Values of local function jl of group g at the lane's point from the radial
+// table whose first shell is sh0 (zeros for padding functions).
+template <int NV, int ST>
+__device__ inline void basis_from_table(const DevGroup& g, const FnMeta* __restrict__ fmeta,
+                                        int jl, int sh0, double ppx, double ppy, double ppz,
+                                        int lrow, const double* rw, double (&o)[NV]) {
+  if (jl >= g.m) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) o[k] = 0.0;
+    return;
+  }
+  const FnMeta mt = fmeta[g.fn_off + jl];
+  const int sl = ((mt.code & 0xffff) - sh0) * 16 + lrow;
+  const double t = rw[sl];
+  const double tg = NV > 1 ? rw[ST + sl] : 0.0;
+  const double th = NV > 4 ? rw[2 * ST + sl] : 0.0;
+  basis_from_radial<NV>((mt.code >> 16) & 3, (mt.code >> 20) & 3, (mt.code >> 24) & 3,
+                        ppx - mt.cx, ppy - mt.cy, ppz - mt.cz, t, tg, th, o);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) o[k] *= mt.scale;
+}
+
+// LDS layout of the K operand values of one K step (32 functions at the
+// block's 64 points), written by the waves that evaluate them and read by the
+// MFMA lanes, [plane][point][function] in FP32 with rows of 36 floats (the 16
+// lanes of a lane group -- 16 consecutive points, 8 consecutive functions
+// each -- then hit distinct banks): planes 0 and 1 are phi as an FP32 pair
+// (the FP64 value rounded, and its remainder), planes 2-4 the gradient.
+constexpr int kEvRow = 36, kEvPlane = 64 * kEvRow;
+
+template <int NT, int ROWS>
+__global__ __launch_bounds__(256, 2) void k_density_ev(const DensityItem* __restrict__ items,
+                                                    const DevGroup* __restrict__ groups,
+                                                    const unsigned short* __restrict__ pg4,
+                                                    const FnMeta* __restrict__ fmeta,
+                                                    const int* __restrict__ shells,
+                                                    const int2* __restrict__ sfn,
+                                                    const int2* __restrict__ kbsh,
+                                                    const int2* __restrict__ tsh,
+                                                    const DevFn* __restrict__ bas,
+                                                    const double* __restrict__ px,
+                                                    const double* __restrict__ py,
+                                                    const double* __restrict__ pz,
+                                                    const double* __restrict__ pw,
+                                                    double* __restrict__ factor,
+                                                    double* __restrict__ epart, int iexch,
+                                                    int spin, double* __restrict__ factor_b,
+                                                    double* __restrict__ sums, long long sstride,
+                                                    float* __restrict__ ybuf) {
+  constexpr int NS = (NT == 4) ? 10 : 1;
+  constexpr int RT = ROWS / 16;
+  constexpr int UNITS = 4 * ROWS * 4;  // 16-byte units of one staged P_g tile
+  constexpr int VBYTES = (NT == 4 ? 5 : 2) * kEvPlane * 4;
+  constexpr int RWS = (NT == 4) ? 768 : 256;  // doubles of one wave's epilogue radial table
+  static_assert(VBYTES >= 4 * RWS * 8, "the epilogue's radial tables reuse the value area");
+  __shared__ u4 sA[UNITS];                      // [piece][row][k octet]
+  __shared__ __attribute__((aligned(16))) char sV[VBYTES];  // K values, or radial tables
+  __shared__ double sE[4][2];
+  __shared__ double sS[NS][64];  // row-dot sums of the block's points between row chunks
+  float* sK = reinterpret_cast<float*>(sV);  // [plane][point][kEvRow]
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;  // uniform over the wave
+  const int mp = g.m_pad, nkb = mp >> 5, nchunk = (mp + ROWS - 1) / ROWS;
+  const long long psz = (long long)mp * mp;
+  // this lane's MFMA point and its evaluation point (block point `lane`);
+  // their coordinates are read where they are needed
+  const int pt = active ? p0 + lrow : it.p0;
+  const int q = g.pt_off + pt;
+  const int qe = g.pt_off + min(it.p0 + lane, g.npts_pad - 1);
+  const unsigned short* Pp = pg4 + 4 * g.pg_off;
+  double* rw = reinterpret_cast<double*>(sV) + RWS * wave;  // epilogue radial table
+  // unit c = threadIdx.x + 256 i of a tile is piece c / (4 ROWS), row
+  // (c / 4) % ROWS, k octet c % 4 (rows past the functions are clamped and
+  // never read)
+  auto dma = [&](int c0, int k0) {
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) {
+      const int c = threadIdx.x + 256 * i;
+      const int sp = c / (4 * ROWS), row = (c >> 2) & (ROWS - 1), oc = c & 3;
+      const int j = min(c0 + row, mp - 1);
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc),
+          (__attribute__((address_space(3))) void*)&sA[256 * i + 64 * wave], 16, 0, 0);
+    }
+  };
+  // K values of K block kb at the block's 64 points: wave w evaluates shells
+  // w, w + 4, ... of the block (one point per lane, so the shell, its
+  // primitives and its functions are the same over the wave); phi in FP64
+  // (kept as an FP32 pair), the gradient in FP32 from the FP64 radial sums.
+  // Padding functions are zero.
+  auto eval_k = [&](int kb) {
+    const int2 kr = kbsh[(g.fn_off >> 5) + kb];
+    const int k0 = 32 * kb;
+    float* row = sK + lane * kEvRow - k0;
+    const double epx = px[qe], epy = py[qe], epz = pz[qe];
+    for (int i = wave; i < kr.y; i += 4) {
+      const int sh = g.shell_off + kr.x + i;
+      const DevFn& b = bas[shells[sh]];
+      const double vx = epx - b.cx, vy = epy - b.cy, vz = epz - b.cz;
+      double t, tg, th;
+      radial_sums<(NT == 4 ? 4 : 1)>(b, vx * vx + vy * vy + vz * vz, t, tg, th);
+      const float fx = (float)vx, fy = (float)vy, fz = (float)vz;
+      const float ft = (float)t, fg = (float)(-2.0 * tg);
+      const int2 fr = sfn[sh];
+      const int jb = max(fr.x, k0), je = min(fr.x + fr.y, k0 + 32);
+      for (int jl = jb; jl < je; ++jl) {
+        const FnMeta mt = fmeta[g.fn_off + jl];
+        const int lx = (mt.code >> 16) & 3, ly = (mt.code >> 20) & 3, lz = (mt.code >> 24) & 3;
+        const double Pd = (lx == 0 ? 1.0 : lx == 1 ? vx : vx * vx) *
+                          (ly == 0 ? 1.0 : ly == 1 ? vy : vy * vy) *
+                          (lz == 0 ? 1.0 : lz == 1 ? vz : vz * vz);
+        const double ph = mt.scale * (Pd * t);
+        const float hi = (float)ph;
+        row[jl] = hi;
+        row[kEvPlane + jl] = (float)(ph - (double)hi);
+        if constexpr (NT == 4) {
+          const float Xm = lx == 0 ? 1.f : lx == 1 ? fx : fx * fx;
+          const float Ym = ly == 0 ? 1.f : ly == 1 ? fy : fy * fy;
+          const float Zm = lz == 0 ? 1.f : lz == 1 ? fz : fz * fz;
+          const float Xd = lx == 0 ? 0.f : lx == 1 ? 1.f : 2.f * fx;
+          const float Yd = ly == 0 ? 0.f : ly == 1 ? 1.f : 2.f * fy;
+          const float Zd = lz == 0 ? 0.f : lz == 1 ? 1.f : 2.f * fz;
+          const float fs = (float)mt.scale, Pf = Xm * Ym * Zm * fg;
+          row[2 * kEvPlane + jl] = fs * (Xd * Ym * Zm * ft + Pf * fx);
+          row[3 * kEvPlane + jl] = fs * (Xm * Yd * Zm * ft + Pf * fy);
+          row[4 * kEvPlane + jl] = fs * (Xm * Ym * Zd * ft + Pf * fz);
+        }
+      }
+    }
+    // padding functions of the block (past the group's last function)
+    if (wave == 0)
+      for (int jl = max(g.m, k0); jl < k0 + 32; ++jl)
+#pragma unroll
+        for (int pl = 0; pl < (NT == 4 ? 5 : 2); ++pl) row[pl * kEvPlane + jl] = 0.f;
+  };
+  if (threadIdx.x < 64)
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sS[k][threadIdx.x] = 0.0;
+  dma(0, 0);
+  eval_k(0);
+  for (int c = 0; c < nchunk; ++c) {
+    const int c0 = c * ROWS, nrt = min(RT, (mp - c0) >> 4);
+    f4 acc[RT][NT];  // plane 0 unused: Y_0 is summed in FP64 (y0s)
+    d4 y0s[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[rt][t] = f4{0, 0, 0, 0};
+      y0s[rt] = d4{0, 0, 0, 0};
+    }
+    for (int kb = 0; kb < nkb; ++kb) {
+      __syncthreads();  // the step's P_g tile and K values are in LDS
+      if (active) {
+        // K operand: functions 32 kb + 8 lk + q of this lane's point
+        const float* kv = sK + (16 * wave + lrow) * kEvRow + 8 * lk;
+        float xh[8], xl[8];
+        float xg[3][8];
+        load8(kv, xh);
+        load8(kv + kEvPlane, xl);
+        if constexpr (NT == 4) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) load8(kv + (2 + a) * kEvPlane, xg[a]);
+        }
+        bf8 B0[4];
+        split4f(xh, xl, B0);
+        bf8 Bg[3][3];
+        if constexpr (NT == 4) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) split3(xg[a], Bg[a][0], Bg[a][1], Bg[a][2]);
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          if (rt < nrt) {
+            const int u = (16 * rt + lrow) * 4 + lk;
+            bf8 A[4];
+#pragma unroll
+            for (int pc = 0; pc < 4; ++pc) A[pc] = __builtin_bit_cast(bf8, sA[pc * ROWS * 4 + u]);
+            f4 hi, lo;
+            mma10(A, B0, hi, lo);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) y0s[rt][i] += (double)hi[i] + (double)lo[i];
+            if constexpr (NT == 4) {
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+                acc[rt][1 + a] += mma6(A[0], A[1], A[2], Bg[a][0], Bg[a][1], Bg[a][2]);
+            }
+          }
+        }
+      }
+      __syncthreads();  // every wave is done with the tile and the K values
+      if (kb == nkb - 1) {
+        // row-dot epilogue of this row chunk with FP64 values (its radial
+        // tables take the K value area, one per wave)
+        if (active) {
+          const double ppx = px[q], ppy = py[q], ppz = pz[q];
+          double s[NS];
+#pragma unroll
+          for (int k = 0; k < NS; ++k) s[k] = 0.0;
+          if (ybuf) {  // Y = P_g phi of this point for the force pass
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+              if (rt < nrt)
+                *reinterpret_cast<fl4*>(ybuf + g.phi_off + (long long)pt * mp + c0 + 16 * rt +
+                                        4 * lk) = fl4{(float)y0s[rt][0], (float)y0s[rt][1],
+                                                      (float)y0s[rt][2], (float)y0s[rt][3]};
+          }
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) {
+            if (rt >= nrt) continue;
+            const int2 tr = tsh[(g.fn_off >> 4) + (c0 >> 4) + rt];
+            wave_lds_sync();  // the previous readers of the table are done
+            radial_table<NS, 256>(g, tr, shells, bas, ppx, ppy, ppz, lrow, lk, rw);
+            wave_lds_sync();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              double v[NS];
+              basis_from_table<NS, 256>(g, fmeta, c0 + 16 * rt + 4 * lk + i, tr.x, ppx, ppy,
+                                        ppz, lrow, rw, v);
+              const double y0 = y0s[rt][i];
+              s[0] += v[0] * y0;
+              if constexpr (NT == 4) {
+                const double yx = acc[rt][1][i], yy = acc[rt][2][i], yz = acc[rt][3][i];
+                s[1] += v[1] * y0;
+                s[2] += v[2] * y0;
+                s[3] += v[3] * y0;
+                s[4] += v[4] * y0 + v[1] * yx;
+                s[5] += v[5] * y0 + v[2] * yy;
+                s[6] += v[6] * y0 + v[3] * yz;
+                s[7] += v[7] * y0 + v[1] * yy;
+                s[8] += v[8] * y0 + v[1] * yz;
+                s[9] += v[9] * y0 + v[2] * yz;
+              }
+            }
+          }
+          // this chunk's sums of the lane's point (its 4 lane groups, in a
+          // fixed order) onto the point's running sums
+#pragma unroll
+          for (int k = 0; k < NS; ++k) {
+            s[k] += __shfl_xor(s[k], 16);
+            s[k] += __shfl_xor(s[k], 32);
+          }
+          if (lk == 0)
+#pragma unroll
+            for (int k = 0; k < NS; ++k) sS[k][16 * wave + lrow] += s[k];
+        }
+        __syncthreads();  // the radial tables are done before the next K values
+      }
+      const int nc = kb + 1 < nkb ? c : c + 1, nk = kb + 1 < nkb ? kb + 1 : 0;
+      if (nc < nchunk) {
+        dma(nc * ROWS, 32 * nk);
+        eval_k(nk);
+      }
+    }
+  }
+  double e_w = 0.0, n_w = 0.0;
+  if (active) {
+    if (lk == 0) {
+      double s[NS];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) s[k] = sS[k][16 * wave + lrow];
+      point_stage<NT>(s, g.pt_off + p0 + lrow, iexch, spin, pw, factor, factor_b, sums, sstride,
+                      e_w, n_w);
+    }
+    for (int o = 8; o >= 1; o >>= 1) {
+      e_w += __shfl_xor(e_w, o);
+      n_w += __shfl_xor(n_w, o);
+    }
+  }
+  if (lane == 0) {
+    sE[wave][0] = e_w;
+    sE[wave][1] = n_w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    epart[2 * blockIdx.x] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
+    epart[2 * blockIdx.x + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // XC forces (computation types 2 and 3)
 // ---------------------------------------------------------------------------
 // LIO's XC force on nucleus A (g2g/cpu/iteration.cpp:213-268) is
@@ -1667,6 +2129,83 @@ __global__ __launch_bounds__(256) void k_force_y(const DensityItem* __restrict__
         w[3 * i] = yy * (double)v[0][i];
         w[3 * i + 1] = yy * (double)v[1][i];
         w[3 * i + 2] = yy * (double)v[2][i];
+      }
+      force_rows(w, lrow, sG[wave], jj);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * kFnChunk) {
+      const int jj = threadIdx.x / 3, a = threadIdx.x % 3;
+      if (c0 + jj < mp)
+        out[(long long)(c0 + jj) * 3 + a] =
+            ((sG[0][jj][a] + sG[1][jj][a]) + sG[2][jj][a]) + sG[3][jj][a];
+    }
+    __syncthreads();
+  }
+}
+
+// Forces of density variant 6: Y = P_g phi from the density kernel (ybuf,
+// point-major, the FP32 sums of the ten-product chain) and the gradient of
+// every function evaluated here in FP64 from the per-shell radial sums of
+// each row tile; the same per-function sums, lane and wave order as
+// k_force_y.
+__global__ __launch_bounds__(256) void k_force_ev(const DensityItem* __restrict__ items,
+                                                  const DevGroup* __restrict__ groups,
+                                                  const float* __restrict__ ybuf,
+                                                  const FnMeta* __restrict__ fmeta,
+                                                  const int* __restrict__ shells,
+                                                  const int2* __restrict__ tsh,
+                                                  const DevFn* __restrict__ bas,
+                                                  const double* __restrict__ px,
+                                                  const double* __restrict__ py,
+                                                  const double* __restrict__ pz,
+                                                  const double* __restrict__ factor,
+                                                  double* __restrict__ gpart) {
+  __shared__ double sG[4][kFnChunk][3];
+  __shared__ double sR[4][512];
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;  // uniform over the wave
+  const int mp = g.m_pad;
+  const int pt = active ? p0 + lrow : it.p0;
+  const int q = g.pt_off + pt;
+  const double ppx = px[q], ppy = py[q], ppz = pz[q];
+  const float* Yp = ybuf + g.phi_off + (long long)pt * mp;
+  const double f = active ? factor[q] : 0.0;
+  double* rw = sR[wave];
+  double* out = gpart + g.force_off + (long long)(it.p0 / kDensPts) * mp * 3;
+  for (int c0 = 0; c0 < mp; c0 += kFnChunk) {
+    const int nrt = min(4, (mp - c0) / 16);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      fl4 y = fl4{0, 0, 0, 0};
+      double v[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) v[i][a] = 0.0;
+      if (active && rt < nrt) {
+        const int2 tr = tsh[(g.fn_off >> 4) + (c0 >> 4) + rt];
+        wave_lds_sync();
+        radial_table<4, 256>(g, tr, shells, bas, ppx, ppy, ppz, lrow, lk, rw);
+        wave_lds_sync();
+        const int j = c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
+        y = *reinterpret_cast<const fl4*>(Yp + j);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          basis_from_table<4, 256>(g, fmeta, j + i, tr.x, ppx, ppy, ppz, lrow, rw, v[i]);
+      }
+      double w[12];
+      int jj[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        jj[i] = 16 * rt + 4 * lk + i;
+        const double yy = f * (double)y[i];
+        w[3 * i] = yy * v[i][1];
+        w[3 * i + 1] = yy * v[i][2];
+        w[3 * i + 2] = yy * v[i][3];
       }
       force_rows(w, lrow, sG[wave], jj);
     }
@@ -2098,7 +2637,9 @@ struct DeviceGrid {
   EvalItem* pmitems = nullptr;  // point-major evaluation blocks (split-BF16)
   int npm = 0;
   float* phiT = nullptr;          // function-major phi (split-BF16 Fock operand)
-  float* ybuf = nullptr;          // Y = P_g phi per point (split-BF16 forces), point-major
+  float* ybuf = nullptr;          // Y = P_g phi per point (split-BF16 forces), point-major;
+                                  // allocated at the first forces build (not in the arena)
+  size_t ybuf_cap = 0;            // its bytes
   DensityItem* ditems = nullptr;
   FockItem* fitems = nullptr;
   int2* gtiles = nullptr;
@@ -2128,6 +2669,11 @@ struct DeviceGrid {
   int* afn = nullptr;
   double* fout = nullptr;         // forces, natom x 3 (atom-major)
   bool radial = false;            // radial-Hessian layout (split-BF16 variant 5)
+  bool inkernel = false;          // basis evaluated in the density kernel (variant 6)
+  int2* kbsh = nullptr;           // per 32-function K block: first shell, shell count
+  int2* sfn = nullptr;            // per group shell: first local function, function count
+  int2* tsh = nullptr;            // per 16-function row tile: first shell, shell count
+  long long phi_elems = 0;        // elements of the function-major phi plane
   int* shells = nullptr;          // first (batch) function of every group shell
   FnMeta* fmeta = nullptr;        // per padded function of every group
   float* rad = nullptr;           // radial planes
@@ -2267,6 +2813,7 @@ static void free_arena(char* a, size_t cap, int device, hipStream_t s) {
 static void release(DeviceGrid* g, hipStream_t s, int device) {
   if (!g) return;
   free_arena(g->arena, g->arena_cap, device, s);
+  free_arena(reinterpret_cast<char*>(g->ybuf), g->ybuf_cap, device, s);
   if (g->h_rho) (void)hipHostFree(g->h_rho);
   if (g->h_F) (void)hipHostFree(g->h_F);
   if (g->h_e) (void)hipHostFree(g->h_e);
@@ -2309,6 +2856,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
                              const std::vector<const GridPartition*>& parts, Precision prec,
                              const DevicePoints* dpts) {
   XCQ_HIP(hipSetDevice(device_));
+  check_not_pending("a grid load");
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
   const double t_plan = wall();
   load_ms_[0] = load_ms_[1] = load_ms_[2] = load_ms_[3] = 0.0;
@@ -2333,11 +2881,19 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   if (tuning_.fock_slice < 128 || tuning_.fock_slice % 128 != 0)
     throw std::runtime_error("g2g: the Fock slice must be a positive multiple of 128 points");
   auto* dg = new DeviceGrid;
+  // until the grid is committed below, every exit frees what it holds (its
+  // arena -- possibly the previous grid's, handed over -- and pinned buffers)
+  struct Guard {
+    DeviceGrid* g;
+    hipStream_t s;
+    int device;
+    ~Guard() { release(g, s, device); }
+  } guard{dg, (hipStream_t)stream_, device_};
   dg->nmol = nmol;
   dg->fock_slice = tuning_.fock_slice;
   dg->density_variant = tuning_.density_variant;
 
-  if (dg->density_variant < 0 || dg->density_variant > 5)
+  if (dg->density_variant < 0 || dg->density_variant > 7)
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
@@ -2346,27 +2902,28 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     if (m->open != dg->open)
       throw std::runtime_error("g2g: the molecules of a batch must all be open or all closed shell");
   if (dg->open && !dg->gga) {
-    delete dg;
     throw std::runtime_error("g2g: open-shell LDA is not provided (LIO's CPU path evaluates no "
                              "functional there and reads unset factors)");
   }
   if (dg->open && prec == Precision::BF16X6 && dg->density_variant != 1 &&
-      dg->density_variant != 2 && dg->density_variant != 5) {
-    delete dg;
-    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2 or 5");
+      dg->density_variant != 2 && dg->density_variant < 5) {
+    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2, 5, 6 or 7");
   }
   const int nspin = dg->open ? 2 : 1;
   // LDA keeps the gradient too: the XC forces need it.  The radial-Hessian
   // layout keeps phi and its gradient as planes and rebuilds the second
   // derivatives from per-shell radial sums.
   dg->radial = prec == Precision::BF16X6 && tuning_.density_variant == 5;
-  dg->nval = (dg->gga && !dg->radial) ? 10 : 4;
+  dg->inkernel = prec == Precision::BF16X6 && tuning_.density_variant >= 6;
+  // in-kernel evaluation keeps no value planes (phi alone, function-major,
+  // for the Fock kernel)
+  dg->nval = dg->inkernel ? 0 : (dg->gga && !dg->radial) ? 10 : 4;
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const bool bf = prec == Precision::BF16X6;
   // split-BF16 K steps are 32 functions; its P_g holds three 2-byte pieces
   const int fn_align = bf ? 32 : kFnAlign;
-  const size_t pbytes = bf ? 3 * sizeof(unsigned short) : vbytes;
+  const size_t pbytes = bf ? (dg->inkernel ? 4 : 3) * sizeof(unsigned short) : vbytes;
 
   // batch numbering: molecule k owns functions [fn_base, fn_base + M) and
   // packed entries [rho_off, rho_off + M (M + 1) / 2)
@@ -2431,6 +2988,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   long long pt = 0, vo = 0, po = 0, fo = 0, go = 0, ro = 0;
   std::vector<int> shells;
   std::vector<FnMeta> fmeta;
+  std::vector<int2> tsh, kbsh, sfn;
   int fock_base = 0;
   std::vector<std::vector<int2>> membership(M);
   for (int g = 0, k = 0, gk = 0; g < G; ++g, ++gk) {
@@ -2468,11 +3026,16 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     d.shell_off = (int)shells.size();
     d.nsh_pad = 0;
     d.rad_off = ro;
-    if (dg->radial) {
+    if (dg->radial || dg->inkernel) {
       const Molecule& mol = *mols[k];
       int first = -1;
+      const size_t meta0 = fmeta.size();
       for (int i = 0; i < d.m_pad; ++i) {
-        FnMeta mt{0.0, 0.0, 0.0, 0.f, 0};
+        FnMeta mt{0.0, 0.0, 0.0, 0.0, 0, 0};
+        if (i >= m && i > 0) {  // padding: scale 0, the last real function's shell
+          mt = fmeta[meta0 + m - 1];
+          mt.scale = 0.0;
+        }
         if (i < m) {
           const int fm = part.fn[part.fn_off[gk] + i], f = hmol[k].fn_base + fm;
           const int comp = fm < mol.ns ? 0
@@ -2491,12 +3054,12 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
             same = b.a[q] == b0.a[q] &&
                    std::fabs(b.c[q] - ratio * b0.c[q]) <= 1e-12 * std::fabs(b.c[q]);
           if (!same)
-            throw std::runtime_error("g2g: density variant 5 needs the functions of a shell to "
-                                     "share centre, exponents and contraction");
+            throw std::runtime_error("g2g: density variants 5 and 6 need the functions of a "
+                                     "shell to share centre, exponents and contraction");
           mt.cx = b.cx;
           mt.cy = b.cy;
           mt.cz = b.cz;
-          mt.scale = (float)ratio;
+          mt.scale = ratio;
           mt.code = ((int)shells.size() - 1 - d.shell_off) | (b.lx << 16) | (b.ly << 20) |
                     (b.lz << 24);
         }
@@ -2505,7 +3068,24 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       const int nsh = (int)shells.size() - d.shell_off;
       d.nsh_pad = (nsh + 3) / 4 * 4;
       for (int q = nsh; q < d.nsh_pad; ++q) shells.push_back(-1);
-      ro += 3LL * d.npts_pad * d.nsh_pad;
+      // first local function and function count of every shell
+      for (int q = 0; q < d.nsh_pad; ++q) sfn.push_back(make_int2(0, 0));
+      for (int i = 0; i < m; ++i) {
+        int2& e = sfn[d.shell_off + (fmeta[meta0 + i].code & 0xffff)];
+        if (e.y == 0) e.x = i;
+        ++e.y;
+      }
+      if (dg->radial) ro += 3LL * d.npts_pad * d.nsh_pad;
+      // shells each 16-function row tile and each 32-function K block touch
+      auto span = [&](int lo, int len) {
+        const int hi = std::min(lo + len, m) - 1;
+        const int last = (fmeta[meta0 + std::max(m - 1, 0)].code & 0xffff);
+        if (lo > hi) return make_int2(last, 0);
+        const int a = fmeta[meta0 + lo].code & 0xffff, b = fmeta[meta0 + hi].code & 0xffff;
+        return make_int2(a, b - a + 1);
+      };
+      for (int t0 = 0; t0 < d.m_pad; t0 += 16) tsh.push_back(span(t0, 16));
+      for (int k0 = 0; k0 < d.m_pad; k0 += 32) kbsh.push_back(span(k0, 32));
     }
     for (int p = 0; !dpts && p < d.npts_pad; ++p) {
       const int src = part.pt_off[gk] + (p < n ? p : 0);  // padding repeats point 0 with w = 0
@@ -2520,6 +3100,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     if (bf) fo += (long long)d.m_pad * d.npts_pad;
   }
   dg->npts_pad = pt;
+  dg->phi_elems = fo;
   dg->val_elems = vo;
   dg->pg_elems = po;
   dg->nfock = fock_base;
@@ -2542,7 +3123,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     for (int k = 0; k < d.m_pad; ++k)
       for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalSpan) eitems.push_back(EvalItem{g, k, p0, 0});
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
-    if (bf)
+    if (bf && !dg->inkernel)
       for (int p0 = 0; p0 < d.npts_pad; p0 += kPmPts)
         for (int k0 = 0; k0 < d.m_pad; k0 += kPmFns) pmitems.push_back(EvalItem{g, k0, p0, 0});
     if (dg->radial)
@@ -2599,7 +3180,6 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->gitems = c.take<GatherItem>(gitems.size());
     dg->pmitems = c.take<EvalItem>(pmitems.size());
     dg->phiT = c.take<float>(fo);
-    dg->ybuf = c.take<float>(fo);
     dg->ditems = c.take<DensityItem>(ditems.size());
     dg->fitems = c.take<FockItem>(fitems.size());
     dg->gtiles = c.take<int2>(gtiles.size());
@@ -2636,6 +3216,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->fmeta = c.take<FnMeta>(fmeta.size() + 1);
     dg->rad = c.take<float>(ro);
     dg->ritems = c.take<GatherItem>(ritems.size());
+    dg->tsh = c.take<int2>(tsh.size() + 1);
+    dg->kbsh = c.take<int2>(kbsh.size() + 1);
+    dg->sfn = c.take<int2>(sfn.size() + 1);
   };
   Carver sizing{nullptr};
   plan(sizing);
@@ -2662,7 +3245,6 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     }
     if (!slab && dg->arena_bytes > free_b + reclaim) {
       const size_t need = dg->arena_bytes;
-      delete dg;
       throw std::runtime_error("g2g: grid needs " + std::to_string(need >> 20) +
                                " MiB of device memory, " + std::to_string(free_b >> 20) +
                                " MiB free");
@@ -2676,7 +3258,6 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       dg->arena = sl.take(dg->arena_bytes);
       if (!dg->arena) {
         const size_t need = dg->arena_bytes;
-        delete dg;
         throw std::runtime_error("g2g: device slab full: grid needs " +
                                  std::to_string(need >> 20) + " MiB (out of memory)");
       }
@@ -2690,6 +3271,12 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   Carver carve{dg->arena};
   plan(carve);
+  // the previous grid's Y buffer serves this one when large enough
+  if (old.g && old.g->ybuf && old.g->ybuf_cap >= (size_t)(dg->open ? 2 : 1) * fo * sizeof(float)) {
+    dg->ybuf = old.g->ybuf;
+    dg->ybuf_cap = old.g->ybuf_cap;
+    old.g->ybuf = nullptr;
+  }
   const double t_up = wall();
   if (old.g && old.g->h_rho && old.g->pinned_packed >= (size_t)(nspin * packed) &&
       old.g->pinned_mols >= nmol) {
@@ -2733,6 +3320,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->shells, shells.data(), shells.size() * sizeof(int));
   up(dg->fmeta, fmeta.data(), fmeta.size() * sizeof(FnMeta));
   up(dg->ritems, ritems.data(), ritems.size() * sizeof(GatherItem));
+  up(dg->tsh, tsh.data(), tsh.size() * sizeof(int2));
+  up(dg->kbsh, kbsh.data(), kbsh.size() * sizeof(int2));
+  up(dg->sfn, sfn.data(), sfn.size() * sizeof(int2));
   up(dg->afn_off, afn_off.data(), afn_off.size() * sizeof(int));
   up(dg->afn, afn.data(), afn.size() * sizeof(int));
   if (dpts) {  // points come from the device rebuild: compact them into place
@@ -2771,6 +3361,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       else
         hipLaunchKernelGGL((k_eval_basis<4, float>), grid, block, 0, s, dg->eitems, dg->groups,
                            dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, 0);
+    } else if (dg->inkernel) {  // phi alone, function-major, for the Fock kernel
+      hipLaunchKernelGGL((k_eval_basis<1, float>), grid, block, 0, s, dg->eitems, dg->groups,
+                         dg->fnl, dg->bas, dg->px, dg->py, dg->pz, dg->phiT, 1);
     } else {
       float* v = static_cast<float*>(dg->vals);
       if (dg->npm > 0) {
@@ -2793,6 +3386,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   XCQ_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
   dg->eval_ms = ms;
   grid_ = dg;
+  guard.g = nullptr;
   const double t_end = wall();
   load_ms_[1] = 1e3 * (t_up - t_plan);
   load_ms_[2] = 1e3 * (t_eval - t_up);
@@ -2803,6 +3397,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
                                               const std::vector<const StagedPartition*>& staged,
                                               const Options& opt, Precision prec) {
   XCQ_HIP(hipSetDevice(device_));
+  check_not_pending("a grid load");
   const double t_start = wall();
   const int nmol = (int)mols.size();
   if (nmol == 0 || staged.size() != mols.size())
@@ -3012,9 +3607,34 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
   unsigned short* pg3 = static_cast<unsigned short*>(spin == 2 ? g.pg2 : g.pg);
   const double* rho = spin == 2 ? d_rho + g.packed : d_rho;
   const float* vals = static_cast<const float*>(g.vals);
+  if (g.inkernel) {  // variants 6 (128-function row chunks) and 7 (64)
+    if (g.ngather > 0)
+      hipLaunchKernelGGL(k_gather_density_bf<4>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
+                         g.groups, g.mols, g.fnl, rho, pg3);
+    if (spin < 2) mark(ev, 1, s);
+    float* y = yb ? yb + (spin == 2 ? g.phi_elems : 0) : nullptr;
+    if (g.ndensity > 0) {
+      const dim3 grid(g.ndensity), block(256);
+#define XCQ_EV(NT, R)                                                                         \
+  hipLaunchKernelGGL((k_density_ev<NT, R>), grid, block, 0, s, g.ditems, g.groups, pg3, g.fmeta, \
+                     g.shells, g.sfn, g.kbsh, g.tsh, g.bas, g.px, g.py, g.pz, g.pw, g.factor, g.epart,  \
+                     g.iexch, spin, g.factor_b, g.sums, g.npts_pad, y)
+      if (g.gga && g.density_variant == 6)
+        XCQ_EV(4, 128);
+      else if (g.gga)
+        XCQ_EV(4, 64);
+      else if (g.density_variant == 6)
+        XCQ_EV(1, 128);
+      else
+        XCQ_EV(1, 64);
+#undef XCQ_EV
+    }
+    XCQ_HIP(hipGetLastError());
+    return;
+  }
   if (g.ngather > 0)
-    hipLaunchKernelGGL(k_gather_density_bf, dim3(g.ngather), dim3(256), 0, s, g.gitems, g.groups,
-                       g.mols, g.fnl, rho, pg3);
+    hipLaunchKernelGGL(k_gather_density_bf<3>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
+                       g.groups, g.mols, g.fnl, rho, pg3);
   if (spin < 2) mark(ev, 1, s);
   if (g.ndensity > 0) {
     if (g.density_variant == 1) {
@@ -3103,6 +3723,7 @@ static void launch_fock(DeviceGrid& g, hipStream_t s, const double* factor) {
 // Closed-shell split-BF16 builds whose density kernel is k_density_bf2
 // (variants 1, 2, 5) leave Y for the forces in ybuf.
 static bool ystore_forces(const DeviceGrid& g) {
+  if (g.inkernel) return g.ybuf != nullptr;  // both spins when open
   return g.prec == Precision::BF16X6 && !g.open && g.ybuf &&
          (g.density_variant == 1 || g.density_variant == 2 || g.density_variant == 5);
 }
@@ -3121,6 +3742,10 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
         hipLaunchKernelGGL(k_force<float>, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
                            static_cast<const float*>(pg), static_cast<const float*>(g.vals),
                            factor, g.gpart);
+      else if (g.inkernel)
+        hipLaunchKernelGGL(k_force_ev, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                           g.ybuf + (sp == 1 ? g.phi_elems : 0), g.fmeta, g.shells, g.tsh, g.bas,
+                           g.px, g.py, g.pz, factor, g.gpart);
       else if (ystore_forces(g))
         hipLaunchKernelGGL(k_force_y, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups, g.ybuf,
                            static_cast<const float*>(g.vals), factor, g.gpart);
@@ -3182,9 +3807,31 @@ static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double
   mark(ev, 4, s);
 }
 
+// The split-BF16 force passes read Y = P_g phi, which the density kernel
+// leaves in a buffer of its own (one FP32 value per padded point and function,
+// per spin): allocated at the first forces build of a grid, from the device
+// slab when one is reserved, else from the stream-ordered pool.
+static void ensure_ybuf(DeviceGrid& g, hipStream_t s, int device) {
+  if (g.prec != Precision::BF16X6 || g.ybuf) return;
+  const size_t bytes = (size_t)(g.open ? 2 : 1) * g.phi_elems * sizeof(float) + 256;
+  Slab& sl = g_slab[device];
+  void* p = nullptr;
+  if (sl.active()) {
+    p = sl.take(bytes);
+    if (!p)
+      throw std::runtime_error("g2g: device slab full: the force pass needs " +
+                               std::to_string(bytes >> 20) + " MiB (out of memory)");
+    g.ybuf_cap = bytes;
+  } else {
+    g.ybuf_cap = pool_alloc(&p, bytes, bytes, s, device);
+  }
+  g.ybuf = static_cast<float*>(p);
+}
+
 SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_energy,
                                  double* d_forces) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  if (d_forces) ensure_ybuf(*grid_, (hipStream_t)stream_, device_);
   launch_all(*grid_, (hipStream_t)stream_, d_rho, d_fock, reinterpret_cast<hipEvent_t*>(events_),
              d_forces);
   (void)want_energy;
@@ -3218,6 +3865,7 @@ void Engine::unload() {
   XCQ_HIP(hipSetDevice(device_));
   hipStream_t s = (hipStream_t)stream_;
   XCQ_HIP(hipStreamSynchronize(s));
+  pending_ = false;  // a queued build's result is dropped with the grid
   drop_graph();
   release(grid_, s, device_);
   grid_ = nullptr;
@@ -3276,8 +3924,15 @@ void Engine::kernel_times(double ms[4]) {
   }
 }
 
+void Engine::check_not_pending(const char* what) const {
+  if (pending_)
+    throw std::runtime_error(std::string("g2g: ") + what +
+                             " while an asynchronous build is queued: collect it first");
+}
+
 void Engine::solve_async(const double* rho_packed) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  check_not_pending("a second asynchronous build");
   if (grid_->open) throw std::runtime_error("g2g: asynchronous builds are closed shell only");
   DeviceGrid& g = *grid_;
   hipStream_t s = (hipStream_t)stream_;
@@ -3288,6 +3943,7 @@ void Engine::solve_async(const double* rho_packed) {
   solve_device(g.rho, g.F, true);
   XCQ_HIP(hipMemcpyAsync(g.h_F, g.F, packed * sizeof(double), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipMemcpyAsync(g.h_e, g.eout, 2 * g.nmol * sizeof(double), hipMemcpyDeviceToHost, s));
+  pending_ = true;
 }
 
 bool Engine::idle() const {
@@ -3299,8 +3955,10 @@ bool Engine::idle() const {
 
 SolveResult Engine::collect(double* fock_accum) {
   if (!grid_) throw std::runtime_error("g2g: no grid");
+  if (!pending_) throw std::runtime_error("g2g: collect without a queued asynchronous build");
   DeviceGrid& g = *grid_;
   XCQ_HIP(hipStreamSynchronize((hipStream_t)stream_));
+  pending_ = false;
   if (fock_accum)
     for (long long k = 0; k < g.packed; ++k) fock_accum[k] += g.h_F[k];
   SolveResult res;
@@ -3320,6 +3978,7 @@ SolveResult Engine::solve_open(const double* rho_a, const double* rho_b, double*
                                double* fock_b, bool want_energy, double* forces_accum,
                                int forces_ld) {
   if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  check_not_pending("a synchronous build");
   const double t0 = wall();
   DeviceGrid& g = *grid_;
   hipStream_t s = (hipStream_t)stream_;

@@ -346,7 +346,7 @@ StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
           unsigned c[3];
           for (int d = 0; d < 3; ++d) c[d] = (unsigned)std::floor((q[d] - lo[d]) / lcs);
           if (c[0] >= dim[0] || c[1] >= dim[1] || c[2] >= dim[2])
-            throw std::runtime_error("Se accedio a un cubo invalido");
+            throw std::runtime_error("grid point outside the cube lattice of the bounding prism");
           st = &cubes[((size_t)c[0] * dim[1] + c[1]) * dim[2] + c[2]];
         }
         st->x.push_back(q[0]);

@@ -1,11 +1,23 @@
 // Fortran-callable named timers and the inert analytic-integral symbols.
 //
-// Timers keep the reference's observable behaviour (g2g/timer.cpp:125-296):
-// with an odd timer level, g2g_timer_stop_/pause_ print "TIMER [name]: t";
-// with a level above 1 the sum timers nest under the currently running one
-// and g2g_timer_summary_ prints the tree with each child's share of its
-// parent.  The device is synchronised before every reading, as the
-// reference's Timer::sync does, so GPU work is attributed correctly.
+// Timers keep the reference's observable behaviour (g2g/timer.cpp:37-112,
+// 125-296):
+//  * timer level odd: g2g_timer_start_ starts a named timer, g2g_timer_stop_
+//    sets it to the interval since that start and prints
+//    "TIMER [name]: <t>", g2g_timer_pause_ adds the interval to what the timer
+//    holds and prints "TIMER [name]: <t>(so far)", where <t> is "<s>s. <us>us."
+//    or "<us>us." below one second;
+//  * timer level above 1: the sum timers nest under the one currently
+//    running (a name is registered under the first parent it runs in), stop
+//    sets and pause accumulates as above, and g2g_timer_summary_ prints the
+//    tree, sorted by name, with each timer's share of its parent ("Total"
+//    for the top level);
+//  * the levels only switch timers on, as in g2g_set_options_
+//    (g2g/init.cpp:427-432).
+// Names keep the exact length Fortran passes.  The device is synchronised
+// before every reading, as the reference's Timer::sync does, so GPU work is
+// attributed correctly.  Output is flushed at once so it interleaves with
+// the caller's own writes.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -20,21 +32,29 @@ namespace xcq {
 
 namespace {
 
+// Nanosecond clock with the reference Timer's start / stop / pause meaning.
 struct Clock {
-  double total = 0.0, started = 0.0;
-  bool running = false;
-  void start() {
-    started = now();
-    running = true;
-  }
-  void stop() {  // stop accumulates, like pause, and ends the interval
-    if (running) total += now() - started;
-    running = false;
-  }
-  static double now() {
+  long long res = 0, t0 = 0;
+  bool started = false;
+  static long long now() {
     using namespace std::chrono;
-    return duration<double>(steady_clock::now().time_since_epoch()).count();
+    return duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count();
   }
+  void start() {
+    started = true;
+    t0 = now();
+  }
+  void stop() {  // the interval since the last start
+    res = now() - t0;
+    started = false;
+  }
+  void pause() {  // adds the interval since the last start
+    res += now() - t0;
+    t0 = now();
+  }
+  unsigned long long sec() const { return (unsigned long long)(res / 1000000000LL); }
+  unsigned long long usec() const { return (unsigned long long)((res % 1000000000LL) / 1000); }
+  float seconds() const { return (float)sec() + (float)usec() / 1000000.0f; }
 };
 
 bool g_single = false, g_sum = false;
@@ -50,25 +70,26 @@ void sync_device() {
   if (hipGetDeviceCount(&n) == hipSuccess && n > 0) (void)hipDeviceSynchronize();
 }
 
-std::string name_of(const char* s, unsigned len) {
-  std::string n(s, len);
-  while (!n.empty() && n.back() == ' ') n.pop_back();
-  return n;
+void print_time(const Clock& c) {
+  if (c.sec() != 0)
+    std::printf("%llus. %lluus.", c.sec(), c.usec());
+  else
+    std::printf("%lluus.", c.usec());
 }
 
-void print_tree(const std::string& indent, const std::string& name, double parent_total,
+void print_tree(const std::string& indent, const std::string& name, float parent_total,
                 const std::string& parent) {
-  const double t = g_all[name].total;
+  const float t = g_all[name].seconds();
   std::printf("%s%-35s%12.6fs (%6.2f%% of %s)\n", indent.c_str(), name.c_str(), t,
-              parent_total > 0 ? 100.0 * t / parent_total : 0.0, parent.c_str());
+              100.0f * t / parent_total, parent.c_str());
   for (const auto& c : g_children[name]) print_tree(indent + "  ", c, t, name);
 }
 
 }  // namespace
 
 void timers_configure(unsigned level) {
-  g_sum = level > 1;
-  g_single = (level % 2) == 1;
+  if (level > 1) g_sum = true;
+  if (level % 2 == 1) g_single = true;
 }
 
 }  // namespace xcq
@@ -79,39 +100,43 @@ extern "C" {
 
 void g2g_timer_start_(const char* name, unsigned len) {
   if (!g_single) return;
+  const std::string n(name, len);
   sync_device();
-  g_single_timers[name_of(name, len)].start();
+  g_single_timers[n].start();
+}
+
+static void single_end(const char* name, unsigned len, bool pause) {
+  const std::string n(name, len);
+  sync_device();
+  if (!g_single_timers.count(n)) std::printf("g2g timer %s was never started\n", n.c_str());
+  Clock& c = g_single_timers[n];
+  if (pause)
+    c.pause();
+  else
+    c.stop();
+  std::printf("TIMER [%s]: ", n.c_str());
+  print_time(c);
+  std::printf(pause ? "(so far)\n" : "\n");
+  std::fflush(stdout);
 }
 
 void g2g_timer_stop_(const char* name, unsigned len) {
-  if (!g_single) return;
-  sync_device();
-  const std::string n = name_of(name, len);
-  if (!g_single_timers.count(n)) std::printf("no such timer (%s)\n", n.c_str());
-  Clock& c = g_single_timers[n];
-  c.stop();
-  std::printf("TIMER [%s]: %.6fs\n", n.c_str(), c.total);
-  c.total = 0.0;
+  if (g_single) single_end(name, len, false);
 }
 
 void g2g_timer_pause_(const char* name, unsigned len) {
-  if (!g_single) return;
-  sync_device();
-  const std::string n = name_of(name, len);
-  if (!g_single_timers.count(n)) std::printf("no such timer (%s)\n", n.c_str());
-  Clock& c = g_single_timers[n];
-  c.stop();
-  std::printf("TIMER [%s]: %.6fs(so far)\n", n.c_str(), c.total);
+  if (g_single) single_end(name, len, true);
 }
 
 void g2g_timer_sum_start_(const char* name, unsigned len) {
   if (!g_sum) return;
-  const std::string n = name_of(name, len);
+  const std::string n(name, len);
   g_children[n];
   if (g_current.empty()) {
     g_top.insert(n);
-  } else if (!g_children[g_current].count(n) && !g_parent.count(n)) {
+  } else if (!g_children[g_current].count(n)) {
     g_children[g_current].insert(n);
+    g_all[n] = Clock();  // registered anew under this parent
     g_parent[n] = g_current;
   }
   sync_device();
@@ -119,24 +144,31 @@ void g2g_timer_sum_start_(const char* name, unsigned len) {
   g_current = n;
 }
 
-static void sum_end(const char* name, unsigned len) {
-  const std::string n = name_of(name, len);
+static void sum_end(const char* name, unsigned len, bool pause) {
+  const std::string n(name, len);
   sync_device();
   if (g_current != n) {
-    std::printf("Error: not the current timer: (%s,%s)\n", n.c_str(), g_current.c_str());
+    if (pause)
+      std::printf("Error: not the current timer: (%s)\n", n.c_str());
+    else
+      std::printf("Error: not the current timer: (%s,%s)\n", n.c_str(), g_current.c_str());
+    std::fflush(stdout);
     return;
   }
-  g_all[n].stop();
+  if (pause)
+    g_all[n].pause();
+  else
+    g_all[n].stop();
   auto it = g_parent.find(n);
   g_current = (it == g_parent.end()) ? std::string() : it->second;
 }
 
 void g2g_timer_sum_stop_(const char* name, unsigned len) {
-  if (g_sum) sum_end(name, len);
+  if (g_sum) sum_end(name, len, false);
 }
 
 void g2g_timer_sum_pause_(const char* name, unsigned len) {
-  if (g_sum) sum_end(name, len);
+  if (g_sum) sum_end(name, len, true);
 }
 
 void g2g_timer_clear_(void) {
@@ -145,18 +177,20 @@ void g2g_timer_clear_(void) {
   g_children.clear();
   g_parent.clear();
   g_top.clear();
-  g_current.clear();
 }
 
 void g2g_timer_summary_(void) {
   if (!g_sum) return;
-  const double total = g_all.count("Total") ? g_all["Total"].total : 0.0;
-  std::printf("-------------------------------------------------------------------------------------------\n");
-  std::printf("                   LIO TIMING INFORMATION:\n");
-  std::printf("-------------------------------------------------------------------------------------------\n\n");
+  // a summary without a "Total" timer reports a zero total (the reference
+  // dereferences a missing entry there)
+  const float total = g_all.count("Total") ? g_all["Total"].seconds() : 0.0f;
+  const char* rule =
+      "-------------------------------------------------------------------------------------------";
+  std::printf("%s\n                   LIO TIMING INFORMATION:\n%s\n\n", rule, rule);
   std::printf("Total time: %gs\n", total);
   for (const auto& t : g_top) print_tree("", t, total, "Total");
-  std::printf("-------------------------------------------------------------------------------------------\n");
+  std::printf("%s\n", rule);
+  std::fflush(stdout);
 }
 
 // Analytic integrals are not part of this library: report GPU level 0 so
@@ -170,8 +204,13 @@ void aint_new_step_(void) {}
 void aint_qmmm_init_(const unsigned*, double*, double*) {}
 void aint_coulomb_init_(void) {}
 void aint_qmmm_forces_(double*, double*) {}
-void aint_qmmm_fock_(double*, double*) {}
+// The Fock terms are zeroed like the reference's (aint_init.cpp:267-269, 326-327)
+// and nothing is added: with GPU level 0 lioamber computes them on the CPU.
+void aint_qmmm_fock_(double* Es, double* Ens) {
+  *Es = 0.0;
+  *Ens = 0.0;
+}
 void aint_coulomb_forces_(double*) {}
-void aint_coulomb_fock_(double*) {}
+void aint_coulomb_fock_(double* Es) { *Es = 0.0; }
 
 }  // extern "C"

@@ -215,7 +215,10 @@ class Engine {
   static void reserve_slab(int device, size_t bytes);  // device -1: the current one
   // Queues a build from a packed host density (copied to pinned memory at
   // once) with its copy-back; idle() tells when it has finished and
-  // collect() waits for it and adds F onto fock_accum (may be null).
+  // collect() waits for it and adds F onto fock_accum (may be null).  One
+  // build may be queued at a time: a second solve_async, a synchronous solve
+  // or a grid (re)load before collect() throws, since they reuse the pinned
+  // buffers the queued build still reads and writes.
   void solve_async(const double* rho_packed);
   bool idle() const;
   SolveResult collect(double* fock_accum);
@@ -263,6 +266,8 @@ class Engine {
   void* stage_ = nullptr;     // device grid rebuild staging (kept, grow-only)
   size_t stage_cap_ = 0;
   double load_ms_[4] = {};
+  bool pending_ = false;      // an asynchronous build is queued and not yet collected
+  void check_not_pending(const char* what) const;
   Tuning tuning_;
   void drop_graph();
   void load_batch_impl(const std::vector<const Molecule*>& mols,

@@ -19,7 +19,7 @@
 //   LDA ..................... g2g/pointxc/calc_ldaCS.h:32-102
 //
 // Parity status: pinned against LIO's CPU path built from its sources
-// (tests/test_oracle_vs_reference.py, tests/golden/*.npz), against closed
+// (tests/test_oracle_golden.py, tests/golden/*.npz), against closed
 // forms for the functional and the basis, and against brute-force numpy
 // contractions on small groups (tests/test_oracle_math.py).
 #include <omp.h>

@@ -7,7 +7,11 @@ XC Fock (sparse: exact zeros are pairs of functions that share no group), the
 XC forces of computation type 2 (natom x 3) and the group inventory.  The inputs are regenerated from the recorded seeds; a
 checksum of the packed density guards against drift.
 
-    python tests/golden/make_golden.py          (takes about a minute)
+    python tests/golden/make_golden.py                 (every case)
+    python tests/golden/make_golden.py mol195_s107_pbe (only the named cases)
+
+The molecules of 58, 60 and 195 atoms are cases of the batched (3-60 atoms)
+and served (10-200 atoms) workloads at their largest sizes.
 """
 import os
 import sys
@@ -22,7 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 from lio_amd import systems  # noqa: E402
 
 # name -> (builder, iexch)
-MOLECULES = [(3, 101), (10, 102), (25, 103), (45, 104)]
+MOLECULES = [(3, 101), (10, 102), (25, 103), (45, 104), (58, 105), (60, 106), (195, 107)]
 
 
 def cases():
@@ -47,10 +51,12 @@ OPEN_CASES = [("c0_pbe_open2", lambda: systems.open_shell(systems.config0())),
 GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsum", "fhash")
 
 
-def main():
+def main(only=()):
     from pyxc import RefG2G
     ref = RefG2G()
     for name, builder in cases():
+        if only and name not in only:
+            continue
         s = builder()
         ref.setup(s)
         E, F, t = ref.solve()
@@ -71,6 +77,8 @@ def main():
         print("%-18s M=%4d groups=%4d Exc=% .12f N=%.9f nnz(F)=%d max|f|=%.6f" %
               (name, s.M, len(groups), E, N, len(nz), np.abs(forces).max()), flush=True)
     for name, builder in OPEN_CASES:
+        if only and name not in only:
+            continue
         s = builder()
         ref.setup(s)
         E, Fa, Fb, _, t = ref.solve_open(0)
@@ -94,4 +102,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))
