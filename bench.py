@@ -60,13 +60,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(system, max_seconds: float = 25.0):
-    """Times full Fock builds of the same system on the host CPU (bounded)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyxc
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    kind = "reference" if os.path.exists(pyxc.REF_SO) else "port"
-    checker = pyxc.RefG2G() if kind == "reference" else pyxc.Oracle()
+def _time_cpu(checker, system, max_seconds):
     t_grid = checker.setup(system)
     times = []
     t_start = time.time()
@@ -75,13 +69,38 @@ def cpu_baseline(system, max_seconds: float = 25.0):
         times.append(t)
         if time.time() - t_start + t > max_seconds or len(times) >= 5:
             break
-    if kind == "reference":
-        checker.teardown()
     times.sort()
-    med = times[len(times) // 2]
-    return {"value": 1.0 / med, "unit": "Fock builds/s", "cores": threads, "kind": kind,
-            "sample": "%d full XC Fock builds of the same system after a %.1f s grid build "
-                      "(median %.3f s per build, OMP_NUM_THREADS=%d)" % (len(times), t_grid, med, threads)}
+    return t_grid, times[len(times) // 2], len(times)
+
+
+def cpu_baseline(system, max_seconds: float = 15.0):
+    """Times full Fock builds of the same system on the host CPU (bounded):
+    LIO's own CPU g2g path (oracle/_ref/libg2g_ref.so, built from LIO's
+    sources where they are present and shipped with the tree) when it is
+    there, and this repository's FP64 restatement of it beside it, so the
+    line says which one ``value`` is and how the two compare on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyxc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"unit": "Fock builds/s", "cores": threads}
+    port_grid, port_med, port_n = _time_cpu(pyxc.Oracle(), system, max_seconds)
+    out["port_value"] = 1.0 / port_med
+    if os.path.exists(pyxc.REF_SO):
+        ref = pyxc.RefG2G()
+        grid, med, n = _time_cpu(ref, system, max_seconds)
+        ref.teardown()
+        out.update(value=1.0 / med, kind="reference",
+                   sample="%d full XC Fock builds of LIO's CPU g2g (oracle/_ref) after a %.1f s grid "
+                          "build (median %.3f s per build, OMP_NUM_THREADS=%d); the FP64 "
+                          "restatement (oracle/liboracle.so): %d builds, median %.3f s"
+                          % (n, grid, med, threads, port_n, port_med),
+                   port_over_reference_time=port_med / med)
+    else:
+        out.update(value=1.0 / port_med, kind="port",
+                   sample="%d full XC Fock builds of the FP64 restatement (oracle/liboracle.so; "
+                          "LIO's own build absent) after a %.1f s grid build (median %.3f s per "
+                          "build, OMP_NUM_THREADS=%d)" % (port_n, port_grid, port_med, threads))
+    return out
 
 
 def run_mode(s, precision, steps, warmup, latency_samples):
@@ -150,6 +169,36 @@ def run_mode(s, precision, steps, warmup, latency_samples):
                 dens_bytes=dens_bytes)
 
 
+def bench_abi(s, steps=20, warmup=3):
+    """The drop-in path itself: g2g_solve_groups_(0) through the Fortran ABI
+    on the same system, the density in the caller's host array and the Fock
+    matrix accumulated into the caller's host array (copy of rho into pinned
+    memory, host-to-device copy, the build, device-to-host copy of F, host
+    +=).  Wall time per call, both modes (LIO_XC_PRECISION selects the mode
+    at g2g_init_)."""
+    import numpy as np
+    from lio_amd import g2g
+    out = {"what": "g2g_solve_groups_(0) with host rho and host F (copies and host += "
+                   "included), wall time per call"}
+    for name, prec in (("fp64", "fp64"), ("mixed", MIXED)):
+        os.environ["LIO_XC_PRECISION"] = prec
+        run = g2g.G2G()
+        run.setup(s)
+        for _ in range(warmup):
+            run.solve(0)
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            run.solve(0)
+            ts.append(time.perf_counter() - t0)
+        run.deinit()
+        ts = np.sort(np.array(ts)) * 1e3
+        out[name] = {"builds_per_s": 1e3 / float(np.mean(ts)), "ms_p50": float(np.percentile(ts, 50)),
+                     "ms_p99": float(np.percentile(ts, 99))}
+    os.environ.pop("LIO_XC_PRECISION", None)
+    return out
+
+
 def bench_batch(precision, nmol=64, seed=3, reps=5):
     """One build of nmol molecules per graph replay; returns timings."""
     import numpy as np
@@ -172,13 +221,20 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     for _ in range(reps):
         b.replay()
     end.record(stream)
-    E, N = b.energies()
+    b.energies()
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / reps
+    # one more build on a zeroed F for the per-molecule results
+    F.zero_()
+    torch.cuda.synchronize()
+    b.solve_device(rho.data_ptr(), F.data_ptr())
+    E, N = b.energies()
+    torch.cuda.synchronize()
     out = {"molecules": nmol, "atoms": int(sum(m.natom for m in mols)),
            "functions": int(sum(m.M for m in mols)), "ms_per_batch_build": ms,
            "molecules_per_s": nmol / (ms * 1e-3), "grid_build_s": t_build,
-           "arena_GB": b.arena_bytes() / 1e9, "energies": E.tolist()}
+           "arena_GB": b.arena_bytes() / 1e9, "energies": E, "electrons": N,
+           "fock": F.cpu().numpy(), "offsets": b.offsets.copy()}
     b.close()
     del rho, F
     torch.cuda.empty_cache()
@@ -259,6 +315,7 @@ def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
     saturated rate: p50/p99 of a loaded but stable server).  Grid arenas come
     from one device slab sized to the free memory (no allocation per
     request); admission keeps the resident replicas within it."""
+    import numpy as np
     import torch
     from lio_amd import g2g, serve
     t0 = time.perf_counter()
@@ -272,10 +329,13 @@ def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
            "hbm_budget_gb": budget / 1e9, "requests": nreq}
     keys = ("builds_per_s", "latency_ms", "service_ms", "phase_ms_p50", "load_phase_ms_p50",
             "evictions", "retries")
+    served = {}
     for name, prec in (("fp64", "fp64"), ("mixed", MIXED)):
         srv = serve.Server(reps, precision=prec, slots=slots, hbm_budget=budget,
                            grid_mode="device", slab=True)
-        sat = srv.run(serve.poisson_arrivals(nreq, 1e4, nrep, seed=seed), max_seconds=300)
+        sat = srv.run(serve.poisson_arrivals(nreq, 1e4, nrep, seed=seed), keep_fock=True,
+                      max_seconds=300)
+        served[name] = {(q.replica, q.step): q for q in sat["requests"]}
         rate = 0.6 * sat["builds_per_s"]
         paced = srv.run(serve.poisson_arrivals(nreq, rate, nrep, seed=seed + 1), max_seconds=300)
         srv.close()
@@ -285,6 +345,24 @@ def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
             name, sat["builds_per_s"], paced["latency_ms"]["p50"], paced["latency_ms"]["p99"]))
     out["speedup_saturated"] = (out["mixed"]["saturated"]["builds_per_s"]
                                 / out["fp64"]["saturated"]["builds_per_s"])
+    # the saturated runs of both modes serve the same (replica, step) requests
+    # (same arrivals, fresh servers): the mixed result of every request
+    # against the FP64 one
+    dE, dF, dN = [], [], []
+    for key, q in sorted(served["mixed"].items()):
+        r = served["fp64"][key]
+        dE.append(q.energy - r.energy)
+        dN.append(q.electrons - r.electrons)
+        dF.append(float(np.max(np.abs(q.fock - r.fock))))
+    dE, dF, dN = np.abs(dE), np.array(dF), np.abs(dN)
+    ok = (dE <= 1e-6) & (dF <= 1e-6) & (dN <= 1e-6)
+    out["mixed_vs_fp64_per_request"] = {
+        "requests": len(dE), "within_1e-6": int(ok.sum()),
+        "max_abs_dExc": float(dE.max()), "max_abs_dF": float(dF.max()),
+        "max_abs_dN": float(dN.max()),
+        "abs_dExc": [float("%.3g" % v) for v in dE], "max_abs_dF_each": [float("%.3g" % v) for v in dF],
+        "abs_dN": [float("%.3g" % v) for v in dN],
+        "atoms": [int(reps[k[0]].base.natom) for k in sorted(served["mixed"])]}
     return out
 
 
@@ -296,6 +374,8 @@ def main():
     ap.add_argument("--latency-samples", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-abi", action="store_true",
+                    help="skip timing g2g_solve_groups_ with host arrays")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-next-rows", action="store_true",
                     help="skip the forces and open-shell lines")
@@ -338,8 +418,10 @@ def main():
     fp64_density = 0.8 * r64["info"]["flops"] / (r64["phases"][1] * 1e-3) / 1e12
 
     result = {
-        "metric": "XC Fock builds/s, mixed (split-BF16) vs FP64 GPU at |dExc|<=1e-6 Ha "
-                  "(drop-in g2g_solve_groups_)",
+        "metric": "Fock builds/s & grid pts/s, mixed vs FP64 GPU at |dExc|<=1e-6 Ha; p99 latency",
+        "value_what": "mixed-precision XC Fock builds/s of the build behind g2g_solve_groups_ with "
+                      "the packed density already in HBM and the packed Fock left in HBM; the "
+                      "ABI call with the caller's host arrays is timed under 'abi'",
         "value": 1000.0 / rmx["ms"],
         "unit": "Fock builds/s",
         "n_gpus": 1,
@@ -382,6 +464,10 @@ def main():
                      "mfma_tflops_executed": mfma_tflops,
                      "mfma_frac": mfma_tflops / PEAK_BF16_TFLOPS},
     }
+    if not args.no_abi:
+        log("g2g_solve_groups_ through the ABI ...")
+        result["abi"] = bench_abi(s)
+        log("abi:", json.dumps(result["abi"]))
     if not args.no_next_rows:
         log("forces and open shell ...")
         result["next_rows"] = bench_next_rows(s)
@@ -389,10 +475,24 @@ def main():
     if not args.no_batch:
         log("batch of 64 molecules ...")
         bb = {p: bench_batch(p) for p in ("fp64", MIXED)}
-        e64, emx = np.array(bb["fp64"].pop("energies")), np.array(bb[MIXED].pop("energies"))
-        result["batch"] = {"fp64": bb["fp64"], "mixed": bb[MIXED],
-                           "speedup": bb[MIXED]["molecules_per_s"] / bb["fp64"]["molecules_per_s"],
-                           "max_dExc_vs_fp64": float(np.max(np.abs(emx - e64)))}
+        a, m = bb["fp64"], bb[MIXED]
+        off = a.pop("offsets")
+        m.pop("offsets")
+        dE = np.abs(m.pop("energies") - a.pop("energies"))
+        dN = np.abs(m.pop("electrons") - a.pop("electrons"))
+        Fa, Fm = a.pop("fock"), m.pop("fock")
+        dF = np.array([np.max(np.abs(Fm[off[k]:off[k + 1]] - Fa[off[k]:off[k + 1]]))
+                       for k in range(len(off) - 1)])
+        ok = (dE <= 1e-6) & (dF <= 1e-6) & (dN <= 1e-6)
+        result["batch"] = {"fp64": a, "mixed": m,
+                           "speedup": m["molecules_per_s"] / a["molecules_per_s"],
+                           "mixed_vs_fp64_per_molecule": {
+                               "molecules": len(dE), "within_1e-6": int(ok.sum()),
+                               "max_abs_dExc": float(dE.max()), "max_abs_dF": float(dF.max()),
+                               "max_abs_dN": float(dN.max()),
+                               "abs_dExc": [float("%.3g" % v) for v in dE],
+                               "max_abs_dF_each": [float("%.3g" % v) for v in dF],
+                               "abs_dN": [float("%.3g" % v) for v in dN]}}
         log("batch:", json.dumps(result["batch"]))
     if not args.no_serve:
         log("serving replicas ...")

@@ -22,7 +22,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libg2g.so")
+# LIO_AMD_LIBRARY names another build of the library (e.g. the host code
+# built with AddressSanitizer by `make -C lio_amd/csrc asan`)
+LIB_PATH = os.environ.get("LIO_AMD_LIBRARY") or os.path.join(HERE, "libg2g.so")
 
 _dp = C.POINTER(C.c_double)
 _up = C.POINTER(C.c_uint)
