@@ -102,13 +102,17 @@ int xcq_set_precision(xcq_context* ctx, int precision);
 
 /* Builds grid and partition for grid_type (0/1/2) and uploads them. */
 int xcq_build_grid(xcq_context* ctx, unsigned grid_type);
-/* Where the group-local Becke weights and the zero-weight removal run at the
- * next xcq_build_grid: XCQ_GRID_HOST (default; OpenMP, as g2g/cpu/weight.cpp)
- * or XCQ_GRID_DEVICE (HIP kernels; points generated, assigned to cubes and
- * spheres and screened on the host, weighted and compacted on the GPU).  The
- * groups and weights are the same.  No reference counterpart (LIO's GPU
- * build weights points on the GPU unconditionally). */
-enum { XCQ_GRID_HOST = 0, XCQ_GRID_DEVICE = 1 };
+/* Where the next xcq_build_grid builds the grid and partition:
+ * XCQ_GRID_HOST (default; OpenMP, as g2g/regenerate_partition.cpp and
+ * g2g/cpu/weight.cpp), XCQ_GRID_DEVICE (all on the GPU: points, cube and
+ * sphere groups, significant functions, group-local Becke weights, zero-weight
+ * removal and compaction; the host computes only the bounding prism, the
+ * radial nodes and the shell thresholds) or XCQ_GRID_STAGED (points, groups
+ * and functions on the host, weights and compaction on the GPU).  The groups,
+ * points and weights are the same in every mode.  No reference counterpart
+ * (LIO's GPU build generates and groups points on the host and weighs them on
+ * the GPU). */
+enum { XCQ_GRID_HOST = 0, XCQ_GRID_DEVICE = 1, XCQ_GRID_STAGED = 2 };
 int xcq_set_grid_mode(xcq_context* ctx, int mode);
 /* Frees the context's device memory (grid, values, work buffers); the
  * problem stays set and the next xcq_build_grid restores it.  No reference

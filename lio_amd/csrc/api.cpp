@@ -33,7 +33,7 @@ struct xcq_context {
   double flops = 0.0;
   int density_tiles = 0, fock_items = 0;
   xcq::Precision prec = xcq::Precision::FP64;
-  bool device_grid = false;  // Becke weights and zero-weight removal on the device
+  int grid_mode = XCQ_GRID_HOST;  // where the grid and partition are built
   bool points_on_host = true;
 };
 
@@ -123,7 +123,12 @@ void build(xcq_context* ctx, unsigned grid_type) {
   if (!ctx->have_problem) throw std::runtime_error("g2g: grid requested before parameter init");
   check_problem(ctx->pv);
   ctx->mol = xcq::capture(ctx->pv, grid_type);
-  if (ctx->device_grid) {
+  if (ctx->grid_mode == XCQ_GRID_DEVICE) {
+    const double t0 = wall();
+    ctx->part = ctx->engine->load_generated({&ctx->mol}, ctx->opt, ctx->prec)[0];
+    ctx->upload_seconds = wall() - t0 - ctx->part.seconds;
+    ctx->points_on_host = false;
+  } else if (ctx->grid_mode == XCQ_GRID_STAGED) {
     const xcq::StagedPartition st = xcq::stage_partition(ctx->mol, ctx->opt);
     const double t0 = wall();
     ctx->part = ctx->engine->load_staged({&ctx->mol}, {&st}, ctx->opt, ctx->prec)[0];
@@ -224,9 +229,9 @@ int xcq_release_device(xcq_context* ctx) {
 
 int xcq_set_grid_mode(xcq_context* ctx, int mode) {
   return guarded([&] {
-    if (mode != XCQ_GRID_HOST && mode != XCQ_GRID_DEVICE)
+    if (mode != XCQ_GRID_HOST && mode != XCQ_GRID_DEVICE && mode != XCQ_GRID_STAGED)
       throw std::runtime_error("g2g: unknown grid mode");
-    ctx->device_grid = mode == XCQ_GRID_DEVICE;
+    ctx->grid_mode = mode;
   });
 }
 
@@ -381,9 +386,9 @@ struct xcq_batch {
   std::unique_ptr<xcq::Engine> engine;
   std::vector<xcq::Molecule> mols;
   std::vector<xcq::GridPartition> parts;
-  std::vector<xcq::StagedPartition> staged;  // device-grid mode
-  xcq::Options opt;                           // device-grid mode
-  bool device_grid = false;
+  std::vector<xcq::StagedPartition> staged;  // staged-grid mode
+  xcq::Options opt;                           // device and staged grid modes
+  int grid_mode = XCQ_GRID_HOST;
   xcq::Precision prec = xcq::Precision::FP64;
 };
 
@@ -392,7 +397,11 @@ namespace {
 void load_batch(xcq_batch* b) {
   std::vector<const xcq::Molecule*> m;
   for (size_t k = 0; k < b->mols.size(); ++k) m.push_back(&b->mols[k]);
-  if (b->device_grid) {
+  if (b->grid_mode == XCQ_GRID_DEVICE) {
+    b->parts = b->engine->load_generated(m, b->opt, b->prec);
+    return;
+  }
+  if (b->grid_mode == XCQ_GRID_STAGED) {
     std::vector<const xcq::StagedPartition*> st;
     for (size_t k = 0; k < b->staged.size(); ++k) st.push_back(&b->staged[k]);
     b->parts = b->engine->load_staged(m, st, b->opt, b->prec);
@@ -416,7 +425,8 @@ int xcq_batch_create_ex(int device, int n, const xcq_problem* problems,
                         const unsigned* grid_types, int precision, int grid_mode,
                         xcq_batch** out) {
   return guarded([&] {
-    if (grid_mode != XCQ_GRID_HOST && grid_mode != XCQ_GRID_DEVICE)
+    if (grid_mode != XCQ_GRID_HOST && grid_mode != XCQ_GRID_DEVICE &&
+        grid_mode != XCQ_GRID_STAGED)
       throw std::runtime_error("g2g: unknown grid mode");
     if (n <= 0 || !problems || !grid_types) throw std::runtime_error("g2g: empty batch");
     if (precision != XCQ_FP64 && precision != XCQ_FP32 && precision != XCQ_BF16X6)
@@ -426,9 +436,9 @@ int xcq_batch_create_ex(int device, int n, const xcq_problem* problems,
     b->engine.reset(new xcq::Engine(device));
     b->mols.resize(n);
     b->parts.resize(n);
-    b->device_grid = grid_mode == XCQ_GRID_DEVICE;
+    b->grid_mode = grid_mode;
     std::vector<xcq::StagedPartition>& staged = b->staged;
-    staged.resize(b->device_grid ? n : 0);
+    staged.resize(grid_mode == XCQ_GRID_STAGED ? n : 0);
     xcq::Options& opt0 = b->opt;
     for (int k = 0; k < n; ++k) {
       xcq::ProblemView pv;
@@ -437,11 +447,19 @@ int xcq_batch_create_ex(int device, int n, const xcq_problem* problems,
       check_problem(pv);
       if (k == 0) opt0 = opt;
       b->mols[k] = xcq::capture(pv, grid_types[k]);
-      if (grid_mode == XCQ_GRID_DEVICE) {
+      if (grid_mode == XCQ_GRID_STAGED) {
         if (opt.remove_zero_weights != opt0.remove_zero_weights ||
             opt.min_points_per_cube != opt0.min_points_per_cube)
-          throw std::runtime_error("g2g: a device-grid batch needs one weight-removal setting");
+          throw std::runtime_error("g2g: a staged-grid batch needs one weight-removal setting");
         staged[k] = xcq::stage_partition(b->mols[k], opt);
+      } else if (grid_mode == XCQ_GRID_DEVICE) {
+        if (opt.remove_zero_weights != opt0.remove_zero_weights ||
+            opt.min_points_per_cube != opt0.min_points_per_cube ||
+            opt.little_cube_size != opt0.little_cube_size ||
+            opt.sphere_radius != opt0.sphere_radius ||
+            opt.assign_all_functions != opt0.assign_all_functions ||
+            opt.max_function_exponent != opt0.max_function_exponent)
+          throw std::runtime_error("g2g: a device-grid batch needs one set of grid options");
       } else {
         b->parts[k] = xcq::build_partition(b->mols[k], opt);
       }
@@ -619,7 +637,9 @@ void g2g_init_(void) {
       g_ctx->engine.reset(new xcq::Engine(-1));
     }
     g_ctx->prec = precision_from_env();
-    if (const char* v = std::getenv("LIO_XC_DEVICE_GRID")) g_ctx->device_grid = v[0] == '1';
+    if (const char* v = std::getenv("LIO_XC_DEVICE_GRID"))
+      g_ctx->grid_mode = v[0] == '1' ? XCQ_GRID_DEVICE : v[0] == '2' ? XCQ_GRID_STAGED
+                                                                       : XCQ_GRID_HOST;
     g_ctx->opt = g_opt;
     if (g_opt.verbose > 2) std::printf("  Using HIP device %d for the XC quadrature.\n", g_ctx->engine->device());
   });

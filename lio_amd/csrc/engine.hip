@@ -667,7 +667,9 @@ __device__ inline void load8(const float* p, float (&x)[8]) {
 // per plane in one 16-byte store.  The tile's phi values also go to the
 // function-major phi plane the Fock kernel reads (rows of 64 points).
 constexpr int kPmPts = 64, kPmFns = 16;
-template <int NV>
+// LO: one more plane after the NV planes holds phi's remainder,
+// (float)(phi - (double)(float)phi), so hi + lo carries phi to about 2^-48.
+template <int NV, bool LO = false>
 __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
                                                  const int* __restrict__ fnl,
@@ -677,7 +679,8 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
                                                  const double* __restrict__ pz,
                                                  float* __restrict__ vals,
                                                  float* __restrict__ phiT) {
-  __shared__ float sT[NV][kPmFns][kPmPts + 1];
+  constexpr int NP = NV + (LO ? 1 : 0);
+  __shared__ float sT[NP][kPmFns][kPmPts + 1];
   const EvalItem it = items[blockIdx.x];
   const DevGroup g = groups[it.group];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -693,6 +696,7 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
       for (int c = 0; c < NV; ++c) v[c] = 0.0;
     }
     for (int c = 0; c < NV; ++c) sT[c][kk][lane] = (float)v[c];
+    if (LO) sT[NP - 1][kk][lane] = (float)(v[0] - (double)(float)v[0]);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kPmFns * kPmPts; e += 256) {
@@ -705,7 +709,7 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
   if (pp >= g.npts_pad) return;
   const long long plane = (long long)g.m_pad * g.npts_pad;
   float* out = vals + g.val_off + (long long)pp * g.m_pad + it.k + wf;
-  for (int c = 0; c < NV; ++c)
+  for (int c = 0; c < NP; ++c)
     *reinterpret_cast<fl4*>(out + c * plane) =
         fl4{sT[c][wf][wp], sT[c][wf + 1][wp], sT[c][wf + 2][wp], sT[c][wf + 3][wp]};
 }
@@ -960,7 +964,17 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
 // second derivatives are rebuilt from the shells' radial sums (FP32) and the
 // functions' angular factors in the epilogue, which then reads 4 planes
 // instead of 10.
-template <int NT, int ROWS, bool RAD = false>
+__device__ inline void split4f(const float (&hi)[8], const float (&lo)[8], bf8 (&pc)[4]);
+__device__ inline void mma10(const bf8 (&a)[4], const bf8 (&b)[4], f4& hi, f4& lo);
+
+// LO (density variant 8): Y_0 = P_g phi from phi's FP32 pair (plane 0 and
+// plane lopl, hi + lo) split into four BF16 pieces against four-piece P_g
+// (the ten piece products of weight >= 2^-24, mma10), and the epilogue's phi
+// is hi + lo in FP64.  Single-precision phi leaves a rounding error that does
+// not average out over the grid (every point of a radial shell sees the
+// same value of its atom's s functions), which is what bounds the electron
+// count of the three-piece form; Y_x, Y_y, Y_z keep the six-product chain.
+template <int NT, int ROWS, bool RAD = false, bool LO = false>
 __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const DensityItem* __restrict__ items,
                                                      const DevGroup* __restrict__ groups,
                                                      const unsigned short* __restrict__ pg3,
@@ -975,10 +989,11 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
                                                      const double* __restrict__ pz,
                                                      const FnMeta* __restrict__ fmeta,
                                                      const float* __restrict__ rad,
-                                                     float* __restrict__ ybuf) {
+                                                     float* __restrict__ ybuf, int lopl) {
   constexpr int NS = (NT == 4) ? 10 : 1;
   constexpr int RT = ROWS / 16;
-  constexpr int UNITS = 3 * ROWS * 4;      // 16-byte units of one staged tile
+  constexpr int NPC = LO ? 4 : 3;            // P_g pieces
+  constexpr int UNITS = NPC * ROWS * 4;      // 16-byte units of one staged tile
   __shared__ u4 sA[2][UNITS];              // [buf][piece][row][k octet]
   __shared__ double sE[4][2];
   const DensityItem it = items[blockIdx.x];
@@ -992,7 +1007,8 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
   const long long psz = (long long)mp * mp;
   const int pt = active ? p0 + lrow : it.p0;  // inactive waves still stage tiles
   const float* Vp = vals + g.val_off + (long long)pt * mp;  // this lane's point
-  const unsigned short* Pp = pg3 + 3 * g.pg_off;
+  const float* Vl = Vp + (long long)lopl * plane;            // phi's remainder (LO)
+  const unsigned short* Pp = pg3 + NPC * g.pg_off;
   double ppx = 0.0, ppy = 0.0, ppz = 0.0;
   if constexpr (RAD) {
     ppx = px[g.pt_off + pt];
@@ -1034,8 +1050,14 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       if (more) dma(c0, k0 + 32, buf ^ 1);
       const int kk = k0 + 8 * lk;
       bf8 bh[NT], bm[NT], bl[NT];
+      bf8 b4[LO ? 4 : 1];
+      if constexpr (LO) {  // phi's remainder of this step (not loaded ahead: registers)
+        float xl[8];
+        load8(Vl + kk, xl);
+        split4f(xb[0], xl, b4);
+      }
 #pragma unroll
-      for (int t = 0; t < NT; ++t) split3(xb[t], bh[t], bm[t], bl[t]);
+      for (int t = LO ? 1 : 0; t < NT; ++t) split3(xb[t], bh[t], bm[t], bl[t]);
       if (more) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) load8(Vp + t * plane + kk + 32, xb[t]);
@@ -1048,8 +1070,15 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
             const bf8 ah = __builtin_bit_cast(bf8, sA[buf][u]);
             const bf8 am = __builtin_bit_cast(bf8, sA[buf][ROWS * 4 + u]);
             const bf8 al = __builtin_bit_cast(bf8, sA[buf][2 * ROWS * 4 + u]);
+            if constexpr (LO) {
+              const bf8 a4[4] = {ah, am, al, __builtin_bit_cast(bf8, sA[buf][3 * ROWS * 4 + u])};
+              f4 hi, lo;
+              mma10(a4, b4, hi, lo);
+              acc[rt][0] += hi + lo;
+            }
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[rt][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
+            for (int t = LO ? 1 : 0; t < NT; ++t)
+              acc[rt][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
           }
         }
       }
@@ -1091,12 +1120,14 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
 #pragma unroll
           for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
         }
+        fl4 vlo = fl4{0, 0, 0, 0};
+        if constexpr (LO) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
                        yz = acc[rt][3][i];
           const double v1 = v[1][i], v2 = v[2][i], v3 = v[3][i];
-          s[0] += (double)v[0][i] * y0;
+          s[0] += ((double)v[0][i] + (double)vlo[i]) * y0;
           s[1] += v1 * y0;
           s[2] += v2 * y0;
           s[3] += v3 * y0;
@@ -1109,8 +1140,11 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
         }
       } else {
         const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
+        fl4 vlo = fl4{0, 0, 0, 0};
+        if constexpr (LO) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) s[0] += (double)v0[i] * (double)acc[rt][0][i];
+        for (int i = 0; i < 4; ++i)
+          s[0] += ((double)v0[i] + (double)vlo[i]) * (double)acc[rt][0][i];
       }
     }
 #pragma unroll
@@ -2599,6 +2633,310 @@ __global__ __launch_bounds__(256) void k_compact(const CompactItem* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Device grid build: points, cube/sphere groups and significant functions
+// ---------------------------------------------------------------------------
+// The same points, groups and function sets as the host staging
+// (g2g/regenerate_partition.cpp:274-462, g2g/classify_functions.cpp:12-141,
+// g2g/partition.cpp:229-254), from the molecule's grid frame (prism, cube
+// lattice, sphere shells, radial nodes, shell thresholds; grid_frame on the
+// host, O(atoms^2 + shells)).  Every raw point (atom, radial shell, angular
+// node) gets the key of its group -- a cube of the prism's lattice, the
+// atom's sphere, or none when it lies outside the prism -- and a stable sort
+// by key puts each group's points together in generation order, the order
+// in which the host appends them.  The arithmetic that decides a point's
+// position, weight, cube and a shell's significance is the host's, operation
+// for operation (no contraction into fused multiply-adds), so every decision
+// is the same.
+
+constexpr int kGenMaxAtoms = 8192;  // atoms of one molecule (LDS flags of the screening)
+
+struct GenAtom {
+  double x, y, z;
+  long long raw_off;  // first raw point of the atom (batch-wide)
+  int mol, local;     // molecule, molecule-local index
+  int rad_off;        // first radial node (r1, wrad)
+  int sphere_from;    // first radial shell inside the atom's sphere
+};
+struct GenMol {
+  double lo[3], hi[3], lcs;
+  unsigned dim[3];
+  int ncubes, natom;
+  int bin_base;       // first group key of the molecule (cubes, then spheres)
+  int atom_base;      // first GenAtom / batch atom
+  int ang_off, ngrid; // angular rule
+  int sh_off, nsh;    // basis shells (shell tables)
+  long long slot_off; // shell slots: key b at slot_off + (b - bin_base) * nsh
+  long long aslot_off;  // atom slots: key b at aslot_off + (b - bin_base) * natom
+  long long rij_off;    // natom x natom distances
+};
+struct BinInfo {
+  int nsel, m, ns, np, nd, nnuc;
+  unsigned long long fhash;
+};
+
+__global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, int n, int v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Largest a with atoms[a].raw_off <= r.
+__device__ inline int gen_atom_of(const GenAtom* __restrict__ atoms, int n, long long r) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (atoms[mid].raw_off <= r)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Position and weight of raw point r (regenerate:274-300): the atom's centre
+// plus the angular node times the radial node, weight wrad * wang.
+__device__ inline void gen_point(const GenAtom& A, const GenMol& M, long long r,
+                                 const double* __restrict__ r1, const double* __restrict__ wrad,
+                                 const double* __restrict__ ex, const double* __restrict__ ey,
+                                 const double* __restrict__ ez, const double* __restrict__ ew,
+                                 double (&q)[3], double& w, int& sh) {
+#pragma clang fp contract(off)
+  const long long local = r - A.raw_off;
+  sh = (int)(local / M.ngrid);
+  const int k = (int)(local - (long long)sh * M.ngrid);
+  const double rr = r1[A.rad_off + sh];
+  q[0] = A.x + ex[M.ang_off + k] * rr;
+  q[1] = A.y + ey[M.ang_off + k] * rr;
+  q[2] = A.z + ez[M.ang_off + k] * rr;
+  w = wrad[A.rad_off + sh] * ew[M.ang_off + k];
+}
+
+// Group key of every raw point (no_group, past the last key, for a point
+// outside the prism); value = the raw index.  err gets 1 when a point inside
+// the prism falls outside its cube lattice (the host throws).
+__global__ __launch_bounds__(256) void k_gen_keys(const GenAtom* __restrict__ atoms, int natoms,
+                                                  const GenMol* __restrict__ mols,
+                                                  const double* __restrict__ r1,
+                                                  const double* __restrict__ wrad,
+                                                  const double* __restrict__ ex,
+                                                  const double* __restrict__ ey,
+                                                  const double* __restrict__ ez,
+                                                  const double* __restrict__ ew, long long nraw,
+                                                  unsigned no_group, unsigned* __restrict__ keys,
+                                                  unsigned* __restrict__ vals,
+                                                  int* __restrict__ err) {
+#pragma clang fp contract(off)
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= nraw) return;
+  const GenAtom A = atoms[gen_atom_of(atoms, natoms, r)];
+  const GenMol& M = mols[A.mol];
+  double q[3], w;
+  int sh;
+  gen_point(A, M, r, r1, wrad, ex, ey, ez, ew, q, w, sh);
+  unsigned key = no_group;
+  if (M.lo[0] <= q[0] && q[0] <= M.hi[0] && M.lo[1] <= q[1] && q[1] <= M.hi[1] &&
+      M.lo[2] <= q[2] && q[2] <= M.hi[2]) {
+    if (sh >= A.sphere_from) {
+      key = (unsigned)(M.bin_base + M.ncubes + A.local);
+    } else {
+      unsigned c[3];
+      for (int d = 0; d < 3; ++d) c[d] = (unsigned)floor((q[d] - M.lo[d]) / M.lcs);
+      if (c[0] >= M.dim[0] || c[1] >= M.dim[1] || c[2] >= M.dim[2])
+        atomicOr(err, 1);
+      else
+        key = (unsigned)M.bin_base + (c[0] * M.dim[1] + c[1]) * M.dim[2] + c[2];
+    }
+  }
+  keys[r] = key;
+  vals[r] = (unsigned)r;
+}
+
+// Points per group key (integer counts: the order of the additions does not
+// matter).
+__global__ __launch_bounds__(256) void k_gen_count(const unsigned* __restrict__ keys, long long n,
+                                                   unsigned no_group, int* __restrict__ count) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const unsigned k = keys[i];
+  if (k < no_group) atomicAdd(&count[k], 1);
+}
+
+// Staged point arrays in sorted (group, generation) order: coordinates,
+// weight wrad * wang, owning atom (molecule-local).
+__global__ __launch_bounds__(256) void k_gen_gather(const GenAtom* __restrict__ atoms, int natoms,
+                                                    const GenMol* __restrict__ mols,
+                                                    const double* __restrict__ r1,
+                                                    const double* __restrict__ wrad,
+                                                    const double* __restrict__ ex,
+                                                    const double* __restrict__ ey,
+                                                    const double* __restrict__ ez,
+                                                    const double* __restrict__ ew,
+                                                    const unsigned* __restrict__ vals, long long n,
+                                                    double* __restrict__ sx, double* __restrict__ sy,
+                                                    double* __restrict__ sz, double* __restrict__ sw,
+                                                    int* __restrict__ owner) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const long long r = vals[i];
+  const GenAtom A = atoms[gen_atom_of(atoms, natoms, r)];
+  double q[3], w;
+  int sh;
+  gen_point(A, mols[A.mol], r, r1, wrad, ex, ey, ez, ew, q, w, sh);
+  sx[i] = q[0];
+  sy[i] = q[1];
+  sz[i] = q[2];
+  sw[i] = w;
+  owner[i] = A.local;
+}
+
+// One block per group key: the significant shells of the group (in shell
+// order: s, p, d) into its shell slots, the atoms owning them (ascending)
+// into its atom slots, and the counts and FNV-1a hash of the first function
+// indices.  The squared distance of an atom to the group is the host's: to
+// the cube's box, or to the sphere's surface (zero for the sphere's own
+// atom).
+__global__ __launch_bounds__(256) void k_gen_screen(const GenMol* __restrict__ mols,
+                                                    const int* __restrict__ bin_mol,
+                                                    const GenAtom* __restrict__ atoms,
+                                                    const double* __restrict__ srad,
+                                                    const double* __restrict__ rij,
+                                                    const int* __restrict__ sh_first,
+                                                    const int* __restrict__ sh_len,
+                                                    const int* __restrict__ sh_atom,
+                                                    const double* __restrict__ sh_a,
+                                                    const double* __restrict__ sh_thr,
+                                                    int assign_all, int* __restrict__ slots,
+                                                    int* __restrict__ aslots,
+                                                    BinInfo* __restrict__ info) {
+#pragma clang fp contract(off)
+  __shared__ unsigned char sown[kGenMaxAtoms];
+  __shared__ int swave[4];
+  const int key = blockIdx.x;
+  const GenMol M = mols[bin_mol[key]];
+  const int lb = key - M.bin_base;
+  const bool cube = lb < M.ncubes;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int a = threadIdx.x; a < M.natom; a += 256) sown[a] = 0;
+  double corner[3] = {0, 0, 0};
+  int sa = -1;
+  if (cube) {
+    const unsigned ci = (unsigned)lb / (M.dim[1] * M.dim[2]);
+    const unsigned cj = ((unsigned)lb / M.dim[2]) % M.dim[1];
+    const unsigned ck = (unsigned)lb % M.dim[2];
+    corner[0] = M.lo[0] + ci * M.lcs;
+    corner[1] = M.lo[1] + cj * M.lcs;
+    corner[2] = M.lo[2] + ck * M.lcs;
+  } else {
+    sa = lb - M.ncubes;
+  }
+  __syncthreads();
+  int* slot = slots + M.slot_off + (long long)lb * M.nsh;
+  int base = 0;
+  for (int c0 = 0; c0 < M.nsh; c0 += 256) {
+    const int k = c0 + threadIdx.x;
+    bool keep = false;
+    int len = 0;
+    if (k < M.nsh) {
+      const int s = M.sh_off + k;
+      const int at = sh_atom[s];
+      len = sh_len[s];
+      double d2;
+      if (cube) {
+        const GenAtom& A = atoms[M.atom_base + at];
+        const double p[3] = {A.x, A.y, A.z};
+        double v[3];
+        for (int d = 0; d < 3; ++d) {
+          if (p[d] < corner[d])
+            v[d] = corner[d] - p[d];
+          else if (p[d] > (corner[d] + M.lcs))
+            v[d] = p[d] - (corner[d] + M.lcs);
+          else
+            v[d] = 0;
+        }
+        d2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      } else if (at == sa) {
+        d2 = 0;
+      } else {
+        const double dab = rij[M.rij_off + (long long)at * M.natom + sa];
+        const double sr = srad[M.atom_base + sa];
+        const double gap = (sr > dab ? 0 : dab - sr);
+        d2 = gap * gap;
+      }
+      keep = assign_all || (len == 1 ? sh_a[s] * d2 < sh_thr[s] : sqrt(d2) < sh_thr[s]);
+      if (keep) sown[at] = 1;
+    }
+    const unsigned long long mask = __ballot(keep);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) swave[wave] = __popcll(mask);
+    __syncthreads();
+    int before = base;
+    for (int w = 0; w < wave; ++w) before += swave[w];
+    if (keep) slot[before + rank] = k;
+    base += swave[0] + swave[1] + swave[2] + swave[3];
+    __syncthreads();
+  }
+  // atoms owning a kept shell, ascending
+  int* aslot = aslots + M.aslot_off + (long long)lb * M.natom;
+  int abase = 0;
+  for (int a0 = 0; a0 < M.natom; a0 += 256) {
+    const int a = a0 + threadIdx.x;
+    const bool own = a < M.natom && sown[a];
+    const unsigned long long mask = __ballot(own);
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) swave[wave] = __popcll(mask);
+    __syncthreads();
+    int before = abase;
+    for (int w = 0; w < wave; ++w) before += swave[w];
+    if (own) aslot[before + rank] = a;
+    abase += swave[0] + swave[1] + swave[2] + swave[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    BinInfo bi;
+    bi.nsel = base;
+    bi.nnuc = abase;
+    bi.ns = bi.np = bi.nd = bi.m = 0;
+    unsigned long long h = 1469598103934665603ull;
+    for (int i = 0; i < base; ++i) {
+      const int s = M.sh_off + slot[i];
+      const int len = sh_len[s];
+      bi.m += len;
+      if (len == 1)
+        ++bi.ns;
+      else if (len == 3)
+        ++bi.np;
+      else
+        ++bi.nd;
+      const unsigned x = (unsigned)sh_first[s];
+      for (int b = 0; b < 4; ++b) {
+        h ^= (x >> (8 * b)) & 0xffu;
+        h *= 1099511628211ull;
+      }
+    }
+    bi.fhash = h;
+    info[key] = bi;
+  }
+}
+
+// Kept groups' shell and atom lists packed densely for the copy to the host.
+struct PackItem {
+  int key, nsel, nnuc, soff, aoff, pad_;
+};
+__global__ __launch_bounds__(256) void k_gen_pack(const PackItem* __restrict__ items,
+                                                  const GenMol* __restrict__ mols,
+                                                  const int* __restrict__ bin_mol,
+                                                  const int* __restrict__ slots,
+                                                  const int* __restrict__ aslots,
+                                                  int* __restrict__ pk, int* __restrict__ pa) {
+  const PackItem it = items[blockIdx.x];
+  const GenMol M = mols[bin_mol[it.key]];
+  const int lb = it.key - M.bin_base;
+  const int* slot = slots + M.slot_off + (long long)lb * M.nsh;
+  const int* aslot = aslots + M.aslot_off + (long long)lb * M.natom;
+  for (int i = threadIdx.x; i < it.nsel; i += 256) pk[it.soff + i] = slot[i];
+  for (int i = threadIdx.x; i < it.nnuc; i += 256) pa[it.aoff + i] = aslot[i];
+}
+
 double wall() {
   using namespace std::chrono;
   return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -2669,7 +3007,8 @@ struct DeviceGrid {
   int* afn = nullptr;
   double* fout = nullptr;         // forces, natom x 3 (atom-major)
   bool radial = false;            // radial-Hessian layout (split-BF16 variant 5)
-  bool inkernel = false;          // basis evaluated in the density kernel (variant 6)
+  bool inkernel = false;          // basis evaluated in the density kernel (variants 6, 7)
+  bool lo = false;                // phi's FP32 remainder plane, four-piece Y_0 (variant 8)
   int2* kbsh = nullptr;           // per 32-function K block: first shell, shell count
   int2* sfn = nullptr;            // per group shell: first local function, function count
   int2* tsh = nullptr;            // per 16-function row tile: first shell, shell count
@@ -2893,7 +3232,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->fock_slice = tuning_.fock_slice;
   dg->density_variant = tuning_.density_variant;
 
-  if (dg->density_variant < 0 || dg->density_variant > 7)
+  if (dg->density_variant < 0 || dg->density_variant > 8)
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
@@ -2907,23 +3246,25 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   if (dg->open && prec == Precision::BF16X6 && dg->density_variant != 1 &&
       dg->density_variant != 2 && dg->density_variant < 5) {
-    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2, 5, 6 or 7");
+    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2, 5, 6, 7 or 8");
   }
   const int nspin = dg->open ? 2 : 1;
   // LDA keeps the gradient too: the XC forces need it.  The radial-Hessian
   // layout keeps phi and its gradient as planes and rebuilds the second
   // derivatives from per-shell radial sums.
   dg->radial = prec == Precision::BF16X6 && tuning_.density_variant == 5;
-  dg->inkernel = prec == Precision::BF16X6 && tuning_.density_variant >= 6;
+  dg->inkernel = prec == Precision::BF16X6 &&
+                 (tuning_.density_variant == 6 || tuning_.density_variant == 7);
+  dg->lo = prec == Precision::BF16X6 && tuning_.density_variant == 8;
   // in-kernel evaluation keeps no value planes (phi alone, function-major,
-  // for the Fock kernel)
-  dg->nval = dg->inkernel ? 0 : (dg->gga && !dg->radial) ? 10 : 4;
+  // for the Fock kernel); variant 8 one more plane (phi's remainder)
+  dg->nval = dg->inkernel ? 0 : ((dg->gga && !dg->radial) ? 10 : 4) + (dg->lo ? 1 : 0);
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const bool bf = prec == Precision::BF16X6;
   // split-BF16 K steps are 32 functions; its P_g holds three 2-byte pieces
   const int fn_align = bf ? 32 : kFnAlign;
-  const size_t pbytes = bf ? (dg->inkernel ? 4 : 3) * sizeof(unsigned short) : vbytes;
+  const size_t pbytes = bf ? (dg->inkernel || dg->lo ? 4 : 3) * sizeof(unsigned short) : vbytes;
 
   // batch numbering: molecule k owns functions [fn_base, fn_base + M) and
   // packed entries [rho_off, rho_off + M (M + 1) / 2)
@@ -3367,7 +3708,13 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     } else {
       float* v = static_cast<float*>(dg->vals);
       if (dg->npm > 0) {
-        if (dg->gga && !dg->radial)
+        if (dg->lo && dg->gga)
+          hipLaunchKernelGGL((k_eval_pm<10, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
+                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
+        else if (dg->lo)
+          hipLaunchKernelGGL((k_eval_pm<4, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
+                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
+        else if (dg->gga && !dg->radial)
           hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
         else
@@ -3393,116 +3740,101 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   load_ms_[3] = 1e3 * (t_end - t_eval);
 }
 
-std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*>& mols,
-                                              const std::vector<const StagedPartition*>& staged,
-                                              const Options& opt, Precision prec) {
-  XCQ_HIP(hipSetDevice(device_));
-  check_not_pending("a grid load");
-  const double t_start = wall();
-  const int nmol = (int)mols.size();
-  if (nmol == 0 || staged.size() != mols.size())
-    throw std::runtime_error("g2g: a staged batch needs one staging per molecule");
-  // batch-wide staging tables
+// Host side of a staged batch: the staged groups (points on the device), the
+// nuclei of each group, the Becke blocks and the atom tables, and per
+// molecule the staged groups' metadata (kind, functions, counts, hash).
+struct StageHost {
   std::vector<StGroup> sg;
-  std::vector<StMol> sm(nmol);
-  std::vector<int> owner, gatoms;
+  std::vector<StMol> sm;
+  std::vector<int> gatoms;
   std::vector<int2> bitems;  // (staged group, first point) per Becke block
-  std::vector<double> sx, sy, sz, sw, ax, ay, az, aw, rij;
-  for (int k = 0; k < nmol; ++k) {
-    const Molecule& mol = *mols[k];
-    const StagedPartition& st = *staged[k];
-    sm[k].atom_base = (int)ax.size();
-    sm[k].natom = mol.natom;
-    sm[k].rij_off = (long long)rij.size();
-    ax.insert(ax.end(), mol.x.begin(), mol.x.end());
-    ay.insert(ay.end(), mol.y.begin(), mol.y.end());
-    az.insert(az.end(), mol.z.begin(), mol.z.end());
-    rij.insert(rij.end(), st.rij.begin(), st.rij.end());
-    for (int a = 0; a < mol.natom; ++a)
-      for (int b = 0; b < mol.natom; ++b) aw.push_back(becke_size_term(mol.rm[a], mol.rm[b]));
-    const GridPartition& gp = st.groups;
-    for (int g = 0; g < gp.ngroups; ++g) {
-      StGroup G;
-      G.pt_off = (int)sx.size();
-      G.npts = gp.npts[g];
-      G.atom_off = (int)gatoms.size();
-      G.natoms = st.atom_off[g + 1] - st.atom_off[g];
-      G.mol = k;
-      G.pad_ = 0;
-      gatoms.insert(gatoms.end(), st.atoms.begin() + st.atom_off[g],
-                    st.atoms.begin() + st.atom_off[g + 1]);
-      const int b = gp.pt_off[g];
-      sx.insert(sx.end(), gp.x.begin() + b, gp.x.begin() + b + G.npts);
-      sy.insert(sy.end(), gp.y.begin() + b, gp.y.begin() + b + G.npts);
-      sz.insert(sz.end(), gp.z.begin() + b, gp.z.begin() + b + G.npts);
-      sw.insert(sw.end(), gp.w.begin() + b, gp.w.begin() + b + G.npts);
-      owner.insert(owner.end(), st.owner.begin() + b, st.owner.begin() + b + G.npts);
-      for (int o = 0; o < G.npts; o += 256) bitems.push_back(make_int2((int)sg.size(), G.pt_off + o));
-      sg.push_back(G);
-    }
-  }
-  const int nst = (int)sg.size();
-  const long long npts = (long long)sx.size();
-  if (npts > (1LL << 31) - 256) throw std::runtime_error("g2g: too many staged points");
-  // one staging allocation, freed once the points are compacted into the arena
-  struct Stage {
-    StGroup* groups;
-    StMol* mols;
-    int2* bitems;
-    int *owner, *gatoms, *count;
-    double *ax, *ay, *az, *aw, *rij, *x, *y, *z, *w;
-  } d{};
-  Carver plan{nullptr};
-  auto carve = [&](Carver& c) {
-    d.groups = c.take<StGroup>(nst + 1);
-    d.mols = c.take<StMol>(nmol);
-    d.bitems = c.take<int2>(bitems.size() + 1);
-    d.owner = c.take<int>(npts + 1);
-    d.gatoms = c.take<int>(gatoms.size() + 1);
-    d.count = c.take<int>(nst + 1);
-    d.ax = c.take<double>(ax.size());
-    d.ay = c.take<double>(ay.size());
-    d.az = c.take<double>(az.size());
-    d.aw = c.take<double>(aw.size());
-    d.rij = c.take<double>(rij.size());
-    d.x = c.take<double>(npts);
-    d.y = c.take<double>(npts);
-    d.z = c.take<double>(npts);
-    d.w = c.take<double>(npts);
-  };
-  carve(plan);
-  // the staging buffer is kept (grow-only) across rebuilds, for the same
-  // reason as the arena
-  if (stage_cap_ < plan.used + 256) {
+  std::vector<double> ax, ay, az, aw, rij;
+  std::vector<GridPartition> meta;  // per molecule (point arrays unused)
+};
+// Device side: the staged points and the tables of StageHost.
+struct StageDev {
+  StGroup* groups;
+  StMol* mols;
+  int2* bitems;
+  int *owner, *gatoms, *count;
+  double *ax, *ay, *az, *aw, *rij, *x, *y, *z, *w;
+};
+
+namespace {
+
+// Atom tables of one molecule for the Becke weights: positions, the size
+// terms of every atom pair and the distances.
+void stage_atoms(const Molecule& mol, const std::vector<double>& rij, StageHost& h) {
+  StMol m;
+  m.atom_base = (int)h.ax.size();
+  m.natom = mol.natom;
+  m.rij_off = (long long)h.rij.size();
+  h.sm.push_back(m);
+  h.ax.insert(h.ax.end(), mol.x.begin(), mol.x.end());
+  h.ay.insert(h.ay.end(), mol.y.begin(), mol.y.end());
+  h.az.insert(h.az.end(), mol.z.begin(), mol.z.end());
+  h.rij.insert(h.rij.end(), rij.begin(), rij.end());
+  for (int a = 0; a < mol.natom; ++a)
+    for (int b = 0; b < mol.natom; ++b) h.aw.push_back(becke_size_term(mol.rm[a], mol.rm[b]));
+}
+
+// Carves the tables of h and n_points staged points.
+void carve_stage(Carver& c, const StageHost& h, long long n_points, StageDev& d) {
+  d.groups = c.take<StGroup>(h.sg.size() + 1);
+  d.mols = c.take<StMol>(h.sm.size() + 1);
+  d.bitems = c.take<int2>(h.bitems.size() + 1);
+  d.owner = c.take<int>(n_points + 1);
+  d.gatoms = c.take<int>(h.gatoms.size() + 1);
+  d.count = c.take<int>(h.sg.size() + 1);
+  d.ax = c.take<double>(h.ax.size() + 1);
+  d.ay = c.take<double>(h.ay.size() + 1);
+  d.az = c.take<double>(h.az.size() + 1);
+  d.aw = c.take<double>(h.aw.size() + 1);
+  d.rij = c.take<double>(h.rij.size() + 1);
+  d.x = c.take<double>(n_points + 1);
+  d.y = c.take<double>(n_points + 1);
+  d.z = c.take<double>(n_points + 1);
+  d.w = c.take<double>(n_points + 1);
+}
+
+}  // namespace
+
+// The grow-only staging buffer of the device grid rebuilds, at least `bytes`.
+char* Engine::stage_buffer(size_t bytes) {
+  if (stage_cap_ < bytes + 256) {
     if (stage_) XCQ_HIP(hipFreeAsync(stage_, (hipStream_t)stream_));
     stage_ = nullptr;
     stage_cap_ = 0;
-    const size_t want = plan.used + 256 + (plan.used + 256) / 4;
-    stage_cap_ = pool_alloc(&stage_, want, plan.used + 256, (hipStream_t)stream_, device_);
+    const size_t want = bytes + 256 + (bytes + 256) / 4;
+    stage_cap_ = pool_alloc(&stage_, want, bytes + 256, (hipStream_t)stream_, device_);
   }
-  char* base = static_cast<char*>(stage_);
-  Carver c{base};
-  carve(c);
+  return static_cast<char*>(stage_);
+}
+
+// Becke weights, zero-weight removal and the group counts on the device,
+// then the final groups (at least min_points_per_cube points) are compacted
+// into the arena by the batch load.
+std::vector<GridPartition> Engine::weigh_and_load(const std::vector<const Molecule*>& mols,
+                                                  const StageHost& h, const StageDev& d,
+                                                  const Options& opt, Precision prec,
+                                                  double t_start) {
+  const int nmol = (int)mols.size();
+  const int nst = (int)h.sg.size();
   hipStream_t s = (hipStream_t)stream_;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     if (bytes) XCQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   };
-  up(d.groups, sg.data(), sg.size() * sizeof(StGroup));
-  up(d.mols, sm.data(), sm.size() * sizeof(StMol));
-  up(d.bitems, bitems.data(), bitems.size() * sizeof(int2));
-  up(d.owner, owner.data(), owner.size() * sizeof(int));
-  up(d.gatoms, gatoms.data(), gatoms.size() * sizeof(int));
-  up(d.ax, ax.data(), ax.size() * sizeof(double));
-  up(d.ay, ay.data(), ay.size() * sizeof(double));
-  up(d.az, az.data(), az.size() * sizeof(double));
-  up(d.aw, aw.data(), aw.size() * sizeof(double));
-  up(d.rij, rij.data(), rij.size() * sizeof(double));
-  up(d.x, sx.data(), npts * sizeof(double));
-  up(d.y, sy.data(), npts * sizeof(double));
-  up(d.z, sz.data(), npts * sizeof(double));
-  up(d.w, sw.data(), npts * sizeof(double));
-  if (!bitems.empty())
-    hipLaunchKernelGGL(k_becke, dim3((unsigned)bitems.size()), dim3(256), 0, s, d.groups,
+  up(d.groups, h.sg.data(), h.sg.size() * sizeof(StGroup));
+  up(d.mols, h.sm.data(), h.sm.size() * sizeof(StMol));
+  up(d.bitems, h.bitems.data(), h.bitems.size() * sizeof(int2));
+  up(d.gatoms, h.gatoms.data(), h.gatoms.size() * sizeof(int));
+  up(d.ax, h.ax.data(), h.ax.size() * sizeof(double));
+  up(d.ay, h.ay.data(), h.ay.size() * sizeof(double));
+  up(d.az, h.az.data(), h.az.size() * sizeof(double));
+  up(d.aw, h.aw.data(), h.aw.size() * sizeof(double));
+  up(d.rij, h.rij.data(), h.rij.size() * sizeof(double));
+  if (!h.bitems.empty())
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)h.bitems.size()), dim3(256), 0, s, d.groups,
                        d.bitems, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.aw, d.rij, d.x,
                        d.y, d.z, d.w);
   const int drop = opt.remove_zero_weights ? 1 : 0;
@@ -3524,7 +3856,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   dp.w = d.w;
   dp.drop_zero = drop;
   for (int k = 0, gs = 0; k < nmol; ++k) {
-    const GridPartition& in = staged[k]->groups;
+    const GridPartition& in = h.meta[k];
     GridPartition& out = parts[k];
     int pts = 0;
     for (int g = 0; g < in.ngroups; ++g, ++gs) {
@@ -3553,6 +3885,327 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
   load_batch_impl(mols, pp, prec, &dp);
   load_ms_[0] = weights_ms;
   return parts;
+}
+
+std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*>& mols,
+                                              const std::vector<const StagedPartition*>& staged,
+                                              const Options& opt, Precision prec) {
+  XCQ_HIP(hipSetDevice(device_));
+  check_not_pending("a grid load");
+  const double t_start = wall();
+  const int nmol = (int)mols.size();
+  if (nmol == 0 || staged.size() != mols.size())
+    throw std::runtime_error("g2g: a staged batch needs one staging per molecule");
+  StageHost h;
+  std::vector<int> owner;
+  std::vector<double> sx, sy, sz, sw;
+  for (int k = 0; k < nmol; ++k) {
+    const Molecule& mol = *mols[k];
+    const StagedPartition& st = *staged[k];
+    stage_atoms(mol, st.rij, h);
+    const GridPartition& gp = st.groups;
+    h.meta.push_back(gp);
+    for (int g = 0; g < gp.ngroups; ++g) {
+      StGroup G;
+      G.pt_off = (int)sx.size();
+      G.npts = gp.npts[g];
+      G.atom_off = (int)h.gatoms.size();
+      G.natoms = st.atom_off[g + 1] - st.atom_off[g];
+      G.mol = k;
+      G.pad_ = 0;
+      h.gatoms.insert(h.gatoms.end(), st.atoms.begin() + st.atom_off[g],
+                      st.atoms.begin() + st.atom_off[g + 1]);
+      const int b = gp.pt_off[g];
+      sx.insert(sx.end(), gp.x.begin() + b, gp.x.begin() + b + G.npts);
+      sy.insert(sy.end(), gp.y.begin() + b, gp.y.begin() + b + G.npts);
+      sz.insert(sz.end(), gp.z.begin() + b, gp.z.begin() + b + G.npts);
+      sw.insert(sw.end(), gp.w.begin() + b, gp.w.begin() + b + G.npts);
+      owner.insert(owner.end(), st.owner.begin() + b, st.owner.begin() + b + G.npts);
+      for (int o = 0; o < G.npts; o += 256)
+        h.bitems.push_back(make_int2((int)h.sg.size(), G.pt_off + o));
+      h.sg.push_back(G);
+    }
+  }
+  const long long npts = (long long)sx.size();
+  if (npts > (1LL << 31) - 256) throw std::runtime_error("g2g: too many staged points");
+  StageDev d{};
+  Carver plan{nullptr};
+  carve_stage(plan, h, npts, d);
+  Carver c{stage_buffer(plan.used)};
+  carve_stage(c, h, npts, d);
+  hipStream_t s = (hipStream_t)stream_;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) XCQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  up(d.owner, owner.data(), owner.size() * sizeof(int));
+  up(d.x, sx.data(), npts * sizeof(double));
+  up(d.y, sy.data(), npts * sizeof(double));
+  up(d.z, sz.data(), npts * sizeof(double));
+  up(d.w, sw.data(), npts * sizeof(double));
+  return weigh_and_load(mols, h, d, opt, prec, t_start);
+}
+
+// The whole grid rebuild on the device: points, groups and significant
+// functions (k_gen_*), then the weights and the load as for a staged batch.
+std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecule*>& mols,
+                                                  const Options& opt, Precision prec) {
+  XCQ_HIP(hipSetDevice(device_));
+  check_not_pending("a grid load");
+  const double t_start = wall();
+  const int nmol = (int)mols.size();
+  if (nmol == 0) throw std::runtime_error("g2g: empty batch");
+  // host: the grid frames (O(atoms^2 + shells) per molecule) and the tables
+  std::vector<GenAtom> gatom;
+  std::vector<GenMol> gmol(nmol);
+  std::vector<double> r1, wrad, ex, ey, ez, ew, srad, sh_a, sh_thr;
+  std::vector<int> sh_first, sh_len, sh_atom, bin_mol;
+  StageHost h;
+  long long nraw = 0, nslot = 0, naslot = 0;
+  std::vector<GridFrame> frames(nmol);
+  for (int k = 0; k < nmol; ++k) {
+    const Molecule& mol = *mols[k];
+    if (mol.natom > kGenMaxAtoms)
+      throw std::runtime_error("g2g: the device grid build takes at most " +
+                               std::to_string(kGenMaxAtoms) + " atoms per molecule");
+    GridFrame& fr = frames[k];
+    fr = grid_frame(mol, opt);
+    stage_atoms(mol, fr.rij, h);
+    GenMol& M = gmol[k];
+    for (int d = 0; d < 3; ++d) {
+      M.lo[d] = fr.lo[d];
+      M.hi[d] = fr.hi[d];
+      M.dim[d] = fr.dim[d];
+    }
+    M.lcs = opt.little_cube_size;
+    if (fr.ncubes + (size_t)mol.natom + bin_mol.size() >= (size_t)1 << 30)
+      throw std::runtime_error("g2g: too many cubes in the bounding prism");
+    M.ncubes = (int)fr.ncubes;
+    M.natom = mol.natom;
+    M.bin_base = (int)bin_mol.size();
+    M.atom_base = (int)gatom.size();
+    M.ang_off = (int)ex.size();
+    M.ngrid = mol.ngrid;
+    M.sh_off = (int)sh_first.size();
+    M.nsh = (int)fr.sh_first.size();
+    M.slot_off = nslot;
+    M.aslot_off = naslot;
+    M.rij_off = h.sm[k].rij_off;
+    const long long nbins = (long long)fr.ncubes + mol.natom;
+    nslot += nbins * M.nsh;
+    naslot += nbins * mol.natom;
+    bin_mol.insert(bin_mol.end(), (size_t)nbins, k);
+    ex.insert(ex.end(), mol.ex.begin(), mol.ex.end());
+    ey.insert(ey.end(), mol.ey.begin(), mol.ey.end());
+    ez.insert(ez.end(), mol.ez.begin(), mol.ez.end());
+    ew.insert(ew.end(), mol.ew.begin(), mol.ew.end());
+    for (int a = 0; a < mol.natom; ++a) {
+      GenAtom A;
+      A.x = mol.x[a];
+      A.y = mol.y[a];
+      A.z = mol.z[a];
+      A.raw_off = nraw;
+      A.mol = k;
+      A.local = a;
+      A.rad_off = (int)r1.size() + fr.rad_off[a];
+      A.sphere_from = (int)fr.sphere_from[a];
+      gatom.push_back(A);
+      nraw += (long long)mol.shells[a] * mol.ngrid;
+    }
+    r1.insert(r1.end(), fr.r1.begin(), fr.r1.end());
+    wrad.insert(wrad.end(), fr.wrad.begin(), fr.wrad.end());
+    srad.insert(srad.end(), fr.srad.begin(), fr.srad.end());
+    sh_first.insert(sh_first.end(), fr.sh_first.begin(), fr.sh_first.end());
+    sh_len.insert(sh_len.end(), fr.sh_len.begin(), fr.sh_len.end());
+    sh_atom.insert(sh_atom.end(), fr.sh_atom.begin(), fr.sh_atom.end());
+    sh_a.insert(sh_a.end(), fr.sh_a.begin(), fr.sh_a.end());
+    sh_thr.insert(sh_thr.end(), fr.sh_thr.begin(), fr.sh_thr.end());
+  }
+  if (nraw >= (1LL << 31) - 256) throw std::runtime_error("g2g: too many grid points");
+  const int nbins = (int)bin_mol.size();
+  const double frame_s = wall() - t_start;
+  int bits = 1;  // keys 0 .. nbins (nbins: no group)
+  while ((1u << bits) <= (unsigned)nbins) ++bits;
+  // one staging plan: generation scratch, then the Becke tables at their
+  // upper bounds (groups <= keys, nuclei lists <= keys x atoms)
+  size_t sort_bytes = 0;
+  XCQ_HIP((hipError_t)radix_sort_pairs(nullptr, &sort_bytes, nullptr, nullptr, nullptr, nullptr,
+                                       (int)nraw, bits, stream_));
+  struct Gen {
+    GenAtom* atoms;
+    GenMol* mols;
+    double *r1, *wrad, *ex, *ey, *ez, *ew, *srad, *sh_a, *sh_thr;
+    int *sh_first, *sh_len, *sh_atom, *bin_mol, *count, *err, *slots, *aslots, *pk, *pa;
+    unsigned *keys, *vals, *skeys, *svals;
+    BinInfo* info;
+    PackItem* pitems;
+    void* sort;
+  } g{};
+  StageHost bound;  // sizes only
+  bound.sg.resize(nbins);
+  bound.sm = h.sm;
+  bound.gatoms.resize((size_t)naslot);
+  bound.bitems.resize((size_t)(nraw / 256 + nbins));
+  bound.ax = h.ax;
+  bound.ay = h.ay;
+  bound.az = h.az;
+  bound.aw = h.aw;
+  bound.rij = h.rij;
+  StageDev d{};
+  auto carve = [&](Carver& c) {
+    g.atoms = c.take<GenAtom>(gatom.size());
+    g.mols = c.take<GenMol>(nmol);
+    g.r1 = c.take<double>(r1.size());
+    g.wrad = c.take<double>(wrad.size());
+    g.ex = c.take<double>(ex.size());
+    g.ey = c.take<double>(ey.size());
+    g.ez = c.take<double>(ez.size());
+    g.ew = c.take<double>(ew.size());
+    g.srad = c.take<double>(srad.size());
+    g.sh_a = c.take<double>(sh_a.size() + 1);
+    g.sh_thr = c.take<double>(sh_thr.size() + 1);
+    g.sh_first = c.take<int>(sh_first.size() + 1);
+    g.sh_len = c.take<int>(sh_len.size() + 1);
+    g.sh_atom = c.take<int>(sh_atom.size() + 1);
+    g.bin_mol = c.take<int>(nbins);
+    g.count = c.take<int>(nbins);
+    g.err = c.take<int>(1);
+    g.slots = c.take<int>(nslot + 1);
+    g.aslots = c.take<int>(naslot + 1);
+    g.pk = c.take<int>(nslot + 1);
+    g.pa = c.take<int>(naslot + 1);
+    g.keys = c.take<unsigned>(nraw);
+    g.vals = c.take<unsigned>(nraw);
+    g.skeys = c.take<unsigned>(nraw);
+    g.svals = c.take<unsigned>(nraw);
+    g.info = c.take<BinInfo>(nbins);
+    g.pitems = c.take<PackItem>(nbins);
+    g.sort = c.take<char>(sort_bytes);
+    carve_stage(c, bound, nraw, d);
+  };
+  Carver plan{nullptr};
+  carve(plan);
+  Carver c{stage_buffer(plan.used)};
+  carve(c);
+  hipStream_t s = (hipStream_t)stream_;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) XCQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  up(g.atoms, gatom.data(), gatom.size() * sizeof(GenAtom));
+  up(g.mols, gmol.data(), gmol.size() * sizeof(GenMol));
+  up(g.r1, r1.data(), r1.size() * sizeof(double));
+  up(g.wrad, wrad.data(), wrad.size() * sizeof(double));
+  up(g.ex, ex.data(), ex.size() * sizeof(double));
+  up(g.ey, ey.data(), ey.size() * sizeof(double));
+  up(g.ez, ez.data(), ez.size() * sizeof(double));
+  up(g.ew, ew.data(), ew.size() * sizeof(double));
+  up(g.srad, srad.data(), srad.size() * sizeof(double));
+  up(g.sh_a, sh_a.data(), sh_a.size() * sizeof(double));
+  up(g.sh_thr, sh_thr.data(), sh_thr.size() * sizeof(double));
+  up(g.sh_first, sh_first.data(), sh_first.size() * sizeof(int));
+  up(g.sh_len, sh_len.data(), sh_len.size() * sizeof(int));
+  up(g.sh_atom, sh_atom.data(), sh_atom.size() * sizeof(int));
+  up(g.bin_mol, bin_mol.data(), bin_mol.size() * sizeof(int));
+  // counts and the error flag start at zero (a fill kernel, no memset)
+  hipLaunchKernelGGL(k_fill_int, dim3((nbins + 255) / 256), dim3(256), 0, s, g.count, nbins, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s, g.err, 1, 0);
+  const int natoms = (int)gatom.size();
+  if (nraw > 0)
+    hipLaunchKernelGGL(k_gen_keys, dim3((unsigned)((nraw + 255) / 256)), dim3(256), 0, s,
+                       g.atoms, natoms, g.mols, g.r1, g.wrad, g.ex, g.ey, g.ez, g.ew, nraw,
+                       (unsigned)nbins, g.keys, g.vals, g.err);
+  XCQ_HIP(hipGetLastError());
+  XCQ_HIP((hipError_t)radix_sort_pairs(g.sort, &sort_bytes, g.keys, g.skeys, g.vals, g.svals,
+                                       (int)nraw, bits, stream_));
+  if (nraw > 0)
+    hipLaunchKernelGGL(k_gen_count, dim3((unsigned)((nraw + 255) / 256)), dim3(256), 0, s,
+                       g.skeys, nraw, (unsigned)nbins, g.count);
+  up(d.rij, h.rij.data(), h.rij.size() * sizeof(double));
+  hipLaunchKernelGGL(k_gen_screen, dim3(nbins), dim3(256), 0, s, g.mols, g.bin_mol, g.atoms,
+                     g.srad, d.rij, g.sh_first, g.sh_len, g.sh_atom, g.sh_a, g.sh_thr,
+                     opt.assign_all_functions ? 1 : 0, g.slots, g.aslots, g.info);
+  XCQ_HIP(hipGetLastError());
+  std::vector<int> count(nbins);
+  std::vector<BinInfo> info(nbins);
+  int err = 0;
+  XCQ_HIP(hipMemcpyAsync(count.data(), g.count, nbins * sizeof(int), hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipMemcpyAsync(info.data(), g.info, nbins * sizeof(BinInfo), hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipMemcpyAsync(&err, g.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipStreamSynchronize(s));
+  if (err) throw std::runtime_error("grid point outside the cube lattice of the bounding prism");
+  // the staged groups, as the host keeps them: cubes with significant
+  // functions and enough points, in lattice order, then the spheres
+  std::vector<long long> start(nbins + 1, 0);
+  for (int b = 0; b < nbins; ++b) start[b + 1] = start[b] + count[b];
+  std::vector<PackItem> items;
+  std::vector<int> kept;
+  int soff = 0, aoff = 0;
+  for (int b = 0; b < nbins; ++b) {
+    const GenMol& M = gmol[bin_mol[b]];
+    const bool cube = b - M.bin_base < M.ncubes;
+    const BinInfo& bi = info[b];
+    if (cube && (bi.nsel == 0 || (unsigned)count[b] < opt.min_points_per_cube)) continue;
+    if (!cube && (opt.sphere_radius <= 0 || (unsigned)count[b] < opt.min_points_per_cube))
+      continue;
+    items.push_back(PackItem{b, bi.nsel, bi.nnuc, soff, aoff, 0});
+    kept.push_back(b);
+    soff += bi.nsel;
+    aoff += bi.nnuc;
+  }
+  std::vector<int> pk(soff), pa(aoff);
+  if (!items.empty()) {
+    up(g.pitems, items.data(), items.size() * sizeof(PackItem));
+    hipLaunchKernelGGL(k_gen_pack, dim3((unsigned)items.size()), dim3(256), 0, s, g.pitems,
+                       g.mols, g.bin_mol, g.slots, g.aslots, g.pk, g.pa);
+    XCQ_HIP(hipGetLastError());
+    if (soff) XCQ_HIP(hipMemcpyAsync(pk.data(), g.pk, soff * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (aoff) XCQ_HIP(hipMemcpyAsync(pa.data(), g.pa, aoff * sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  // staged points in sorted order (the points of every key, kept or not)
+  const long long nvalid = start[nbins];
+  if (nvalid > 0)
+    hipLaunchKernelGGL(k_gen_gather, dim3((unsigned)((nvalid + 255) / 256)), dim3(256), 0, s,
+                       g.atoms, natoms, g.mols, g.r1, g.wrad, g.ex, g.ey, g.ez, g.ew, g.svals,
+                       nvalid, d.x, d.y, d.z, d.w, d.owner);
+  XCQ_HIP(hipGetLastError());
+  XCQ_HIP(hipStreamSynchronize(s));
+  h.meta.assign(nmol, GridPartition());
+  for (size_t i = 0; i < kept.size(); ++i) {
+    const int b = kept[i], k = bin_mol[b];
+    const GenMol& M = gmol[k];
+    const BinInfo& bi = info[b];
+    const PackItem& it = items[i];
+    GridPartition& o = h.meta[k];
+    o.ngroups++;
+    o.kind.push_back(b - M.bin_base < M.ncubes ? 0 : 1);
+    o.npts.push_back(count[b]);
+    o.pt_off.push_back(0);
+    o.fn_off.push_back((int)o.fn.size());
+    for (int q = 0; q < bi.nsel; ++q) {
+      const int sh = pk[it.soff + q];
+      const GridFrame& fr = frames[k];
+      for (int cpt = 0; cpt < fr.sh_len[sh]; ++cpt) o.fn.push_back(fr.sh_first[sh] + cpt);
+    }
+    o.m.push_back(bi.m);
+    o.nshells.push_back(bi.nsel);
+    o.ns.push_back(bi.ns);
+    o.np.push_back(bi.np);
+    o.nd.push_back(bi.nd);
+    o.nnuc.push_back(bi.nnuc);
+    o.fhash.push_back(bi.fhash);
+    StGroup G;
+    G.pt_off = (int)start[b];
+    G.npts = count[b];
+    G.atom_off = (int)h.gatoms.size();
+    G.natoms = bi.nnuc;
+    G.mol = k;
+    G.pad_ = 0;
+    h.gatoms.insert(h.gatoms.end(), pa.begin() + it.aoff, pa.begin() + it.aoff + bi.nnuc);
+    for (int o2 = 0; o2 < G.npts; o2 += 256)
+      h.bitems.push_back(make_int2((int)h.sg.size(), G.pt_off + o2));
+    h.sg.push_back(G);
+  }
+  for (int k = 0; k < nmol; ++k) h.meta[k].seconds = frame_s;
+  return weigh_and_load(mols, h, d, opt, prec, t_start);
 }
 
 void Engine::download_points(double* x, double* y, double* z, double* w) {
@@ -3632,6 +4285,28 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
     XCQ_HIP(hipGetLastError());
     return;
   }
+  if (g.lo) {  // variant 8: phi's remainder plane, four-piece P_g
+    if (g.ngather > 0)
+      hipLaunchKernelGGL(k_gather_density_bf<4>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
+                         g.groups, g.mols, g.fnl, rho, pg3);
+    if (spin < 2) mark(ev, 1, s);
+    float* y = yb ? yb + (spin == 2 ? g.phi_elems : 0) : nullptr;
+    const int lopl = g.nval - 1;
+    if (g.ndensity > 0) {
+      if (g.gga)
+        hipLaunchKernelGGL((k_density_bf2<4, 64, false, true>), dim3(g.ndensity), dim3(256), 0, s,
+                           g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y,
+                           lopl);
+      else
+        hipLaunchKernelGGL((k_density_bf2<1, 64, false, true>), dim3(g.ndensity), dim3(256), 0, s,
+                           g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y,
+                           lopl);
+    }
+    XCQ_HIP(hipGetLastError());
+    return;
+  }
   if (g.ngather > 0)
     hipLaunchKernelGGL(k_gather_density_bf<3>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
                        g.groups, g.mols, g.fnl, rho, pg3);
@@ -3641,29 +4316,29 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
     } else if (g.density_variant == 2) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
     } else if (g.density_variant == 5) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
                            g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
                            g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
     } else if (g.density_variant == 3 || g.density_variant == 4) {
       const int nr = g.nrowsplit, ns = g.ndensity - nr;
       const DensityItem* si = g.ditems + nr;
@@ -3678,7 +4353,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr, 0);
       } else {
         if (nr > 0)
           hipLaunchKernelGGL((k_density_bf3<1, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
@@ -3686,7 +4361,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr, 0);
       }
     } else {
       if (g.gga)
@@ -3723,7 +4398,7 @@ static void launch_fock(DeviceGrid& g, hipStream_t s, const double* factor) {
 // Closed-shell split-BF16 builds whose density kernel is k_density_bf2
 // (variants 1, 2, 5) leave Y for the forces in ybuf.
 static bool ystore_forces(const DeviceGrid& g) {
-  if (g.inkernel) return g.ybuf != nullptr;  // both spins when open
+  if (g.inkernel || g.lo) return g.ybuf != nullptr;  // both spins when open
   return g.prec == Precision::BF16X6 && !g.open && g.ybuf &&
          (g.density_variant == 1 || g.density_variant == 2 || g.density_variant == 5);
 }
@@ -3747,7 +4422,8 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
                            g.ybuf + (sp == 1 ? g.phi_elems : 0), g.fmeta, g.shells, g.tsh, g.bas,
                            g.px, g.py, g.pz, factor, g.gpart);
       else if (ystore_forces(g))
-        hipLaunchKernelGGL(k_force_y, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups, g.ybuf,
+        hipLaunchKernelGGL(k_force_y, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,
+                           g.ybuf + (sp == 1 ? g.phi_elems : 0),
                            static_cast<const float*>(g.vals), factor, g.gpart);
       else
         hipLaunchKernelGGL(k_force_bf, dim3(g.nforce), dim3(256), 0, s, g.citems, g.groups,

@@ -33,11 +33,13 @@ double wall() {
 const int kP[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 const int kD[6][3] = {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}};
 
-// Radius test for one shell (g2g/partition.cpp:229-254): an s shell is kept
-// when a d^2 < mfe - ln((2a/pi)^3)/4; p and d shells when d is inside the
-// radius found by fixed-point iteration on r = sqrt((mfe - ln N + l ln r)/a).
-bool shell_reaches(int len, double a, double d2, unsigned mfe) {
-  if (len == 1) return a * d2 < mfe - std::log(std::pow((2. * a / M_PI), 3)) / 4;
+// Significance test of one shell (g2g/partition.cpp:229-254): an s shell is
+// kept when a d^2 < mfe - ln((2a/pi)^3)/4; p and d shells when d is inside
+// the radius found by fixed-point iteration on r = sqrt((mfe - ln N + l ln r)/a).
+// shell_radius gives the right-hand side (s) or the radius (p, d) once per
+// shell; shell_keeps applies it to a squared distance.
+double shell_radius(int len, double a, unsigned mfe) {
+  if (len == 1) return mfe - std::log(std::pow((2. * a / M_PI), 3)) / 4;
   double nrm = std::pow((2.0 * a / M_PI), 3);
   nrm = std::sqrt(nrm * 4.0 * a);
   const double lnn = std::log(len == 3 ? std::sqrt(nrm) : std::fabs(nrm));
@@ -48,7 +50,11 @@ bool shell_reaches(int len, double a, double d2, unsigned mfe) {
     step = std::fabs(r - r_next);
     r = r_next;
   } while (step > 0.1);
-  return std::sqrt(d2) < r;
+  return r;
+}
+
+inline bool shell_keeps(int len, double a, double thr, double d2) {
+  return len == 1 ? a * d2 < thr : std::sqrt(d2) < thr;
 }
 
 struct Staging {
@@ -63,24 +69,18 @@ struct Selected {
   int ns = 0, np = 0, nd = 0;
 };
 
-void select(const Molecule& mol, const Options& opt, const std::vector<double>& d2,
-            const std::vector<double>& amin, Selected& s) {
+void select(const Molecule& mol, const Options& opt, const GridFrame& fr,
+            const std::vector<double>& d2, Selected& s) {
   std::vector<char> owner(mol.natom, 0);
-  const int pend = mol.ns + 3 * mol.np, dend = pend + 6 * mol.nd;
-  auto run = [&](int b, int e, int len, int& count) {
-    for (int f = b; f < e; f += len) {
-      const int at = mol.fn[f].atom;
-      if (opt.assign_all_functions || shell_reaches(len, amin[f], d2[at], opt.max_function_exponent)) {
-        s.first.push_back(f);
-        s.len.push_back(len);
-        owner[at] = 1;
-        ++count;
-      }
+  for (size_t k = 0; k < fr.sh_first.size(); ++k) {
+    const int at = fr.sh_atom[k], len = fr.sh_len[k];
+    if (opt.assign_all_functions || shell_keeps(len, fr.sh_a[k], fr.sh_thr[k], d2[at])) {
+      s.first.push_back(fr.sh_first[k]);
+      s.len.push_back(len);
+      owner[at] = 1;
+      (len == 1 ? s.ns : len == 3 ? s.np : s.nd)++;
     }
-  };
-  run(0, mol.ns, 1, s.ns);
-  run(mol.ns, pend, 3, s.np);
-  run(pend, dend, 6, s.nd);
+  }
   for (int a = 0; a < mol.natom; ++a)
     if (owner[a]) s.atoms.push_back(a);
 }
@@ -268,10 +268,10 @@ Molecule capture(const ProblemView& pv, unsigned grid_type) {
   return mol;
 }
 
-StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
-  const double t_start = wall();
+GridFrame grid_frame(const Molecule& mol, const Options& opt) {
   const int na = mol.natom, M = mol.M;
   if (na == 0) throw std::runtime_error("g2g: no atoms");
+  GridFrame fr;
   // most diffuse exponent per element and per function (regenerate:165-186)
   std::vector<double> amin_z(130, std::numeric_limits<double>::max());
   std::vector<double> amin_f(M, std::numeric_limits<double>::max());
@@ -284,46 +284,44 @@ StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
     }
   }
   // prism around all atoms (regenerate:189-219)
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int a = 0; a < na; ++a) {
     const double reach = 2 * std::sqrt(opt.max_function_exponent / amin_z[mol.Z[a]]);
     const double p[3] = {mol.x[a], mol.y[a], mol.z[a]};
     for (int k = 0; k < 3; ++k) {
       if (a == 0) {
-        lo[k] = p[k] - reach;
-        hi[k] = p[k] + reach;
+        fr.lo[k] = p[k] - reach;
+        fr.hi[k] = p[k] + reach;
       } else {
-        lo[k] = std::min(lo[k], p[k] - reach);
-        hi[k] = std::max(hi[k], p[k] + reach);
+        fr.lo[k] = std::min(fr.lo[k], p[k] - reach);
+        fr.hi[k] = std::max(fr.hi[k], p[k] + reach);
       }
     }
   }
   const double lcs = opt.little_cube_size;
-  unsigned dim[3];
-  for (int k = 0; k < 3; ++k) dim[k] = (unsigned)std::ceil((hi[k] - lo[k]) / lcs);
-  const size_t ncubes = (size_t)dim[0] * dim[1] * dim[2];
+  for (int k = 0; k < 3; ++k) fr.dim[k] = (unsigned)std::ceil((fr.hi[k] - fr.lo[k]) / lcs);
+  fr.ncubes = (size_t)fr.dim[0] * fr.dim[1] * fr.dim[2];
 
-  std::vector<double> srad(na, 0.0);
-  std::vector<unsigned> sphere_from(na, 0);  // first shell index in the sphere
+  fr.srad.assign(na, 0.0);
+  fr.sphere_from.assign(na, 0);
   for (int a = 0; a < na; ++a) {
     const unsigned n = (unsigned)mol.shells[a];
     const unsigned inc = (unsigned)std::ceil(opt.sphere_radius * n);
-    sphere_from[a] = opt.sphere_radius > 0 ? n - inc : n;
+    fr.sphere_from[a] = opt.sphere_radius > 0 ? n - inc : n;
     if (opt.sphere_radius > 0 && inc > 0) {
       const double x = std::cos((M_PI / (n + 1)) * (n - inc + 1));
-      srad[a] = mol.rm[a] * (1.0 + x) / (1.0 - x);
+      fr.srad[a] = mol.rm[a] * (1.0 + x) / (1.0 - x);
     }
   }
-  std::vector<double> rij((size_t)na * na);
+  fr.rij.resize((size_t)na * na);
   for (int i = 0; i < na; ++i)
     for (int j = 0; j < na; ++j) {
       const double dx = mol.x[i] - mol.x[j], dy = mol.y[i] - mol.y[j], dz = mol.z[i] - mol.z[j];
-      rij[(size_t)i * na + j] = std::sqrt(dx * dx + dy * dy + dz * dz);
+      fr.rij[(size_t)i * na + j] = std::sqrt(dx * dx + dy * dy + dz * dz);
     }
-
-  // radial Gauss-Chebyshev (second kind) x angular rule (regenerate:274-322)
-  std::vector<Staging> cubes(ncubes), spheres(na);
+  // radial Gauss-Chebyshev (second kind) nodes of every atom (regenerate:274-300)
+  fr.rad_off.resize(na + 1);
   for (int a = 0; a < na; ++a) {
+    fr.rad_off[a] = (int)fr.r1.size();
     const int n = mol.shells[a];
     const double t0 = M_PI / (n + 1), rm = mol.rm[a];
     for (int sh = 0; sh < n; ++sh) {
@@ -331,8 +329,47 @@ StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
       const double x = std::cos(t1);
       const double wr = t0 * std::fabs(std::sin(t1));
       const double r1 = rm * (1.0 + x) / (1.0 - x);
-      const double wrad = wr * (r1 * r1) * rm * 2.0 / ((1.0 - x) * (1.0 - x));
-      const bool in_sphere = (unsigned)sh >= sphere_from[a];
+      fr.r1.push_back(r1);
+      fr.wrad.push_back(wr * (r1 * r1) * rm * 2.0 / ((1.0 - x) * (1.0 - x)));
+    }
+  }
+  fr.rad_off[na] = (int)fr.r1.size();
+  // significance test of every basis shell (partition.cpp:229-254)
+  const int pend = mol.ns + 3 * mol.np, dend = pend + 6 * mol.nd;
+  auto shells = [&](int b, int e, int len) {
+    for (int f = b; f < e; f += len) {
+      const double a = amin_f[f];
+      fr.sh_first.push_back(f);
+      fr.sh_len.push_back(len);
+      fr.sh_atom.push_back(mol.fn[f].atom);
+      fr.sh_a.push_back(a);
+      fr.sh_thr.push_back(shell_radius(len, a, opt.max_function_exponent));
+    }
+  };
+  shells(0, mol.ns, 1);
+  shells(mol.ns, pend, 3);
+  shells(pend, dend, 6);
+  return fr;
+}
+
+StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
+  const double t_start = wall();
+  const int na = mol.natom;
+  const GridFrame fr = grid_frame(mol, opt);
+  const double* lo = fr.lo;
+  const double* hi = fr.hi;
+  const unsigned* dim = fr.dim;
+  const double lcs = opt.little_cube_size;
+  const std::vector<double>& rij = fr.rij;
+  const std::vector<double>& srad = fr.srad;
+
+  // points of every atom shell and angular node (regenerate:274-322)
+  std::vector<Staging> cubes(fr.ncubes), spheres(na);
+  for (int a = 0; a < na; ++a) {
+    const int n = mol.shells[a];
+    for (int sh = 0; sh < n; ++sh) {
+      const double r1 = fr.r1[fr.rad_off[a] + sh], wrad = fr.wrad[fr.rad_off[a] + sh];
+      const bool in_sphere = (unsigned)sh >= fr.sphere_from[a];
       for (int k = 0; k < mol.ngrid; ++k) {
         const double q[3] = {mol.x[a] + mol.ex[k] * r1, mol.y[a] + mol.ey[k] * r1,
                              mol.z[a] + mol.ez[k] * r1};
@@ -402,7 +439,7 @@ StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
           d2[a] = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         }
         Selected s;
-        select(mol, opt, d2, amin_f, s);
+        select(mol, opt, fr, d2, s);
         if (s.first.empty() || st.x.size() < opt.min_points_per_cube) continue;
         emit(0, st, s);
       }
@@ -419,7 +456,7 @@ StagedPartition stage_partition(const Molecule& mol, const Options& opt) {
         }
       }
       Selected s;
-      select(mol, opt, d2, amin_f, s);
+      select(mol, opt, fr, d2, s);
       if (st.x.size() < opt.min_points_per_cube) continue;
       emit(1, st, s);
     }

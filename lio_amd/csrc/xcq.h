@@ -120,6 +120,37 @@ struct StagedPartition {
   std::vector<double> rij;  // natom x natom distances
 };
 StagedPartition stage_partition(const Molecule& mol, const Options& opt);
+
+// Everything about a molecule's grid that the partition rules need before
+// any point exists (g2g/regenerate_partition.cpp:161-322,
+// g2g/classify_functions.cpp:12-141): the bounding prism and its cube
+// lattice, the sphere shells and radii of every atom, the radial nodes
+// (radius, radial weight) of every atom shell, the atom distances and, per
+// basis shell, the significance test of g2g/partition.cpp:229-254 reduced to
+// a threshold.  The host staging and the device grid build both use it, so
+// their points, groups and function sets are the same.
+struct GridFrame {
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  unsigned dim[3] = {0, 0, 0};
+  size_t ncubes = 0;
+  std::vector<double> srad;          // per atom: sphere radius (0: no sphere)
+  std::vector<unsigned> sphere_from; // per atom: first radial shell inside the sphere
+  std::vector<int> rad_off;          // per atom: first entry of r1/wrad (atom-major)
+  std::vector<double> r1, wrad;      // per atom shell: radius and radial weight
+  std::vector<double> rij;           // natom x natom distances
+  // basis shells in function order (s, then p, then d): first function,
+  // length (1/3/6), atom, and the test: len 1: keep when a d^2 < thr;
+  // len 3/6: keep when sqrt(d^2) < thr
+  std::vector<int> sh_first, sh_len, sh_atom;
+  std::vector<double> sh_a, sh_thr;
+};
+GridFrame grid_frame(const Molecule& mol, const Options& opt);
+
+// Stable device radix sort of (key, value) pairs on the low `bits` key bits
+// (gridsort.hip); temp == nullptr asks for the scratch size.  HIP status.
+int radix_sort_pairs(void* temp, size_t* temp_bytes, const unsigned* keys_in,
+                     unsigned* keys_out, const unsigned* vals_in, unsigned* vals_out, int n,
+                     int bits, void* stream);
 GridPartition finish_partition(const Molecule& mol, const Options& opt,
                                const StagedPartition& staged);
 
@@ -192,6 +223,12 @@ class Engine {
   std::vector<GridPartition> load_staged(const std::vector<const Molecule*>& mols,
                                          const std::vector<const StagedPartition*>& staged,
                                          const Options& opt, Precision prec);
+  // The whole rebuild on the device: points, cube/sphere groups and their
+  // significant functions (the host computes only each molecule's grid
+  // frame), then the Becke weights, zero-weight removal and the load as in
+  // load_staged.  Returns the final partitions (points on the device).
+  std::vector<GridPartition> load_generated(const std::vector<const Molecule*>& mols,
+                                            const Options& opt, Precision prec);
   // Frees the device grid (arena, pinned buffers, graph); a load restores it.
   void unload();
   // Padded point arrays of the loaded grid (padded_points() entries each).
@@ -270,6 +307,10 @@ class Engine {
   void check_not_pending(const char* what) const;
   Tuning tuning_;
   void drop_graph();
+  char* stage_buffer(size_t bytes);
+  std::vector<GridPartition> weigh_and_load(const std::vector<const Molecule*>& mols,
+                                            const struct StageHost& h, const struct StageDev& d,
+                                            const Options& opt, Precision prec, double t_start);
   void load_batch_impl(const std::vector<const Molecule*>& mols,
                        const std::vector<const GridPartition*>& parts, Precision prec,
                        const struct DevicePoints* dpts);

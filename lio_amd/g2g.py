@@ -141,7 +141,7 @@ def _ptr(a, t):
 
 
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x6": 2}
-GRID_MODE = {"host": 0, "device": 1}
+GRID_MODE = {"host": 0, "device": 1, "staged": 2}
 
 DEFAULT_OPTIONS = dict(little_cube_size=8.0, sphere_radius=0.6, assign_all_functions=0,
                        remove_zero_weights=1, min_points_per_cube=1, max_function_exponent=10)
@@ -218,8 +218,10 @@ class Context:
         self.precision = precision
 
     def set_grid_mode(self, mode: str):
-        """'host' or 'device': where the next build_grid applies the Becke
-        weights and the zero-weight removal."""
+        """Where the next build_grid builds grid and partition: 'host',
+        'device' (points, groups, functions, weights and compaction on the
+        GPU) or 'staged' (points, groups and functions on the host, weights
+        and compaction on the GPU).  The result is the same."""
         _check(self.L.xcq_set_grid_mode(self.h, GRID_MODE[mode]), "xcq_set_grid_mode")
         self.grid_mode = mode
 

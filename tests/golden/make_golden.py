@@ -11,7 +11,8 @@ checksum of the packed density guards against drift.
     python tests/golden/make_golden.py mol195_s107_pbe (only the named cases)
 
 The molecules of 58, 60 and 195 atoms are cases of the batched (3-60 atoms)
-and served (10-200 atoms) workloads at their largest sizes.
+and served (10-200 atoms) workloads at their largest sizes; the 195-atom
+Fock matrix (M = 2175) is stored as a sample with per-row sums (fock_fields).
 """
 import os
 import sys
@@ -50,6 +51,30 @@ OPEN_CASES = [("c0_pbe_open2", lambda: systems.open_shell(systems.config0())),
 
 GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsum", "fhash")
 
+# Fock matrices of more than this many functions are stored as a sample:
+# every diagonal entry, SAMPLE seeded random non-zero entries, and per
+# packed row (the entries (i, j >= i)) the sum and the sum of magnitudes of
+# all its entries -- enough to pin every entry statistically and the sample
+# exactly, in a few MB instead of tens.
+SAMPLED_M = 1500
+SAMPLE = 100000
+
+
+def fock_fields(F, M, prefix="fock"):
+    """The fixture fields of a packed Fock matrix (full or sampled)."""
+    nz = np.nonzero(F)[0]
+    if M <= SAMPLED_M:
+        return {prefix + "_index": nz.astype(np.int64), prefix + "_value": F[nz]}
+    i = np.arange(M)
+    diag = i * M - i * (i - 1) // 2
+    rng = np.random.default_rng(12345)
+    pick = np.union1d(diag, rng.choice(nz, size=min(SAMPLE, len(nz)), replace=False))
+    row = np.repeat(i, M - i)  # packed entry -> its row
+    return {prefix + "_index": pick.astype(np.int64), prefix + "_value": F[pick],
+            prefix + "_rowsum": np.bincount(row, weights=F, minlength=M),
+            prefix + "_rowabs": np.bincount(row, weights=np.abs(F), minlength=M),
+            prefix + "_sampled": 1}
+
 
 def main(only=()):
     from pyxc import RefG2G
@@ -67,8 +92,7 @@ def main(only=()):
         out = dict(
             energy=E, electrons=N, M=s.M, natom=s.natom, iexch=s.iexch,
             rho_sum=float(s.rho_packed.sum()), rho_norm=float(np.linalg.norm(s.rho_packed)),
-            fock_index=nz.astype(np.int64), fock_value=F[nz],
-            solve_seconds=t, forces=forces, energy_forces=Ef,
+            solve_seconds=t, forces=forces, energy_forces=Ef, **fock_fields(F, s.M),
         )
         for f in GROUP_FIELDS:
             out["g_" + f] = np.array([g[f] for g in groups],

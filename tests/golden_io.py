@@ -26,22 +26,52 @@ def system_for(name: str):
 
 
 def load(name: str):
+    """The fixture as a dict.  A sampled Fock matrix (large M) has fock=None
+    and fock_sample=(packed indices, values), fock_rowsum, fock_rowabs:
+    compare with fock_errors."""
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
     M = int(z["M"])
+    if "fock_sampled" in z.files:
+        return dict(energy=float(z["energy"]), electrons=float(z["electrons"]), fock=None, M=M,
+                    fock_sample=(np.array(z["fock_index"]), np.array(z["fock_value"])),
+                    fock_rowsum=np.array(z["fock_rowsum"]), fock_rowabs=np.array(z["fock_rowabs"]),
+                    groups=_groups(z), rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]),
+                    forces=np.array(z["forces"]), energy_forces=float(z["energy_forces"]),
+                    fock_b=None)
     F = np.zeros(M * (M + 1) // 2)
     F[z["fock_index"]] = z["fock_value"]
     Fb = None
     if "fock_b_value" in z.files:  # open shell: F is the alpha Fock, Fb the beta one
         Fb = np.zeros(M * (M + 1) // 2)
         Fb[z["fock_index"]] = z["fock_b_value"]
+    return dict(energy=float(z["energy"]), electrons=float(z["electrons"]), fock=F, M=M,
+                groups=_groups(z), rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]),
+                forces=np.array(z["forces"]), energy_forces=float(z["energy_forces"]),
+                fock_b=Fb)
+
+
+def _groups(z):
     groups = []
     for k in range(len(z["g_kind"])):
         groups.append({f: (int(z["g_" + f][k]) if f != "wsum" and f != "xsum" else float(z["g_" + f][k]))
                        for f in GROUP_FIELDS})
-    return dict(energy=float(z["energy"]), electrons=float(z["electrons"]), fock=F, M=M,
-                groups=groups, rho_sum=float(z["rho_sum"]), rho_norm=float(z["rho_norm"]),
-                forces=np.array(z["forces"]), energy_forces=float(z["energy_forces"]),
-                fock_b=Fb)
+    return groups
+
+
+def fock_errors(F, gold):
+    """(max |dF| over the stored entries, max row-sum deviation in units of
+    sqrt(entries in the row)): every entry for a full fixture (the second
+    value then 0), the sampled entries and every packed row's sum for a
+    sampled one."""
+    if gold["fock"] is not None:
+        return float(np.max(np.abs(F - gold["fock"]))), 0.0
+    idx, val = gold["fock_sample"]
+    M = gold["M"]
+    i = np.arange(M)
+    row = np.repeat(i, M - i)
+    rows = np.bincount(row, weights=F, minlength=M)
+    dev = np.abs(rows - gold["fock_rowsum"]) / np.sqrt(M - i)
+    return float(np.max(np.abs(F[idx] - val))), float(dev.max())
 
 
 def names():

@@ -51,7 +51,7 @@ def test_abi_symbol_set_covers_lioamber_calls():
 
 
 @pytest.mark.parametrize("name", ["c0_pbe", "mol3_s101_pbe", "mol10_s102_pbe", "mol25_s103_pbe",
-                                  "mol45_s104_pbe", "c1_pbe"])
+                                  "mol45_s104_pbe", "c1_pbe", "mol58_s105_pbe", "mol60_s106_pbe"])
 def test_host_partition_matches_reference_groups(name):
     gold = load(name)
     s = system_for(name)
