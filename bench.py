@@ -49,11 +49,15 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X dense FP64 matrix
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix
 PEAK_HBM_GBS = 8000.0
 MIXED = "bf16x6"
-BF16_PRODUCTS = 6          # BF16 MFMA products per split-BF16 tile product
-DENSITY_KERNEL = "k_density_bf2<4,64>"
-# HBM bytes per launch of DENSITY_KERNEL on (H2O)32 (rocprofv3 FETCH_SIZE x 2.0
-# + WRITE_SIZE x 1.0 in KiB, copy-kernel calibrated)
-DENSITY_TRAFFIC = (3284730 * 2.0 + 5712 * 1.0) * 1024
+# BF16 MFMA products per tile product, averaged over the density launch's four
+# products: Y_0 takes ten (four-piece operands), Y_x, Y_y, Y_z six each
+BF16_PRODUCTS = (10 + 3 * 6) / 4
+DENSITY_KERNEL = "k_density_bf2<4, 64, false, 1>"
+# HBM bytes per launch of DENSITY_KERNEL on (H2O)32: rocprofv3 FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0 in KiB (copy-kernel calibrated), measured on the tree
+# named in DENSITY_TRAFFIC_TREE (profiles/xc_fock_h2o32.md gives the command)
+DENSITY_TRAFFIC = None
+DENSITY_TRAFFIC_TREE = None
 
 
 def log(*a):
@@ -144,29 +148,39 @@ def run_mode(s, precision, steps, warmup, latency_samples):
         phases += ctx.kernel_times()
     phases /= steps
 
-    # latency of single builds, each waited for (queue -> energy on the host)
-    lat = []
+    # latency of single builds, each waited for (queue -> energy on the host),
+    # with the device time of the same build (HIP events from its first to
+    # its last kernel) to tell device time from host and driver time in the tail
+    lat, dev = [], []
     for _ in range(latency_samples):
         t0 = time.perf_counter()
         ctx.solve_device(rho.data_ptr(), F.data_ptr())
         ctx.sync_energy()
         lat.append(1e3 * (time.perf_counter() - t0))
+        dev.append(float(np.sum(ctx.kernel_times())))
     groups = ctx.groups()
     ctx.close()
-    lat = np.sort(np.array(lat))
+    lat, dev = np.array(lat), np.array(dev)
+    slow = lat >= np.percentile(lat, 99)
+    tail = {"device_ms_p50": float(np.percentile(dev, 50)), "device_ms_p99": float(np.percentile(dev, 99)),
+            "host_minus_device_ms_p50": float(np.percentile(lat - dev, 50)),
+            "slowest_samples": [{"latency_ms": float(a), "device_ms": float(b)}
+                                for a, b in zip(lat[slow], dev[slow])]}
     log("%s: %.3f ms/build, phases ms: expand %.4f density %.4f fock %.4f pack %.4f"
         % ((precision, ms_step) + tuple(phases)))
-    # algorithmic HBM bytes of one density launch: the ten value planes of
-    # every (point, function) pair once, the group density matrices once (FP64,
-    # or three BF16 pieces), the point weights read and the w*y2a factors written
+    # algorithmic HBM bytes of one density launch: the value planes of every
+    # (point, function) pair once (ten FP64 planes; split-BF16: ten FP32 planes
+    # plus phi's FP32 remainder), the group density matrices once (FP64, or
+    # four BF16 pieces), the point weights read and the w*y2a factors written
+    nplanes = 10 if precision == "fp64" else 11
     vb = 8 if precision == "fp64" else 4
-    pb = 8 if precision == "fp64" else 6
+    pb = 8
     pairs = sum(g["npoints"] * g["nfuncs"] for g in groups)
     msq = sum(g["nfuncs"] ** 2 for g in groups)
-    dens_bytes = 10 * vb * pairs + pb * msq + 16 * info["npoints"]
+    dens_bytes = nplanes * vb * pairs + pb * msq + 16 * info["npoints"]
     return dict(info=info, ms=ms_step, energy=energy, electrons=electrons, fock=fock,
                 phases=phases, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
-                dens_bytes=dens_bytes)
+                tail=tail, dens_bytes=dens_bytes)
 
 
 def bench_abi(s, steps=20, warmup=3):
@@ -208,6 +222,7 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     t0 = time.perf_counter()
     b = g2g.Batch(mols, precision=precision, grid_mode="device")
     t_build = time.perf_counter() - t0
+    load_ms = b.load_ms()
     rho = torch.from_numpy(b.packed_density()).cuda()
     F = torch.zeros_like(rho)
     b.solve_device(rho.data_ptr(), F.data_ptr())
@@ -233,6 +248,8 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     out = {"molecules": nmol, "atoms": int(sum(m.natom for m in mols)),
            "functions": int(sum(m.M for m in mols)), "ms_per_batch_build": ms,
            "molecules_per_s": nmol / (ms * 1e-3), "grid_build_s": t_build,
+           "grid_build_phases_ms": dict(zip(["grid_and_weights", "planning", "uploads",
+                                             "evaluation"], load_ms)),
            "arena_GB": b.arena_bytes() / 1e9, "energies": E, "electrons": N,
            "fock": F.cpu().numpy(), "offsets": b.offsets.copy()}
     b.close()
@@ -371,7 +388,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--latency-samples", type=int, default=50)
+    ap.add_argument("--latency-samples", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-abi", action="store_true",
@@ -431,7 +448,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x6 MFMA (fp32 accumulate) + f64 functional/reductions",
+        "dtype": "bf16x6 (split-BF16 MFMA, fp32 accumulate) + f64 functional/reductions",
         "data": "synthetic",
         "config": {"workload": "(H2O)32 water cluster, 6-31G** (M=800), 40x194 grid, PBE, "
                                "split-BF16 contractions with FP64 functional and accumulation",
@@ -440,10 +457,10 @@ def main():
         "grid_points_per_s": npts / (rmx["ms"] * 1e-3),
         "latency_ms": {"p50": rmx["p50"], "p99": rmx["p99"],
                        "samples": args.latency_samples, "what": "single mixed build, queue to "
-                       "energy on the host"},
+                       "energy on the host", "tail": rmx["tail"]},
         "fp64": {"value": 1000.0 / r64["ms"], "ms_per_step": r64["ms"],
                  "grid_points_per_s": npts / (r64["ms"] * 1e-3),
-                 "latency_ms": {"p50": r64["p50"], "p99": r64["p99"]},
+                 "latency_ms": {"p50": r64["p50"], "p99": r64["p99"], "tail": r64["tail"]},
                  "phases_ms": dict(zip(["expand", "density", "fock", "pack"], r64["phases"].tolist())),
                  "density_tflops": fp64_density, "density_frac_of_fp64_peak": fp64_density / PEAK_FP64_TFLOPS},
         "mixed": {"precision": MIXED, "speedup_vs_fp64": r64["ms"] / rmx["ms"],
@@ -455,12 +472,13 @@ def main():
         "energy_fp64": r64["energy"],
         "roofline": {"kernel": DENSITY_KERNEL, "bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": DENSITY_TRAFFIC,
+                     "traffic": DENSITY_TRAFFIC, "traffic_tree": DENSITY_TRAFFIC_TREE,
                      "algorithmic_bytes": rmx["dens_bytes"], "kernel_ms": rmx["phases"][1],
-                     "note": "achieved = algorithmic bytes (40 B per point-function pair, 6 B per "
+                     "note": "achieved = algorithmic bytes (44 B per point-function pair, 8 B per "
                              "P_g element, 16 B per point) / HIP-event duration of the launch; "
                              "traffic = TCC FETCH_SIZE + WRITE_SIZE per launch, calibrated "
-                             "against a plain copy kernel (profiles/xc_fock_h2o32.md)",
+                             "against a plain copy kernel, measured on traffic_tree "
+                             "(profiles/xc_fock_h2o32.md)",
                      "mfma_tflops_executed": mfma_tflops,
                      "mfma_frac": mfma_tflops / PEAK_BF16_TFLOPS},
     }

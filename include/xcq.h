@@ -208,12 +208,18 @@ int xcq_batch_energies(xcq_batch* b, double* energy, double* electrons);
 int xcq_batch_kernel_times(xcq_batch* b, double ms[4]);
 void* xcq_batch_stream(xcq_batch* b);
 long long xcq_batch_arena_bytes(xcq_batch* b);
+/* Host wall time (ms) of the last load's phases, as xcq_grid_info.load_ms. */
+int xcq_batch_load_ms(xcq_batch* b, double ms[4]);
 /* Autotuner: times `reps` builds for every candidate schedule c < ncand --
  * Fock slice length slices[c] (points, multiple of 128) and, when variants is
  * not NULL, split-BF16 density kernel variants[c] (0: 64-function chunks, 1 and 2:
  * 128- and 64-function chunks with LDS-staged density pieces, 3 and 4:
  * row-split 16-point blocks for groups above 64 functions, 5: 2 in the
- * radial-Hessian layout, 4 value planes instead of 10) -- on these buffers
+ * radial-Hessian layout, 4 value planes instead of 10, 6 and 7: the basis
+ * evaluated in the density kernel, 8 (default): 2 with phi's FP32 remainder
+ * plane and four-piece Y_0 products, 9: 2 with the remainder added in the
+ * epilogue; 0-5 and 9 are the faster, less accurate forms: |dN| near 1e-6
+ * at a few hundred electrons) -- on these buffers
  * (rebuilding the work lists for each), keeps the fastest and returns the
  * per-candidate ms in ms_out; *best is its index.  A captured graph is
  * dropped. */

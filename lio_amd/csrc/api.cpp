@@ -528,6 +528,10 @@ long long xcq_batch_arena_bytes(xcq_batch* b) {
   return b ? (long long)b->engine->arena_bytes() : -1;
 }
 
+int xcq_batch_load_ms(xcq_batch* b, double ms[4]) {
+  return guarded([&] { b->engine->load_phases(ms); });
+}
+
 static xcq::Tuning schedule(xcq::Tuning t, int fock_slice, int density_variant) {
   if (fock_slice > 0) t.fock_slice = fock_slice;
   if (density_variant >= 0) t.density_variant = density_variant;

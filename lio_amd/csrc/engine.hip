@@ -967,14 +967,20 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
 __device__ inline void split4f(const float (&hi)[8], const float (&lo)[8], bf8 (&pc)[4]);
 __device__ inline void mma10(const bf8 (&a)[4], const bf8 (&b)[4], f4& hi, f4& lo);
 
-// LO (density variant 8): Y_0 = P_g phi from phi's FP32 pair (plane 0 and
-// plane lopl, hi + lo) split into four BF16 pieces against four-piece P_g
-// (the ten piece products of weight >= 2^-24, mma10), and the epilogue's phi
-// is hi + lo in FP64.  Single-precision phi leaves a rounding error that does
-// not average out over the grid (every point of a radial shell sees the
-// same value of its atom's s functions), which is what bounds the electron
-// count of the three-piece form; Y_x, Y_y, Y_z keep the six-product chain.
-template <int NT, int ROWS, bool RAD = false, bool LO = false>
+// LO: phi's FP32 rounding, which does not average out over the grid (every
+// point of a radial shell sees the same value of its atom's s functions),
+// is what bounds the electron count of the three-piece form; plane lopl
+// holds its remainder l = phi - h (h = the FP32 plane 0).
+// LO == 1 (density variant 8): Y_0 = P_g phi from the pair (h, l) split into
+// four BF16 pieces against four-piece P_g (the ten piece products of weight
+// >= 2^-24, mma10), and the epilogue's phi is h + l in FP64.
+// LO == 2 (density variant 9): the products stay those of h (Y = P_g h, the
+// six-product chain) and the epilogue adds l to first order: with P_g
+// symmetric, rho = sum_ij P_ij (h + l)_i (h + l)_j = sum_i (h_i + 2 l_i) Y_i
+// + O(l^2), and the gradient sums sum_i d_a phi_i (P_g phi)_i gain
+// sum_j l_j (P_g d_a h)_j = l . Y_a, all from values the lane already holds.
+// Y_x, Y_y, Y_z keep the six-product chain in both forms.
+template <int NT, int ROWS, bool RAD = false, int LO = 0>
 __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const DensityItem* __restrict__ items,
                                                      const DevGroup* __restrict__ groups,
                                                      const unsigned short* __restrict__ pg3,
@@ -992,7 +998,7 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
                                                      float* __restrict__ ybuf, int lopl) {
   constexpr int NS = (NT == 4) ? 10 : 1;
   constexpr int RT = ROWS / 16;
-  constexpr int NPC = LO ? 4 : 3;            // P_g pieces
+  constexpr int NPC = LO == 1 ? 4 : 3;       // P_g pieces
   constexpr int UNITS = NPC * ROWS * 4;      // 16-byte units of one staged tile
   __shared__ u4 sA[2][UNITS];              // [buf][piece][row][k octet]
   __shared__ double sE[4][2];
@@ -1050,14 +1056,14 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       if (more) dma(c0, k0 + 32, buf ^ 1);
       const int kk = k0 + 8 * lk;
       bf8 bh[NT], bm[NT], bl[NT];
-      bf8 b4[LO ? 4 : 1];
-      if constexpr (LO) {  // phi's remainder of this step (not loaded ahead: registers)
+      bf8 b4[LO == 1 ? 4 : 1];
+      if constexpr (LO == 1) {  // phi's remainder of this step (not loaded ahead: registers)
         float xl[8];
         load8(Vl + kk, xl);
         split4f(xb[0], xl, b4);
       }
 #pragma unroll
-      for (int t = LO ? 1 : 0; t < NT; ++t) split3(xb[t], bh[t], bm[t], bl[t]);
+      for (int t = LO == 1 ? 1 : 0; t < NT; ++t) split3(xb[t], bh[t], bm[t], bl[t]);
       if (more) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) load8(Vp + t * plane + kk + 32, xb[t]);
@@ -1070,14 +1076,14 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
             const bf8 ah = __builtin_bit_cast(bf8, sA[buf][u]);
             const bf8 am = __builtin_bit_cast(bf8, sA[buf][ROWS * 4 + u]);
             const bf8 al = __builtin_bit_cast(bf8, sA[buf][2 * ROWS * 4 + u]);
-            if constexpr (LO) {
+            if constexpr (LO == 1) {
               const bf8 a4[4] = {ah, am, al, __builtin_bit_cast(bf8, sA[buf][3 * ROWS * 4 + u])};
               f4 hi, lo;
               mma10(a4, b4, hi, lo);
               acc[rt][0] += hi + lo;
             }
 #pragma unroll
-            for (int t = LO ? 1 : 0; t < NT; ++t)
+            for (int t = LO == 1 ? 1 : 0; t < NT; ++t)
               acc[rt][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
           }
         }
@@ -1121,16 +1127,24 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
           for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
         }
         fl4 vlo = fl4{0, 0, 0, 0};
-        if constexpr (LO) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
+        if constexpr (LO != 0) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
                        yz = acc[rt][3][i];
           const double v1 = v[1][i], v2 = v[2][i], v3 = v[3][i];
-          s[0] += ((double)v[0][i] + (double)vlo[i]) * y0;
-          s[1] += v1 * y0;
-          s[2] += v2 * y0;
-          s[3] += v3 * y0;
+          if constexpr (LO == 2) {
+            const double l = vlo[i];
+            s[0] += ((double)v[0][i] + 2.0 * l) * y0;
+            s[1] += v1 * y0 + l * yx;
+            s[2] += v2 * y0 + l * yy;
+            s[3] += v3 * y0 + l * yz;
+          } else {
+            s[0] += ((double)v[0][i] + (double)vlo[i]) * y0;
+            s[1] += v1 * y0;
+            s[2] += v2 * y0;
+            s[3] += v3 * y0;
+          }
           s[4] += (double)v[4][i] * y0 + v1 * yx;
           s[5] += (double)v[5][i] * y0 + v2 * yy;
           s[6] += (double)v[6][i] * y0 + v3 * yz;
@@ -1141,10 +1155,11 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       } else {
         const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
         fl4 vlo = fl4{0, 0, 0, 0};
-        if constexpr (LO) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
+        if constexpr (LO != 0) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          s[0] += ((double)v0[i] + (double)vlo[i]) * (double)acc[rt][0][i];
+          s[0] += ((double)v0[i] + (LO == 2 ? 2.0 : 1.0) * (double)vlo[i]) *
+                  (double)acc[rt][0][i];
       }
     }
 #pragma unroll
@@ -2951,7 +2966,7 @@ struct DeviceGrid {
   int M = 0, natom = 0, iexch = 9, nval = 10;  // M, natom: whole batch
   int nmol = 1, max_M = 0, max_ng = 0, max_mpad = 0;
   int fock_slice = kSliceDefault;
-  int density_variant = 2;
+  int density_variant = 8;
   int nrowsplit = 0;  // leading density items on the row-split kernel
   long long packed = 0;                 // packed entries of the whole batch
   std::vector<long long> mol_packed;    // per molecule offset, then the total
@@ -3232,7 +3247,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->fock_slice = tuning_.fock_slice;
   dg->density_variant = tuning_.density_variant;
 
-  if (dg->density_variant < 0 || dg->density_variant > 8)
+  if (dg->density_variant < 0 || dg->density_variant > 9)
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
@@ -3246,7 +3261,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   if (dg->open && prec == Precision::BF16X6 && dg->density_variant != 1 &&
       dg->density_variant != 2 && dg->density_variant < 5) {
-    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2, 5, 6, 7 or 8");
+    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2 or 5-9");
   }
   const int nspin = dg->open ? 2 : 1;
   // LDA keeps the gradient too: the XC forces need it.  The radial-Hessian
@@ -3255,7 +3270,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->radial = prec == Precision::BF16X6 && tuning_.density_variant == 5;
   dg->inkernel = prec == Precision::BF16X6 &&
                  (tuning_.density_variant == 6 || tuning_.density_variant == 7);
-  dg->lo = prec == Precision::BF16X6 && tuning_.density_variant == 8;
+  dg->lo = prec == Precision::BF16X6 &&
+           (tuning_.density_variant == 8 || tuning_.density_variant == 9);
   // in-kernel evaluation keeps no value planes (phi alone, function-major,
   // for the Fock kernel); variant 8 one more plane (phi's remainder)
   dg->nval = dg->inkernel ? 0 : ((dg->gga && !dg->radial) ? 10 : 4) + (dg->lo ? 1 : 0);
@@ -3264,7 +3280,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   const bool bf = prec == Precision::BF16X6;
   // split-BF16 K steps are 32 functions; its P_g holds three 2-byte pieces
   const int fn_align = bf ? 32 : kFnAlign;
-  const size_t pbytes = bf ? (dg->inkernel || dg->lo ? 4 : 3) * sizeof(unsigned short) : vbytes;
+  const size_t pbytes =
+      bf ? (dg->inkernel || dg->density_variant == 8 ? 4 : 3) * sizeof(unsigned short) : vbytes;
 
   // batch numbering: molecule k owns functions [fn_base, fn_base + M) and
   // packed entries [rho_off, rho_off + M (M + 1) / 2)
@@ -4285,24 +4302,33 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
     XCQ_HIP(hipGetLastError());
     return;
   }
-  if (g.lo) {  // variant 8: phi's remainder plane, four-piece P_g
-    if (g.ngather > 0)
-      hipLaunchKernelGGL(k_gather_density_bf<4>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
-                         g.groups, g.mols, g.fnl, rho, pg3);
+  if (g.lo) {  // variants 8 and 9: phi's remainder plane (8: four-piece P_g)
+    const bool four = g.density_variant == 8;
+    if (g.ngather > 0) {
+      if (four)
+        hipLaunchKernelGGL(k_gather_density_bf<4>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
+                           g.groups, g.mols, g.fnl, rho, pg3);
+      else
+        hipLaunchKernelGGL(k_gather_density_bf<3>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
+                           g.groups, g.mols, g.fnl, rho, pg3);
+    }
     if (spin < 2) mark(ev, 1, s);
     float* y = yb ? yb + (spin == 2 ? g.phi_elems : 0) : nullptr;
     const int lopl = g.nval - 1;
     if (g.ndensity > 0) {
-      if (g.gga)
-        hipLaunchKernelGGL((k_density_bf2<4, 64, false, true>), dim3(g.ndensity), dim3(256), 0, s,
-                           g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y,
-                           lopl);
+#define XCQ_LO(NT, L)                                                                           \
+  hipLaunchKernelGGL((k_density_bf2<NT, 64, false, L>), dim3(g.ndensity), dim3(256), 0, s,        \
+                     g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,        \
+                     g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y, lopl)
+      if (g.gga && four)
+        XCQ_LO(4, 1);
+      else if (g.gga)
+        XCQ_LO(4, 2);
+      else if (four)
+        XCQ_LO(1, 1);
       else
-        hipLaunchKernelGGL((k_density_bf2<1, 64, false, true>), dim3(g.ndensity), dim3(256), 0, s,
-                           g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y,
-                           lopl);
+        XCQ_LO(1, 2);
+#undef XCQ_LO
     }
     XCQ_HIP(hipGetLastError());
     return;

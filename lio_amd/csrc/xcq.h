@@ -120,6 +120,8 @@ struct StagedPartition {
   std::vector<double> rij;  // natom x natom distances
 };
 StagedPartition stage_partition(const Molecule& mol, const Options& opt);
+GridPartition finish_partition(const Molecule& mol, const Options& opt,
+                               const StagedPartition& staged);
 
 // Everything about a molecule's grid that the partition rules need before
 // any point exists (g2g/regenerate_partition.cpp:161-322,
@@ -151,8 +153,6 @@ GridFrame grid_frame(const Molecule& mol, const Options& opt);
 int radix_sort_pairs(void* temp, size_t* temp_bytes, const unsigned* keys_in,
                      unsigned* keys_out, const unsigned* vals_in, unsigned* vals_out, int n,
                      int bits, void* stream);
-GridPartition finish_partition(const Molecule& mol, const Options& opt,
-                               const StagedPartition& staged);
 
 // ---------------------------------------------------------------------------
 // Device engine
@@ -183,10 +183,19 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 //               with 4 row tiles a wave per pass; 5 = 2 in the radial-Hessian
 //               layout (phi and its gradient as planes, the second
 //               derivatives rebuilt from per-shell radial sums: 4 value
-//               planes instead of 10 in HBM, more arithmetic per point).
+//               planes instead of 10 in HBM, more arithmetic per point);
+//               6 and 7 = the basis evaluated inside the density kernel
+//               (128- and 64-function chunks; no value planes, FP64 values);
+//               8 (default) = 2 with one more plane holding phi's FP32
+//               remainder, Y_0 = P_g phi from four-piece operands (ten piece
+//               products) and phi in FP64 in the epilogue; 9 = 2 with the
+//               remainder added to first order in the epilogue only.
+//               Variants 0-5 and 9 leave |dN| up to about 1e-6 at a few
+//               hundred electrons; 6-8 keep it near 1e-8 (measured errors
+//               against FP64 are recorded by the tests).
 struct Tuning {
   int fock_slice = 1024;
-  int density_variant = 2;
+  int density_variant = 8;
 };
 
 struct SolveResult {

@@ -115,6 +115,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_batch_stream.restype = vp
     L.xcq_batch_arena_bytes.argtypes = [vp]
     L.xcq_batch_arena_bytes.restype = C.c_longlong
+    L.xcq_batch_load_ms.argtypes = [vp, _dp]
     L.xcq_batch_tune.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                  C.c_int, _dp, C.POINTER(C.c_int)]
     L.xcq_set_schedule.argtypes = [vp, C.c_int, C.c_int]
@@ -530,6 +531,13 @@ class Batch:
 
     def arena_bytes(self) -> int:
         return int(self.L.xcq_batch_arena_bytes(self.h))
+
+    def load_ms(self):
+        """Host wall time (ms) of the last load's phases: point weights
+        (device grids), planning, uploads, evaluation."""
+        ms = np.zeros(4)
+        _check(self.L.xcq_batch_load_ms(self.h, _ptr(ms, C.c_double)), "xcq_batch_load_ms")
+        return ms.tolist()
 
     def set_schedule(self, fock_slice=0, density_variant=-1):
         """Sets the kernel schedule and reloads (0 / -1 keep a knob)."""

@@ -1,12 +1,18 @@
 """Mixed-precision contractions against LIO's FP64 golden vectors, on the GPU.
 
-Bar for the mixed path: |dExc| <= 1e-6 Ha and max |dF_ij| <= 1e-6 against
-the FP64 reference on identical inputs, and the electron count within 1e-6
-for the split-BF16 mode.  Plain FP32 holds N to 5e-6 only: the seeded random
-densities of these configurations make rho a strongly cancelling quadratic
-form, which amplifies FP32 rounding of the GEMM accumulation in N more than
-in Exc.  The measured errors are printed so the log records them per
-configuration.
+Bar for the mixed path: |dExc| <= 1e-6 Ha, max |dF_ij| <= 1e-6 and the
+electron count within 1e-6 of the FP64 reference on identical inputs.  The
+split-BF16 default (density variant 8) is held to half of that, 0.5e-6, on
+(H2O)32 and the other cases here (and on a 195-atom molecule in
+test_gpu_scale.py): headroom for larger systems.
+
+Plain FP32 does not meet the bar on the electron count: its value planes are
+FP32, and phi's FP32 rounding repeats over every point of a radial shell
+(the atom's own s functions have one value there), so it does not average
+out; at (H2O)32 |dN| is about 1.1e-6.  Its cases hold N to 5e-6 (TOL_N) and
+Exc and F to the 1e-6 bar; the split-BF16 default is the mixed mode that
+meets the bar.  The measured errors are printed
+so the log records them per configuration.
 """
 import json
 import os
@@ -19,7 +25,8 @@ from golden_io import load, system_for
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
-TOL_N = {"fp32": 5e-6, "bf16x6": 1e-6}
+TOL_MIXED_DEFAULT = 0.5e-6  # split-BF16 default: half the bar
+TOL_N = {"fp32": 5e-6, "bf16x6": TOL_MIXED_DEFAULT}  # FP32 misses the bar on N (see above)
 OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
 
 
@@ -51,7 +58,8 @@ def test_mixed_precision_within_tolerance(name, precision):
     if os.path.isdir(OUT):
         with open(os.path.join(OUT, "precision_errors.jsonl"), "a") as f:
             f.write(json.dumps(rec) + "\n")
-    assert abs(dE) <= TOL and dF <= TOL and abs(dN) <= TOL_N[precision]
+    tol = TOL_MIXED_DEFAULT if precision == "bf16x6" else TOL
+    assert abs(dE) <= tol and dF <= tol and abs(dN) <= TOL_N[precision]
     ctx.close()
 
 
@@ -97,5 +105,5 @@ def test_radial_hessian_layout_within_tolerance(name):
     dN = info["electrons"] - gold["electrons"]
     print("%s radial-Hessian dE %.2e dF %.2e dN %.2e arena %.2f -> %.2f GB" %
           (name, dE, dF, dN, info10["arena_bytes"] / 1e9, info4["arena_bytes"] / 1e9), flush=True)
-    assert abs(dE) <= TOL and dF <= TOL and abs(dN) <= TOL_N["bf16x6"]
+    assert abs(dE) <= TOL and dF <= TOL and abs(dN) <= TOL
     assert info4["arena_bytes"] < info10["arena_bytes"]

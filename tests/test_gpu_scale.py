@@ -149,6 +149,8 @@ def test_195_atom_molecule_against_golden(precision):
     assert abs(dE) <= TOL[precision] and abs(dN) <= TOL[precision]
     assert dF <= TOL[precision] and drow <= TOL[precision]
     assert dfo <= TOL[precision]
+    if precision == "bf16x6":  # the default split-BF16 form keeps half the bar here
+        assert max(abs(dE), abs(dN), dF) <= 0.5e-6
 
 
 CHILD = r"""
