@@ -200,6 +200,20 @@ int xcq_batch_solve_device(xcq_batch* b, const double* d_rho, double* d_fock);
  * may be NULL.  xcq_batch_atoms gives the first batch atom of each molecule
  * (n + 1 entries, the last the total). */
 int xcq_batch_solve_forces(xcq_batch* b, const double* d_rho, double* d_fock, double* d_forces);
+/* xcq_batch_solve_forces without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory during
+ * the call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  xcq_batch_solve_forces allocates, at its first call on a
+ * split-BF16 grid, a buffer for Y = P_g phi of the force pass; this entry
+ * point takes Y from a static device array of the library instead
+ * (xcq_static_scratch_bytes(), 64 MiB, held by each device the library runs
+ * on until the process ends) and fails with an error when the batch needs
+ * more.  Every call on one device shares that array: calls that overlap on
+ * one device (several batches' streams at once) race.  Same results as
+ * xcq_batch_solve_forces. */
+int xcq_batch_solve_forces_static(xcq_batch* b, const double* d_rho, double* d_fock,
+                                  double* d_forces);
+long long xcq_static_scratch_bytes(void);
 int xcq_batch_atoms(xcq_batch* b, int* first_atom);
 int xcq_batch_capture(xcq_batch* b, const double* d_rho, double* d_fock);
 int xcq_batch_replay(xcq_batch* b);

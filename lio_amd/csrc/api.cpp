@@ -491,6 +491,16 @@ int xcq_batch_solve_forces(xcq_batch* b, const double* d_rho, double* d_fock, do
   });
 }
 
+int xcq_batch_solve_forces_static(xcq_batch* b, const double* d_rho, double* d_fock,
+                                  double* d_forces) {
+  return guarded([&] {
+    if (!d_forces) throw std::runtime_error("g2g: no force array");
+    b->engine->solve_device_static(d_rho, d_fock, d_forces);
+  });
+}
+
+long long xcq_static_scratch_bytes(void) { return (long long)xcq::Engine::static_scratch_bytes(); }
+
 int xcq_batch_atoms(xcq_batch* b, int* first_atom) {
   return guarded([&] {
     const int n = b->engine->molecules();

@@ -4540,6 +4540,65 @@ SolveResult Engine::solve_device(const double* d_rho, double* d_fock, bool want_
   return SolveResult{};
 }
 
+// Scratch of solve_device_static: the split-BF16 force pass's Y = P_g phi
+// (FP32, one value per padded point and function, per spin) in a file-scope
+// device array instead of a buffer allocated at the first forces build.
+// Each device holds its own copy from the module's load until the process
+// ends; every call on a device shares it and leaves its contents behind.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+// Device address of this device's scratch (looked up once per device, with
+// that device current; the host-side name of the array is not an address).
+static void* scratch_address(int device) {
+  if (device < 0 || device >= 64) throw std::runtime_error("g2g: device index out of range");
+  std::lock_guard<std::mutex> l(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    XCQ_HIP(hipSetDevice(device));
+    XCQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+size_t Engine::static_scratch_bytes() { return kStaticScratchBytes; }
+
+SolveResult Engine::solve_device_static(const double* d_rho, double* d_fock, double* d_forces) {
+  if (!grid_) throw std::runtime_error("g2g: solve before the grid was built");
+  DeviceGrid& g = *grid_;
+  const bool ystore = g.inkernel || g.lo || (!g.open && (g.density_variant == 1 ||
+                                                  g.density_variant == 2 ||
+                                                  g.density_variant == 5));
+  if (!d_forces || g.prec != Precision::BF16X6 || !ystore) {
+    // no Y buffer on these paths: no scratch beyond the grid's arena
+    launch_all(g, (hipStream_t)stream_, d_rho, d_fock, reinterpret_cast<hipEvent_t*>(events_),
+               d_forces);
+    return SolveResult{};
+  }
+  const size_t need = (size_t)(g.open ? 2 : 1) * (size_t)g.phi_elems * sizeof(float);
+  if (need > kStaticScratchBytes)
+    throw std::runtime_error("g2g: this grid's force pass needs " + std::to_string(need >> 20) +
+                             " MiB of scratch; the static scratch holds " +
+                             std::to_string(kStaticScratchBytes >> 20) +
+                             " MiB (use the allocating entry point)");
+  float* y = static_cast<float*>(scratch_address(device_));
+  XCQ_HIP(hipSetDevice(device_));
+  float* own = g.ybuf;  // the grid's own buffer (if any) is left untouched
+  g.ybuf = y;
+  try {
+    launch_all(g, (hipStream_t)stream_, d_rho, d_fock, reinterpret_cast<hipEvent_t*>(events_),
+               d_forces);
+  } catch (...) {
+    g.ybuf = own;
+    throw;
+  }
+  g.ybuf = own;
+  return SolveResult{};
+}
+
 void Engine::capture_graph(const double* d_rho, double* d_fock) {
   if (!grid_) throw std::runtime_error("g2g: graph capture before the grid was built");
   XCQ_HIP(hipSetDevice(device_));

@@ -273,6 +273,14 @@ class Engine {
   // atom, atom-major (3 a + c), molecule k's atoms from atoms(k).
   SolveResult solve_device(const double* d_rho_packed, double* d_fock, bool want_energy,
                            double* d_forces = nullptr);
+  // solve_device without any device allocation: where the split-BF16 force
+  // pass needs its Y buffer (allocated by solve_device at the first forces
+  // build), it takes it from a static device array of static_scratch_bytes()
+  // instead, which every call on the device shares (calls that overlap on
+  // one device race; each call writes Y in full before reading it).  Throws
+  // when the grid needs more than that.
+  SolveResult solve_device_static(const double* d_rho_packed, double* d_fock, double* d_forces);
+  static size_t static_scratch_bytes();
   // Waits for the stream and reads the last solve's energy and electron count
   // (of the first molecule); read_energies: (Exc, N) of every molecule.
   void read_energy(double* energy, double* electrons);

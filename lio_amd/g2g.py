@@ -95,6 +95,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_solve_forces.argtypes = [vp, _dp, _dp, _dp, C.c_int, C.POINTER(XcqSolveInfo)]
     L.xcq_solve_open.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.POINTER(XcqSolveInfo)]
     L.xcq_batch_solve_forces.argtypes = [vp, vp, vp, vp]
+    L.xcq_batch_solve_forces_static.argtypes = [vp, vp, vp, vp]
+    L.xcq_static_scratch_bytes.restype = C.c_longlong
     L.xcq_batch_atoms.argtypes = [vp, C.POINTER(C.c_int)]
     L.xcq_sync_energy.argtypes = [vp, _dp, _dp]
     L.xcq_kernel_times.argtypes = [vp, _dp]
@@ -498,6 +500,14 @@ class Batch:
         """Full build plus the forces of all batch atoms (device, atom-major)."""
         _check(self.L.xcq_batch_solve_forces(self.h, C.c_void_p(d_rho), C.c_void_p(d_fock),
                                              C.c_void_p(d_forces)), "xcq_batch_solve_forces")
+
+    def solve_forces_static(self, d_rho: int, d_fock: int, d_forces: int):
+        """solve_forces without any device allocation (the force pass's
+        scratch from the library's static device array; see
+        xcq_batch_solve_forces_static)."""
+        _check(self.L.xcq_batch_solve_forces_static(self.h, C.c_void_p(d_rho),
+                                                    C.c_void_p(d_fock), C.c_void_p(d_forces)),
+               "xcq_batch_solve_forces_static")
 
     def atoms(self):
         """First batch atom of each molecule, then the total."""
