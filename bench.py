@@ -56,8 +56,8 @@ DENSITY_KERNEL = "k_density_bf2<4, 64, false, 1>"
 # HBM bytes per launch of DENSITY_KERNEL on (H2O)32: rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 in KiB (copy-kernel calibrated), measured on the tree
 # named in DENSITY_TRAFFIC_TREE (profiles/xc_fock_h2o32.md gives the command)
-DENSITY_TRAFFIC = (3985240 * 2.0 + 135612 * 1.0) * 1024
-DENSITY_TRAFFIC_TREE = "294b854"
+DENSITY_TRAFFIC = None
+DENSITY_TRAFFIC_TREE = None
 
 
 def log(*a):
@@ -170,14 +170,13 @@ def run_mode(s, precision, steps, warmup, latency_samples):
         % ((precision, ms_step) + tuple(phases)))
     # algorithmic HBM bytes of one density launch: the value planes of every
     # (point, function) pair once (ten FP64 planes; split-BF16: ten FP32 planes
-    # plus phi's FP32 remainder), the group density matrices once (FP64, or
+    # plus phi's remainder in BF16), the group density matrices once (FP64, or
     # four BF16 pieces), the point weights read and the w*y2a factors written
-    nplanes = 10 if precision == "fp64" else 11
-    vb = 8 if precision == "fp64" else 4
+    pair_bytes = 80 if precision == "fp64" else 42
     pb = 8
     pairs = sum(g["npoints"] * g["nfuncs"] for g in groups)
     msq = sum(g["nfuncs"] ** 2 for g in groups)
-    dens_bytes = nplanes * vb * pairs + pb * msq + 16 * info["npoints"]
+    dens_bytes = pair_bytes * pairs + pb * msq + 16 * info["npoints"]
     return dict(info=info, ms=ms_step, energy=energy, electrons=electrons, fock=fock,
                 phases=phases, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
                 tail=tail, dens_bytes=dens_bytes)
@@ -225,6 +224,17 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     load_ms = b.load_ms()
     rho = torch.from_numpy(b.packed_density()).cuda()
     F = torch.zeros_like(rho)
+    torch.cuda.synchronize()
+    # the schedule: per size bucket Fock slices and the launch form, from the
+    # autotuner's JSON when it is in the tree, else tuned here and written
+    sched_path = os.path.join(ROOT, "lio_amd", "data", "schedules",
+                              "batch%d_seed%d_%s.json" % (nmol, seed, precision))
+    t0 = time.perf_counter()
+    sched = b.autotune(rho.data_ptr(), F.data_ptr(), path=sched_path)
+    t_tune = time.perf_counter() - t0
+    if not sched["loaded"] and os.path.isdir(os.path.join(ROOT, "gpurun_out")):
+        with open(os.path.join(ROOT, "gpurun_out", os.path.basename(sched_path)), "w") as f:
+            json.dump(sched, f, indent=1)
     b.solve_device(rho.data_ptr(), F.data_ptr())
     b.energies()
     b.capture(rho.data_ptr(), F.data_ptr())
@@ -250,6 +260,10 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
            "molecules_per_s": nmol / (ms * 1e-3), "grid_build_s": t_build,
            "grid_build_phases_ms": dict(zip(["grid_and_weights", "planning", "uploads",
                                              "evaluation"], load_ms)),
+           "schedule": {"source": "file" if sched["loaded"] else "tuned here (%.1f s)" % t_tune,
+                        "bucket_max_functions": sched["bucket_mhi"],
+                        "bucket_fock_slice": sched["bucket_slice"],
+                        "persistent_launch": sched["persistent"]},
            "arena_GB": b.arena_bytes() / 1e9, "energies": E, "electrons": N,
            "fock": F.cpu().numpy(), "offsets": b.offsets.copy()}
     b.close()
@@ -347,17 +361,25 @@ def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
     keys = ("builds_per_s", "latency_ms", "service_ms", "phase_ms_p50", "load_phase_ms_p50",
             "evictions", "retries")
     served = {}
-    for name, prec in (("fp64", "fp64"), ("mixed", MIXED)):
+    # the mixed modes: the default split-BF16 density (value planes in HBM)
+    # and the form that evaluates the basis inside the density kernel (no
+    # value planes: the arena of a replica shrinks about eightfold)
+    for name, prec, variant in (("fp64", "fp64", None), ("mixed", MIXED, None),
+                                ("mixed_inkernel", MIXED, 7)):
         srv = serve.Server(reps, precision=prec, slots=slots, hbm_budget=budget,
-                           grid_mode="device", slab=True)
+                           grid_mode="device", slab=True, density_variant=variant)
         sat = srv.run(serve.poisson_arrivals(nreq, 1e4, nrep, seed=seed), keep_fock=True,
                       max_seconds=300)
         served[name] = {(q.replica, q.step): q for q in sat["requests"]}
         rate = 0.6 * sat["builds_per_s"]
         paced = srv.run(serve.poisson_arrivals(nreq, rate, nrep, seed=seed + 1), max_seconds=300)
         srv.close()
+        built = [a for a in srv.arena if a]
         out[name] = {"saturated": {k: sat[k] for k in keys},
-                     "paced": dict(rate_hz=rate, **{k: paced[k] for k in keys})}
+                     "paced": dict(rate_hz=rate, **{k: paced[k] for k in keys}),
+                     "arena_gb": {"replicas_built": len(built),
+                                  "mean": float(np.mean(built)) / 1e9 if built else 0.0,
+                                  "max": float(np.max(built)) / 1e9 if built else 0.0}}
         log("serve %s: %.1f builds/s saturated, paced p50 %.0f ms p99 %.0f ms" % (
             name, sat["builds_per_s"], paced["latency_ms"]["p50"], paced["latency_ms"]["p99"]))
     out["speedup_saturated"] = (out["mixed"]["saturated"]["builds_per_s"]
@@ -365,21 +387,26 @@ def bench_serve(nrep=128, atoms=(10, 200), nreq=96, seed=4, slots=8):
     # the saturated runs of both modes serve the same (replica, step) requests
     # (same arrivals, fresh servers): the mixed result of every request
     # against the FP64 one
-    dE, dF, dN = [], [], []
-    for key, q in sorted(served["mixed"].items()):
-        r = served["fp64"][key]
-        dE.append(q.energy - r.energy)
-        dN.append(q.electrons - r.electrons)
-        dF.append(float(np.max(np.abs(q.fock - r.fock))))
-    dE, dF, dN = np.abs(dE), np.array(dF), np.abs(dN)
-    ok = (dE <= 1e-6) & (dF <= 1e-6) & (dN <= 1e-6)
-    out["mixed_vs_fp64_per_request"] = {
-        "requests": len(dE), "within_1e-6": int(ok.sum()),
-        "max_abs_dExc": float(dE.max()), "max_abs_dF": float(dF.max()),
-        "max_abs_dN": float(dN.max()),
-        "abs_dExc": [float("%.3g" % v) for v in dE], "max_abs_dF_each": [float("%.3g" % v) for v in dF],
-        "abs_dN": [float("%.3g" % v) for v in dN],
-        "atoms": [int(reps[k[0]].base.natom) for k in sorted(served["mixed"])]}
+    out["speedup_saturated_inkernel"] = (out["mixed_inkernel"]["saturated"]["builds_per_s"]
+                                         / out["fp64"]["saturated"]["builds_per_s"])
+    for mode in ("mixed", "mixed_inkernel"):
+        dE, dF, dN = [], [], []
+        for key, q in sorted(served[mode].items()):
+            r = served["fp64"][key]
+            dE.append(q.energy - r.energy)
+            dN.append(q.electrons - r.electrons)
+            dF.append(float(np.max(np.abs(q.fock - r.fock))))
+        dE, dF, dN = np.abs(dE), np.array(dF), np.abs(dN)
+        ok = (dE <= 1e-6) & (dF <= 1e-6) & (dN <= 1e-6)
+        rec = {"requests": len(dE), "within_1e-6": int(ok.sum()),
+               "max_abs_dExc": float(dE.max()), "max_abs_dF": float(dF.max()),
+               "max_abs_dN": float(dN.max())}
+        if mode == "mixed":
+            rec.update(abs_dExc=[float("%.3g" % v) for v in dE],
+                       max_abs_dF_each=[float("%.3g" % v) for v in dF],
+                       abs_dN=[float("%.3g" % v) for v in dN],
+                       atoms=[int(reps[k[0]].base.natom) for k in sorted(served[mode])])
+        out[mode + "_vs_fp64_per_request"] = rec
     return out
 
 
@@ -474,7 +501,7 @@ def main():
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": DENSITY_TRAFFIC, "traffic_tree": DENSITY_TRAFFIC_TREE,
                      "algorithmic_bytes": rmx["dens_bytes"], "kernel_ms": rmx["phases"][1],
-                     "note": "achieved = algorithmic bytes (44 B per point-function pair, 8 B per "
+                     "note": "achieved = algorithmic bytes (42 B per point-function pair, 8 B per "
                              "P_g element, 16 B per point) / HIP-event duration of the launch; "
                              "traffic = TCC FETCH_SIZE + WRITE_SIZE per launch, calibrated "
                              "against a plain copy kernel, measured on traffic_tree "

@@ -245,6 +245,23 @@ int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps,
  * reloads at once.  No reference counterpart. */
 int xcq_set_schedule(xcq_context* ctx, int fock_slice, int density_variant);
 int xcq_batch_set_schedule(xcq_batch* b, int fock_slice, int density_variant);
+/* on = 1: the whole closed-shell build with the split-BF16 default density
+ * kernel runs as one persistent launch over an ordered queue of every phase's
+ * items (expansion, density and functional, Fock tile slices, tile sums,
+ * packing, energies), dependencies tracked by per-group counters; on = 0
+ * (default): one launch per phase.  The results are bitwise the same.  Takes
+ * effect at the next xcq_build_grid (a batch reloads at once).  Other modes
+ * (FP64, FP32, open shell, other density variants) keep the phase launches. */
+int xcq_set_persistent(xcq_context* ctx, int on);
+/* Fock slice per group size bucket (the autotuner's schedule): groups of at
+ * most m_hi[i] padded functions (m_hi ascending; the first bucket that holds
+ * the group) take slices[i] points per Fock slice (multiples of 128), larger
+ * groups the batch's fock_slice; nb = 0 clears the buckets.  Reloads. */
+int xcq_batch_set_bucket_slices(xcq_batch* b, int nb, const int* m_hi, const int* slices);
+/* Padded function and point counts of every group of the batch (molecule
+ * order); *ngroups gets the count, at most max entries are written. */
+int xcq_batch_group_sizes(xcq_batch* b, int* m_pad, long long* npts_pad, int max, int* ngroups);
+int xcq_batch_set_persistent(xcq_batch* b, int on);
 
 #ifdef __cplusplus
 }

@@ -561,6 +561,49 @@ int xcq_batch_set_schedule(xcq_batch* b, int fock_slice, int density_variant) {
   });
 }
 
+int xcq_batch_set_bucket_slices(xcq_batch* b, int nb, const int* m_hi, const int* slices) {
+  return guarded([&] {
+    if (nb < 0 || (nb > 0 && (!m_hi || !slices))) throw std::runtime_error("g2g: bad buckets");
+    xcq::Tuning t = b->engine->tuning();
+    t.bucket_mhi.assign(m_hi, m_hi + nb);
+    t.bucket_slice.assign(slices, slices + nb);
+    b->engine->set_tuning(t);
+    load_batch(b);
+  });
+}
+
+int xcq_batch_group_sizes(xcq_batch* b, int* m_pad, long long* npts_pad, int max, int* ngroups) {
+  return guarded([&] {
+    int n = 0;
+    for (const xcq::GridPartition& p : b->parts) {
+      for (int g = 0; g < p.ngroups; ++g, ++n) {
+        if (n >= max) continue;
+        const int align = b->prec == xcq::Precision::BF16X6 ? 32 : 16;
+        if (m_pad) m_pad[n] = (p.m[g] + align - 1) / align * align;
+        if (npts_pad) npts_pad[n] = (p.npts[g] + 15) / 16 * 16;
+      }
+    }
+    if (ngroups) *ngroups = n;
+  });
+}
+
+int xcq_set_persistent(xcq_context* ctx, int on) {
+  return guarded([&] {
+    xcq::Tuning t = ctx->engine->tuning();
+    t.persistent = on ? 1 : 0;
+    ctx->engine->set_tuning(t);
+  });
+}
+
+int xcq_batch_set_persistent(xcq_batch* b, int on) {
+  return guarded([&] {
+    xcq::Tuning t = b->engine->tuning();
+    t.persistent = on ? 1 : 0;
+    b->engine->set_tuning(t);
+    load_batch(b);
+  });
+}
+
 int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, const int* slices,
                    const int* variants, int ncand, double* ms_out, int* best) {
   return guarded([&] {

@@ -85,7 +85,7 @@ struct DevGroup {
   long long val_off;   // value planes: nval x m_pad x npts_pad
   long long pg_off;    // P_g / F_g block: m_pad x m_pad
   int fock_base;       // first Fock item
-  int pad_;
+  int slen;            // points per Fock slice (multiple of 128; per size bucket)
   long long phi_off;   // function-major phi plane (split-BF16 mode): m_pad x npts_pad
   int mol;             // molecule of the batch
   int pad2_;
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
                                                  const DevGroup* __restrict__ groups,
                                                  const T* __restrict__ vals,
                                                  const double* __restrict__ factor,
-                                                 double* __restrict__ fpart, int slen) {
+                                                 double* __restrict__ fpart) {
   typedef typename Mma<T>::acc A4;
   typedef T V4 __attribute__((ext_vector_type(4)));
   // FP32 products are summed over at most 32 points in the matrix unit and
@@ -500,6 +500,7 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
   const bool diag = it.ti == it.tj;
+  const int slen = g.slen;
   const int pb = it.slice * slen + wave * (slen / 4);
   const int pe = min(pb + slen / 4, np);
   const T* V = vals + g.val_off;
@@ -667,8 +668,14 @@ __device__ inline void load8(const float* p, float (&x)[8]) {
 // per plane in one 16-byte store.  The tile's phi values also go to the
 // function-major phi plane the Fock kernel reads (rows of 64 points).
 constexpr int kPmPts = 64, kPmFns = 16;
-// LO: one more plane after the NV planes holds phi's remainder,
-// (float)(phi - (double)(float)phi), so hi + lo carries phi to about 2^-48.
+// LO: lo16 (point-major, the planes' layout, group blocks at phi_off) gets
+// phi's remainder phi - (double)(float)phi rounded to BF16 (nearest even).
+__device__ inline unsigned short float_to_bf16(float f) {
+  unsigned u = __builtin_bit_cast(unsigned, f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
 template <int NV, bool LO = false>
 __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
@@ -678,7 +685,8 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
                                                  const double* __restrict__ py,
                                                  const double* __restrict__ pz,
                                                  float* __restrict__ vals,
-                                                 float* __restrict__ phiT) {
+                                                 float* __restrict__ phiT,
+                                                 unsigned short* __restrict__ lo16) {
   constexpr int NP = NV + (LO ? 1 : 0);
   __shared__ float sT[NP][kPmFns][kPmPts + 1];
   const EvalItem it = items[blockIdx.x];
@@ -709,9 +717,17 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
   if (pp >= g.npts_pad) return;
   const long long plane = (long long)g.m_pad * g.npts_pad;
   float* out = vals + g.val_off + (long long)pp * g.m_pad + it.k + wf;
-  for (int c = 0; c < NP; ++c)
+  for (int c = 0; c < NV; ++c)
     *reinterpret_cast<fl4*>(out + c * plane) =
         fl4{sT[c][wf][wp], sT[c][wf + 1][wp], sT[c][wf + 2][wp], sT[c][wf + 3][wp]};
+  if (LO) {
+    unsigned short* o16 = lo16 + g.phi_off + (long long)pp * g.m_pad + it.k + wf;
+    const unsigned a = float_to_bf16(sT[NP - 1][wf][wp]) |
+                       ((unsigned)float_to_bf16(sT[NP - 1][wf + 1][wp]) << 16);
+    const unsigned b = float_to_bf16(sT[NP - 1][wf + 2][wp]) |
+                       ((unsigned)float_to_bf16(sT[NP - 1][wf + 3][wp]) << 16);
+    *reinterpret_cast<uint2*>(o16) = uint2{a, b};
+  }
 }
 
 // Radial sums of every shell of a group at every point (radial-Hessian
@@ -788,13 +804,13 @@ __device__ inline void radial_hessian(const FnMeta& mt, float vx, float vy, floa
 // piece carries the next 8 bits (units of 2^(e-32)) for the products that
 // need P to 2^-32 of the block maximum.
 template <int NPC>
-__global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __restrict__ items,
+__device__ __forceinline__ void gather_bf_item(const int bi, const GatherItem* __restrict__ items,
                                                            const DevGroup* __restrict__ groups,
                                                            const DevMol* __restrict__ mols,
                                                            const int* __restrict__ fnl,
                                                            const double* __restrict__ rho,
                                                            unsigned short* __restrict__ pgs) {
-  const GatherItem it = items[blockIdx.x];
+  const GatherItem it = items[bi];
   const DevGroup g = groups[it.group];
   const int e = it.start + threadIdx.x;
   const long long psz = (long long)g.m_pad * g.m_pad;
@@ -820,6 +836,16 @@ __global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __r
   out[psz] = (__bf16)(float)ldexp(a1, ex - 16);
   out[2 * psz] = (__bf16)(float)ldexp(a2, ex - 24);
   if (NPC == 4) out[3 * psz] = (__bf16)(float)ldexp(rint((r2 - a2) * 256.0), ex - 32);
+}
+
+template <int NPC>
+__global__ __launch_bounds__(256) void k_gather_density_bf(const GatherItem* __restrict__ items,
+                                                           const DevGroup* __restrict__ groups,
+                                                           const DevMol* __restrict__ mols,
+                                                           const int* __restrict__ fnl,
+                                                           const double* __restrict__ rho,
+                                                           unsigned short* __restrict__ pgs) {
+  gather_bf_item<NPC>(blockIdx.x, items, groups, mols, fnl, rho, pgs);
 }
 
 // Density + functional on split-BF16 MFMA: same blocking as k_density (64
@@ -964,13 +990,29 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
 // second derivatives are rebuilt from the shells' radial sums (FP32) and the
 // functions' angular factors in the epilogue, which then reads 4 planes
 // instead of 10.
+// BF16 bit patterns to FP32 (exact: the upper half of the FP32 bits).
+__device__ inline void bf16x8_to_float(const u4& p, float (&x)[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x[2 * q] = __builtin_bit_cast(float, p[q] << 16);
+    x[2 * q + 1] = __builtin_bit_cast(float, p[q] & 0xffff0000u);
+  }
+}
+__device__ inline fl4 bf16x4_to_float(const unsigned short* p) {
+  const uint2 u = *reinterpret_cast<const uint2*>(p);
+  return fl4{__builtin_bit_cast(float, u.x << 16), __builtin_bit_cast(float, u.x & 0xffff0000u),
+             __builtin_bit_cast(float, u.y << 16), __builtin_bit_cast(float, u.y & 0xffff0000u)};
+}
+
 __device__ inline void split4f(const float (&hi)[8], const float (&lo)[8], bf8 (&pc)[4]);
 __device__ inline void mma10(const bf8 (&a)[4], const bf8 (&b)[4], f4& hi, f4& lo);
 
 // LO: phi's FP32 rounding, which does not average out over the grid (every
 // point of a radial shell sees the same value of its atom's s functions),
-// is what bounds the electron count of the three-piece form; plane lopl
-// holds its remainder l = phi - h (h = the FP32 plane 0).
+// is what bounds the electron count of the three-piece form; lo16 holds its
+// remainder l = phi - h (h = the FP32 plane 0) as BF16 (point-major, the
+// layout of the planes; eight bits of l carry phi to about 2^-33, and the
+// four-piece split below needs l to 2^-8 of itself).
 // LO == 1 (density variant 8): Y_0 = P_g phi from the pair (h, l) split into
 // four BF16 pieces against four-piece P_g (the ten piece products of weight
 // >= 2^-24, mma10), and the epilogue's phi is h + l in FP64.
@@ -980,8 +1022,17 @@ __device__ inline void mma10(const bf8 (&a)[4], const bf8 (&b)[4], f4& hi, f4& l
 // + O(l^2), and the gradient sums sum_i d_a phi_i (P_g phi)_i gain
 // sum_j l_j (P_g d_a h)_j = l . Y_a, all from values the lane already holds.
 // Y_x, Y_y, Y_z keep the six-product chain in both forms.
-template <int NT, int ROWS, bool RAD = false, int LO = 0>
-__global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const DensityItem* __restrict__ items,
+// LDS of one density block (the persistent build kernel overlays it with the
+// other phases' LDS).
+template <int NT, int ROWS, int LO>
+struct DensBf2Smem {
+  u4 sA[2][(LO == 1 ? 4 : 3) * ROWS * 4];
+  double sE[4][2];
+  double sS[(NT == 4) ? 10 : 1][256];
+};
+
+template <int NT, int ROWS, bool RAD, int LO>
+__device__ __forceinline__ void density_bf2_item(const int bi, const DensityItem* __restrict__ items,
                                                      const DevGroup* __restrict__ groups,
                                                      const unsigned short* __restrict__ pg3,
                                                      const float* __restrict__ vals,
@@ -995,14 +1046,17 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
                                                      const double* __restrict__ pz,
                                                      const FnMeta* __restrict__ fmeta,
                                                      const float* __restrict__ rad,
-                                                     float* __restrict__ ybuf, int lopl) {
+                                                     float* __restrict__ ybuf,
+                                                     const unsigned short* __restrict__ lo16,
+    DensBf2Smem<NT, ROWS, LO>& sm) {
   constexpr int NS = (NT == 4) ? 10 : 1;
   constexpr int RT = ROWS / 16;
   constexpr int NPC = LO == 1 ? 4 : 3;       // P_g pieces
   constexpr int UNITS = NPC * ROWS * 4;      // 16-byte units of one staged tile
-  __shared__ u4 sA[2][UNITS];              // [buf][piece][row][k octet]
-  __shared__ double sE[4][2];
-  const DensityItem it = items[blockIdx.x];
+  auto& sA = sm.sA;                          // [buf][piece][row][k octet]
+  auto& sE = sm.sE;
+  auto& sS = sm.sS;
+  const DensityItem it = items[bi];
   const DevGroup g = groups[it.group];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lrow = lane & 15, lk = lane >> 4;
@@ -1013,7 +1067,7 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
   const long long psz = (long long)mp * mp;
   const int pt = active ? p0 + lrow : it.p0;  // inactive waves still stage tiles
   const float* Vp = vals + g.val_off + (long long)pt * mp;  // this lane's point
-  const float* Vl = Vp + (long long)lopl * plane;            // phi's remainder (LO)
+  const unsigned short* Vl = lo16 + g.phi_off + (long long)pt * mp;  // phi's remainder (LO)
   const unsigned short* Pp = pg3 + NPC * g.pg_off;
   double ppx = 0.0, ppy = 0.0, ppz = 0.0;
   if constexpr (RAD) {
@@ -1021,7 +1075,6 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
     ppy = py[g.pt_off + pt];
     ppz = pz[g.pt_off + pt];
   }
-  __shared__ double sS[NS][256];
   for (int k = 0; k < NS; ++k) sS[k][threadIdx.x] = 0.0;
   // staging map: 16-byte unit c = threadIdx.x + 256 i of a tile is piece
   // c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4, copied from global
@@ -1049,6 +1102,8 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
     float xb[NT][8];
 #pragma unroll
     for (int t = 0; t < NT; ++t) load8(Vp + t * plane + 8 * lk, xb[t]);
+    u4 xlp = u4{0, 0, 0, 0};  // phi's remainder of the step, 8 BF16 (LO == 1), loaded ahead
+    if constexpr (LO == 1) xlp = *reinterpret_cast<const u4*>(Vl + 8 * lk);
     dma(c0, 0, 0);
     __syncthreads();
     for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
@@ -1057,9 +1112,9 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       const int kk = k0 + 8 * lk;
       bf8 bh[NT], bm[NT], bl[NT];
       bf8 b4[LO == 1 ? 4 : 1];
-      if constexpr (LO == 1) {  // phi's remainder of this step (not loaded ahead: registers)
+      if constexpr (LO == 1) {
         float xl[8];
-        load8(Vl + kk, xl);
+        bf16x8_to_float(xlp, xl);
         split4f(xb[0], xl, b4);
       }
 #pragma unroll
@@ -1067,6 +1122,7 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       if (more) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) load8(Vp + t * plane + kk + 32, xb[t]);
+        if constexpr (LO == 1) xlp = *reinterpret_cast<const u4*>(Vl + kk + 32);
       }
       if (active) {
 #pragma unroll
@@ -1127,7 +1183,7 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
           for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
         }
         fl4 vlo = fl4{0, 0, 0, 0};
-        if constexpr (LO != 0) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
+        if constexpr (LO != 0) vlo = bf16x4_to_float(Vl + c0 + 16 * rt + 4 * lk);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
@@ -1155,7 +1211,7 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
       } else {
         const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
         fl4 vlo = fl4{0, 0, 0, 0};
-        if constexpr (LO != 0) vlo = *reinterpret_cast<const fl4*>(Vl + c0 + 16 * rt + 4 * lk);
+        if constexpr (LO != 0) vlo = bf16x4_to_float(Vl + c0 + 16 * rt + 4 * lk);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
           s[0] += ((double)v0[i] + (LO == 2 ? 2.0 : 1.0) * (double)vlo[i]) *
@@ -1188,9 +1244,31 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    epart[2 * blockIdx.x] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
-    epart[2 * blockIdx.x + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
+    epart[2 * bi] = ((sE[0][0] + sE[1][0]) + sE[2][0]) + sE[3][0];
+    epart[2 * bi + 1] = ((sE[0][1] + sE[1][1]) + sE[2][1]) + sE[3][1];
   }
+}
+
+
+template <int NT, int ROWS, bool RAD = false, int LO = 0>
+__global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const DensityItem* __restrict__ items,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const unsigned short* __restrict__ pg3,
+                                                     const float* __restrict__ vals,
+                                                     const double* __restrict__ pw,
+                                                     double* __restrict__ factor,
+                                                     double* __restrict__ epart, int iexch,
+                                                     int spin, double* __restrict__ factor_b,
+                                                     double* __restrict__ sums, long long sstride,
+                                                     const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz,
+                                                     const FnMeta* __restrict__ fmeta,
+                                                     const float* __restrict__ rad,
+                                                     float* __restrict__ ybuf,
+                                                     const unsigned short* __restrict__ lo16) {
+  __shared__ DensBf2Smem<NT, ROWS, LO> sm;
+  density_bf2_item<NT, ROWS, RAD, LO>(blockIdx.x, items, groups, pg3, vals, pw, factor, epart, iexch, spin, factor_b, sums, sstride, px, py, pz, fmeta, rad, ybuf, lo16, sm);
 }
 
 // Density + functional on split-BF16 MFMA, row-split form for groups of
@@ -1360,19 +1438,21 @@ __global__ __launch_bounds__(256, 2) void k_density_bf3(const DensityItem* __res
 // before its products, so the raw values of both steps fit in registers
 // (253 VGPRs, two waves per SIMD).  Each wave sums its 128 points in FP32 in the matrix
 // unit; the four partial tiles are added in FP64 in wave order.
-__global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__ items,
+__device__ __forceinline__ void fock_bf_item(const int bi, const FockItem* __restrict__ items,
                                                   const DevGroup* __restrict__ groups,
                                                   const float* __restrict__ phiT,
                                                   const double* __restrict__ factor,
-                                                  double* __restrict__ fpart, int slen) {
-  __shared__ double sF[kFnChunk * kFnChunk];
-  const FockItem it = items[blockIdx.x];
+                                                  double* __restrict__ fpart,
+                                             double (&sm)[kFnChunk * kFnChunk]) {
+  double* sF = sm;
+  const FockItem it = items[bi];
   const DevGroup g = groups[it.group];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lrow = lane & 15, lk = lane >> 4;
   const int np = g.npts_pad, mp = g.m_pad;
   const int i0 = kFnChunk * it.ti, j0 = kFnChunk * it.tj;
   const int nri = min(4, (mp - i0) / 16), nrj = min(4, (mp - j0) / 16);
   const bool diag = it.ti == it.tj;
+  const int slen = g.slen;
   const int pb = it.slice * slen + wave * (slen / 4);
   const int pe = min(pb + slen / 4, np);
   const float* Phi = phiT + g.phi_off;
@@ -1466,8 +1546,17 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
     }
     __syncthreads();
   }
-  double* out = fpart + (long long)blockIdx.x * kFnChunk * kFnChunk;
+  double* out = fpart + (long long)bi * kFnChunk * kFnChunk;
   for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) out[e] = sF[e];
+}
+
+__global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__ items,
+                                                  const DevGroup* __restrict__ groups,
+                                                  const float* __restrict__ phiT,
+                                                  const double* __restrict__ factor,
+                                                  double* __restrict__ fpart) {
+  __shared__ double sF[kFnChunk * kFnChunk];
+  fock_bf_item(blockIdx.x, items, groups, phiT, factor, fpart, sF);
 }
 
 // ---------------------------------------------------------------------------
@@ -2334,11 +2423,11 @@ __global__ __launch_bounds__(64) void k_force_atom(const int* __restrict__ afn_o
 
 // F_g tile (ti, tj) = sum over slices, in slice order.  Block b of a tile
 // pair handles elements [256 (b % 16), +256) of the 64 x 64 tile.
-__global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__ groups,
+__device__ __forceinline__ void fock_group_item(const int bi, const DevGroup* __restrict__ groups,
                                                     const int2* __restrict__ tiles,
                                                     const double* __restrict__ fpart,
                                                     double* __restrict__ fg) {
-  const int2 gt = tiles[blockIdx.x >> 4];  // x: group, y: tile-pair index
+  const int2 gt = tiles[bi >> 4];  // x: group, y: tile-pair index
   const DevGroup g = groups[gt.x];
   int ti = 0, rem = gt.y;
   while (rem >= g.ntile - ti) {
@@ -2346,7 +2435,7 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
     ++ti;
   }
   const int tj = ti + rem;
-  const int e = ((blockIdx.x & 15) << 8) + threadIdx.x;
+  const int e = ((bi & 15) << 8) + threadIdx.x;
   const int r = e / kFnChunk, c = e % kFnChunk;
   const int i = kFnChunk * ti + r, j = kFnChunk * tj + c;
   if (i >= g.m_pad || j >= g.m_pad) return;
@@ -2355,6 +2444,13 @@ __global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__
 #pragma unroll 8
   for (int sl = 0; sl < g.nslice; ++sl) s += src[(long long)sl * kFnChunk * kFnChunk];
   fg[g.pg_off + (long long)i * g.m_pad + j] = s;
+}
+
+__global__ __launch_bounds__(256) void k_fock_group(const DevGroup* __restrict__ groups,
+                                                    const int2* __restrict__ tiles,
+                                                    const double* __restrict__ fpart,
+                                                    double* __restrict__ fg) {
+  fock_group_item(blockIdx.x, groups, tiles, fpart, fg);
 }
 
 // Packed F: entry (I <= J) sums F_g(li, lj) over the groups holding both
@@ -2449,6 +2545,228 @@ __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epar
   if (threadIdx.x == 0) {
     out[2 * blockIdx.x] = se[0];
     out[2 * blockIdx.x + 1] = sn[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One persistent launch per build (split-BF16 default density, closed shell)
+// ---------------------------------------------------------------------------
+// Every phase of a build -- density expansion, density and functional, Fock
+// tiles, tile sums over slices, packing, energies -- as items of one queue,
+// taken in order by a fixed set of resident blocks (atomic head counter).
+// An item waits only for items earlier in the queue: a density block for
+// its group's expansion blocks, a Fock tile slice for its group's density
+// blocks, a tile sum for its slices, the packing for every tile sum, an
+// energy for every density block.  An item is taken only after every item
+// before it was taken by a running block that does not wait for anything
+// later, so every wait ends (no co-residency of the whole grid is needed,
+// and builds on other streams may share the device).  Producers publish
+// with an agent-scope release after the block's stores, consumers poll with
+// relaxed loads and take an agent-scope acquire before reading.  A wait
+// that has not ended after about 2^26 polls sets the error word and the
+// energies come out NaN (a safety net; the order above rules it out).
+// Every item computes exactly what its block computes in the phase
+// launches, with the same item index, so the results are bitwise the same.
+
+struct PersistArgs {
+  int n0, n1, n2, n3, n4, n5;  // queue: [0,n0) expansion, [n0,n1) density, [n1,n2) Fock,
+                               // [n2,n3) tile sums (16 per tile pair), [n3,n4) packing,
+                               // [n4,n5) energies
+  int* sync;                   // head, error, tile sums done, density done, then per group
+                               // expansion done, density done, then per tile pair slices done
+  int ngroups, ntiles;
+  const int* gneed;            // expansion blocks per group
+  const int* dneed;            // density blocks per group
+  const int* tbase;            // first tile pair (gtiles index) of each group
+  int fgneed, dallneed;
+  const GatherItem* gitems;
+  const DevGroup* groups;
+  const DevMol* mols;
+  const int* fnl;
+  unsigned short* pg;
+  const DensityItem* ditems;
+  const float* vals;
+  const double* pw;
+  double* factor;
+  double* epart;
+  int iexch;
+  const double *px, *py, *pz;
+  const FnMeta* fmeta;
+  const float* rad;
+  const unsigned short* lo16;
+  const FockItem* fitems;
+  const int* fperm;            // queue order of the Fock items (heavy groups first)
+  const float* phiT;
+  double* fpart;
+  const int2* gtiles;
+  const int* tperm;            // queue order of the tile-sum items
+  double* fg;
+  const int2* pitems;          // packing: (batch function I, 256-wide block of J)
+  const int* memb_off;
+  const int2* memb;
+  const int* fn_mol;
+  const int* eidx;
+  double* eout;
+};
+
+// Once a wait has timed out every other wait gives up at once, so a failed
+// build drains quickly (its energies are NaN).
+__device__ inline void persist_wait(int* cnt, int need, int* err) {
+  if (threadIdx.x == 0) {
+    long long spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+      __builtin_amdgcn_s_sleep(2);
+      if ((++spins & 1023) == 0 &&
+          __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        break;
+      if (spins > (1LL << 26)) {
+        __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+__device__ inline void persist_signal(int* cnt) {
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Each phase of the persistent build is its own (not inlined) function, so
+// each is register-allocated alone, as in its phase kernel: inlined into one
+// loop, values the phases share got hoisted and stayed live through the
+// density's 256-register body (100+ registers spilled).
+template <int NT>
+__device__ __attribute__((noinline)) void persist_density(const PersistArgs* pa, int d,
+                                                          float* ybuf,
+                                                          DensBf2Smem<NT, 64, 1>* sm) {
+  const PersistArgs& a = *pa;
+  density_bf2_item<NT, 64, false, 1>(d, a.ditems, a.groups, a.pg, a.vals, a.pw, a.factor, a.epart,
+                                     a.iexch, 0, nullptr, nullptr, 0, a.px, a.py, a.pz, a.fmeta,
+                                     a.rad, ybuf, a.lo16, *sm);
+}
+
+__device__ __attribute__((noinline)) void persist_fock(const PersistArgs* pa, int f,
+                                                       double (*sm)[kFnChunk * kFnChunk]) {
+  const PersistArgs& a = *pa;
+  fock_bf_item(f, a.fitems, a.groups, a.phiT, a.factor, a.fpart, *sm);
+}
+
+union PersistSmem {
+  DensBf2Smem<4, 64, 1> d4;
+  DensBf2Smem<1, 64, 1> d1;
+  double f[kFnChunk * kFnChunk];
+  double e[2][1024];
+};
+
+// The grid's tables live in device memory (written once per grid load) and
+// are read again for every item; the buffers of the call (packed rho and F,
+// the force pass's Y) are kernel arguments, so a captured graph records them.
+template <int NT>
+__global__ __launch_bounds__(256, 2) void k_build_persist(PersistArgs* pargs, int* sync,
+                                                         const double* rho, double* F,
+                                                         float* ybuf) {
+  __shared__ PersistSmem sm;
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = __hip_atomic_fetch_add(sync, 1, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    const PersistArgs& a = *pargs;
+    int* err = a.sync + 1;
+    int* fgdone = a.sync + 2;
+    int* dall = a.sync + 3;
+    int* gdone = a.sync + 4;
+    int* ddone = gdone + a.ngroups;
+    int* tdone = ddone + a.ngroups;
+    if (it >= a.n5) break;
+    if (it < a.n0) {
+      gather_bf_item<4>(it, a.gitems, a.groups, a.mols, a.fnl, rho, a.pg);
+      persist_signal(&gdone[a.gitems[it].group]);
+    } else if (it < a.n1) {
+      const int d = it - a.n0, g = a.ditems[d].group;
+      persist_wait(&gdone[g], a.gneed[g], err);
+      if constexpr (NT == 4)
+        persist_density<4>(pargs, d, ybuf, &sm.d4);
+      else
+        persist_density<1>(pargs, d, ybuf, &sm.d1);
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(&ddone[g], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(dall, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (it < a.n2) {
+      const int f = a.fperm[it - a.n1];
+      const FockItem fi = a.fitems[f];
+      persist_wait(&ddone[fi.group], a.dneed[fi.group], err);
+      persist_fock(pargs, f, &sm.f);
+      const DevGroup& G = a.groups[fi.group];
+      persist_signal(&tdone[a.tbase[fi.group] + (f - G.fock_base) / G.nslice]);
+    } else if (it < a.n3) {
+      const int q = a.tperm[it - a.n2], t = q >> 4;
+      persist_wait(&tdone[t], a.groups[a.gtiles[t].x].nslice, err);
+      fock_group_item(q, a.groups, a.gtiles, a.fpart, a.fg);
+      persist_signal(fgdone);
+    } else if (it < a.n4) {
+      persist_wait(fgdone, a.fgneed, err);
+      const int2 pi = a.pitems[it - a.n3];
+      const int I = pi.x;
+      const DevMol mol = a.mols[a.fn_mol[I]];
+      const int J = mol.fn_base + pi.y * 256 + (int)threadIdx.x;
+      if (J < mol.fn_base + mol.M && J >= I) {
+        int u0 = a.memb_off[I], ue = a.memb_off[I + 1];
+        int v0 = a.memb_off[J], ve = a.memb_off[J + 1];
+        double sum = 0.0;
+        while (u0 < ue && v0 < ve) {
+          const int2 u = a.memb[u0], v = a.memb[v0];
+          if (u.x < v.x) {
+            ++u0;
+          } else if (v.x < u.x) {
+            ++v0;
+          } else {
+            const DevGroup gg = a.groups[u.x];
+            const int li = min(u.y, v.y), lj = max(u.y, v.y);
+            sum += a.fg[gg.pg_off + (long long)li * gg.m_pad + lj];
+            ++u0;
+            ++v0;
+          }
+        }
+        F[mol.rho_off + packed_index(I - mol.fn_base, J - mol.fn_base, mol.M)] = sum;
+      }
+    } else {
+      // energies: the 1024-lane fixed-order reduction of the phase launch,
+      // four lanes per thread
+      persist_wait(dall, a.dallneed, err);
+      const int m = it - a.n4;
+      const DevMol mol = a.mols[m];
+      for (int v = threadIdx.x; v < 1024; v += 256) {
+        double e = 0.0, c = 0.0;
+        for (int i = mol.eb + v; i < mol.ee; i += 1024) {
+          e += a.epart[2 * a.eidx[i]];
+          c += a.epart[2 * a.eidx[i] + 1];
+        }
+        sm.e[0][v] = e;
+        sm.e[1][v] = c;
+      }
+      __syncthreads();
+      for (int st = 512; st > 0; st >>= 1) {
+        for (int v = threadIdx.x; v < st; v += 256) {
+          sm.e[0][v] += sm.e[0][v + st];
+          sm.e[1][v] += sm.e[1][v + st];
+        }
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        const bool bad = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        a.eout[2 * m] = bad ? __builtin_nan("") : sm.e[0][0];
+        a.eout[2 * m + 1] = bad ? __builtin_nan("") : sm.e[1][0];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -3023,7 +3341,13 @@ struct DeviceGrid {
   double* fout = nullptr;         // forces, natom x 3 (atom-major)
   bool radial = false;            // radial-Hessian layout (split-BF16 variant 5)
   bool inkernel = false;          // basis evaluated in the density kernel (variants 6, 7)
-  bool lo = false;                // phi's FP32 remainder plane, four-piece Y_0 (variant 8)
+  bool lo = false;                // phi's remainder (lo16), four-piece Y_0 (variants 8, 9)
+  unsigned short* lo16 = nullptr; // phi's remainder in BF16, point-major (phi_elems)
+  bool persistent = false;        // one persistent launch per build (k_build_persist)
+  PersistArgs pargs{};            // its queue and tables (device pointers)
+  PersistArgs* dpargs = nullptr;  // device copy of pargs, read by the kernel
+  int nsync = 0;                  // counters the launch clears first
+  int persist_blocks = 0;         // resident blocks of the persistent launch (2 per CU)
   int2* kbsh = nullptr;           // per 32-function K block: first shell, shell count
   int2* sfn = nullptr;            // per group shell: first local function, function count
   int2* tsh = nullptr;            // per 16-function row tile: first shell, shell count
@@ -3234,6 +3558,13 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       throw std::runtime_error("g2g: the molecules of a batch must share the functional");
   if (tuning_.fock_slice < 128 || tuning_.fock_slice % 128 != 0)
     throw std::runtime_error("g2g: the Fock slice must be a positive multiple of 128 points");
+  if (tuning_.bucket_mhi.size() != tuning_.bucket_slice.size())
+    throw std::runtime_error("g2g: one Fock slice per size bucket");
+  for (size_t i = 0; i < tuning_.bucket_slice.size(); ++i)
+    if (tuning_.bucket_slice[i] < 128 || tuning_.bucket_slice[i] % 128 != 0 ||
+        (i > 0 && tuning_.bucket_mhi[i] <= tuning_.bucket_mhi[i - 1]))
+      throw std::runtime_error("g2g: bucket slices must be positive multiples of 128 points and "
+                               "the bucket bounds ascending");
   auto* dg = new DeviceGrid;
   // until the grid is committed below, every exit frees what it holds (its
   // arena -- possibly the previous grid's, handed over -- and pinned buffers)
@@ -3274,7 +3605,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
            (tuning_.density_variant == 8 || tuning_.density_variant == 9);
   // in-kernel evaluation keeps no value planes (phi alone, function-major,
   // for the Fock kernel); variant 8 one more plane (phi's remainder)
-  dg->nval = dg->inkernel ? 0 : ((dg->gga && !dg->radial) ? 10 : 4) + (dg->lo ? 1 : 0);
+  dg->nval = dg->inkernel ? 0 : (dg->gga && !dg->radial) ? 10 : 4;
   dg->prec = prec;
   const size_t vbytes = prec == Precision::FP64 ? sizeof(double) : sizeof(float);
   const bool bf = prec == Precision::BF16X6;
@@ -3367,11 +3698,11 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     d.pt_off = (int)pt;
     d.fn_off = (int)fnl.size();
     d.ntile = (d.m_pad + kFnChunk - 1) / kFnChunk;
-    d.nslice = (d.npts_pad + tuning_.fock_slice - 1) / tuning_.fock_slice;
+    d.nslice = (d.npts_pad + tuning_.slice_for(d.m_pad) - 1) / tuning_.slice_for(d.m_pad);
     d.val_off = vo;
     d.pg_off = po;
     d.fock_base = fock_base;
-    d.pad_ = 0;
+    d.slen = tuning_.slice_for(d.m_pad);
     d.phi_off = fo;
     d.force_off = go;
     go += (long long)(d.npts_pad + kDensPts - 1) / kDensPts * d.m_pad * 3;
@@ -3516,6 +3847,39 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
           fitems[d.fock_base + pair * d.nslice + s] = FockItem{g, ti, tj, s};
       }
   }
+  // persistent build: per-group item counts, each group's first tile pair,
+  // the packing items and the queue orders of the Fock and tile-sum items
+  // (groups in the density order, heavy first)
+  dg->persistent = tuning_.persistent != 0 && bf && dg->density_variant == 8 && !dg->open;
+  std::vector<int> gneed, dneed, tbase, fperm, tperm;
+  std::vector<int2> pitems;
+  if (dg->persistent) {
+    gneed.assign(G, 0);
+    dneed.assign(G, 0);
+    tbase.assign(G, 0);
+    for (const GatherItem& gi : gitems) gneed[gi.group]++;
+    for (const DensityItem& di : ditems) dneed[di.group]++;
+    for (int g = 0, t = 0; g < G; ++g) {
+      tbase[g] = t;
+      t += groups[g].ntile * (groups[g].ntile + 1) / 2;
+    }
+    for (int g : order) {
+      const DevGroup& d = groups[g];
+      const int pairs = d.ntile * (d.ntile + 1) / 2;
+      for (int q = 0; q < pairs * d.nslice; ++q) fperm.push_back(d.fock_base + q);
+      for (int q = 0; q < pairs * 16; ++q) tperm.push_back(16 * tbase[g] + q);
+    }
+    for (int k = 0; k < nmol; ++k)
+      for (int I = 0; I < hmol[k].M; ++I)
+        for (int xb = I / 256; xb * 256 < hmol[k].M; ++xb)
+          pitems.push_back(make_int2(hmol[k].fn_base + I, xb));
+  }
+  dg->nsync = dg->persistent ? 4 + 2 * G + (int)gtiles.size() : 0;
+  if (dg->persistent) {
+    int cus = 0;
+    XCQ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+    dg->persist_blocks = 2 * std::max(1, cus);
+  }
   dg->neval = (int)eitems.size();
   dg->ngather = (int)gitems.size();
   dg->npm = (int)pmitems.size();
@@ -3577,6 +3941,16 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->tsh = c.take<int2>(tsh.size() + 1);
     dg->kbsh = c.take<int2>(kbsh.size() + 1);
     dg->sfn = c.take<int2>(sfn.size() + 1);
+    dg->lo16 = c.take<unsigned short>(dg->lo ? fo : 0);
+    PersistArgs& pa = dg->pargs;
+    dg->dpargs = c.take<PersistArgs>(1);
+    pa.sync = c.take<int>(dg->nsync + 1);
+    pa.gneed = c.take<int>(gneed.size() + 1);
+    pa.dneed = c.take<int>(dneed.size() + 1);
+    pa.tbase = c.take<int>(tbase.size() + 1);
+    pa.fperm = c.take<int>(fperm.size() + 1);
+    pa.tperm = c.take<int>(tperm.size() + 1);
+    pa.pitems = c.take<int2>(pitems.size() + 1);
   };
   Carver sizing{nullptr};
   plan(sizing);
@@ -3683,6 +4057,53 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->sfn, sfn.data(), sfn.size() * sizeof(int2));
   up(dg->afn_off, afn_off.data(), afn_off.size() * sizeof(int));
   up(dg->afn, afn.data(), afn.size() * sizeof(int));
+  if (dg->persistent) {
+    PersistArgs& pa = dg->pargs;
+    up(const_cast<int*>(pa.gneed), gneed.data(), gneed.size() * sizeof(int));
+    up(const_cast<int*>(pa.dneed), dneed.data(), dneed.size() * sizeof(int));
+    up(const_cast<int*>(pa.tbase), tbase.data(), tbase.size() * sizeof(int));
+    up(const_cast<int*>(pa.fperm), fperm.data(), fperm.size() * sizeof(int));
+    up(const_cast<int*>(pa.tperm), tperm.data(), tperm.size() * sizeof(int));
+    up(const_cast<int2*>(pa.pitems), pitems.data(), pitems.size() * sizeof(int2));
+    pa.n0 = (int)gitems.size();
+    pa.n1 = pa.n0 + (int)ditems.size();
+    pa.n2 = pa.n1 + (int)fitems.size();
+    pa.n3 = pa.n2 + 16 * (int)gtiles.size();
+    pa.n4 = pa.n3 + (int)pitems.size();
+    pa.n5 = pa.n4 + nmol;
+    pa.ngroups = G;
+    pa.ntiles = (int)gtiles.size();
+    pa.fgneed = 16 * (int)gtiles.size();
+    pa.dallneed = (int)ditems.size();
+    pa.gitems = dg->gitems;
+    pa.groups = dg->groups;
+    pa.mols = dg->mols;
+    pa.fnl = dg->fnl;
+    pa.pg = static_cast<unsigned short*>(dg->pg);
+    pa.ditems = dg->ditems;
+    pa.vals = static_cast<const float*>(dg->vals);
+    pa.pw = dg->pw;
+    pa.factor = dg->factor;
+    pa.epart = dg->epart;
+    pa.iexch = dg->iexch;
+    pa.px = dg->px;
+    pa.py = dg->py;
+    pa.pz = dg->pz;
+    pa.fmeta = dg->fmeta;
+    pa.rad = dg->rad;
+    pa.lo16 = dg->lo16;
+    pa.fitems = dg->fitems;
+    pa.phiT = dg->phiT;
+    pa.fpart = dg->fpart;
+    pa.gtiles = dg->gtiles;
+    pa.fg = dg->fg;
+    pa.memb_off = dg->memb_off;
+    pa.memb = dg->memb;
+    pa.fn_mol = dg->fn_mol;
+    pa.eidx = dg->eidx;
+    pa.eout = dg->eout;
+    up(dg->dpargs, &pa, sizeof(PersistArgs));
+  }
   if (dpts) {  // points come from the device rebuild: compact them into place
     if ((int)dpts->src.size() != G) throw std::runtime_error("g2g: staged group map mismatch");
     std::vector<CompactItem> ci(G);
@@ -3727,16 +4148,16 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       if (dg->npm > 0) {
         if (dg->lo && dg->gga)
           hipLaunchKernelGGL((k_eval_pm<10, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
-                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
+                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
         else if (dg->lo)
           hipLaunchKernelGGL((k_eval_pm<4, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
-                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
+                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
         else if (dg->gga && !dg->radial)
           hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
-                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
+                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
         else
           hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
-                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT);
+                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
       }
       if (dg->nrad > 0)
         hipLaunchKernelGGL(k_eval_radial, dim3(dg->nrad), block, 0, s, dg->ritems, dg->groups,
@@ -4314,12 +4735,12 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
     }
     if (spin < 2) mark(ev, 1, s);
     float* y = yb ? yb + (spin == 2 ? g.phi_elems : 0) : nullptr;
-    const int lopl = g.nval - 1;
+
     if (g.ndensity > 0) {
 #define XCQ_LO(NT, L)                                                                           \
   hipLaunchKernelGGL((k_density_bf2<NT, 64, false, L>), dim3(g.ndensity), dim3(256), 0, s,        \
                      g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,        \
-                     g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y, lopl)
+                     g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, y, g.lo16)
       if (g.gga && four)
         XCQ_LO(4, 1);
       else if (g.gga)
@@ -4342,29 +4763,29 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, nullptr);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 128>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, nullptr);
     } else if (g.density_variant == 2) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, nullptr);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(g.ndensity), dim3(256), 0, s, g.ditems,
                            g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, nullptr);
     } else if (g.density_variant == 5) {
       if (g.gga)
         hipLaunchKernelGGL((k_density_bf2<4, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
                            g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, nullptr);
       else
         hipLaunchKernelGGL((k_density_bf2<1, 64, true>), dim3(g.ndensity), dim3(256), 0, s,
                            g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,
-                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, 0);
+                           g.factor_b, g.sums, g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, yb, nullptr);
     } else if (g.density_variant == 3 || g.density_variant == 4) {
       const int nr = g.nrowsplit, ns = g.ndensity - nr;
       const DensityItem* si = g.ditems + nr;
@@ -4379,7 +4800,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<4, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr, 0);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr, nullptr);
       } else {
         if (nr > 0)
           hipLaunchKernelGGL((k_density_bf3<1, 6>), dim3(nr), dim3(256), 0, s, g.ditems, g.groups,
@@ -4387,7 +4808,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
         if (ns > 0)
           hipLaunchKernelGGL((k_density_bf2<1, 64>), dim3(ns), dim3(256), 0, s, si, g.groups, pg3,
                              vals, g.pw, g.factor, se, g.iexch, 0, g.factor_b, g.sums,
-                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr, 0);
+                             g.npts_pad, g.px, g.py, g.pz, g.fmeta, g.rad, (float*)nullptr, nullptr);
       }
     } else {
       if (g.gga)
@@ -4406,13 +4827,13 @@ static void launch_fock(DeviceGrid& g, hipStream_t s, const double* factor) {
   if (g.nfock <= 0) return;
   if (g.prec == Precision::FP64)
     hipLaunchKernelGGL(k_fock<double>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups,
-                       static_cast<const double*>(g.vals), factor, g.fpart, g.fock_slice);
+                       static_cast<const double*>(g.vals), factor, g.fpart);
   else if (g.prec == Precision::FP32)
     hipLaunchKernelGGL(k_fock<float>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups,
-                       static_cast<const float*>(g.vals), factor, g.fpart, g.fock_slice);
+                       static_cast<const float*>(g.vals), factor, g.fpart);
   else
     hipLaunchKernelGGL(k_fock_bf, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.phiT,
-                       factor, g.fpart, g.fock_slice);
+                       factor, g.fpart);
   hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups, g.gtiles,
                      g.fpart, g.fg);
   XCQ_HIP(hipGetLastError());
@@ -4477,6 +4898,29 @@ static void launch_forces(DeviceGrid& g, hipStream_t s, double* d_forces) {
 static void launch_all(DeviceGrid& g, hipStream_t s, const double* d_rho, double* d_fock,
                        hipEvent_t* ev, double* d_forces = nullptr) {
   mark(ev, 0, s);
+  if (g.persistent && d_fock) {
+    // one launch for the whole build (the counters cleared by a fill kernel,
+    // not a memset, so the pair can be captured in a graph); the HIP-event
+    // phases then read: expand = clearing, density = the whole build
+    const PersistArgs& pa = g.pargs;
+    float* yb = d_forces && ystore_forces(g) ? g.ybuf : nullptr;
+    hipLaunchKernelGGL(k_fill_int, dim3((g.nsync + 255) / 256), dim3(256), 0, s, pa.sync, g.nsync,
+                       0);
+    mark(ev, 1, s);
+    const int nblocks = std::max(1, std::min(g.persist_blocks, pa.n5));
+    if (g.gga)
+      hipLaunchKernelGGL(k_build_persist<4>, dim3(nblocks), dim3(256), 0, s, g.dpargs, pa.sync,
+                         d_rho, d_fock, yb);
+    else
+      hipLaunchKernelGGL(k_build_persist<1>, dim3(nblocks), dim3(256), 0, s, g.dpargs, pa.sync,
+                         d_rho, d_fock, yb);
+    XCQ_HIP(hipGetLastError());
+    mark(ev, 2, s);
+    mark(ev, 3, s);
+    if (d_forces) launch_forces(g, s, d_forces);
+    mark(ev, 4, s);
+    return;
+  }
   const int nspin = g.open ? 2 : 1;
   // closed-shell split-BF16 forces take Y = P_g phi from the density pass
   const bool ystore = d_forces && ystore_forces(g);

@@ -193,9 +193,25 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 //               Variants 0-5 and 9 leave |dN| up to about 1e-6 at a few
 //               hundred electrons; 6-8 keep it near 1e-8 (measured errors
 //               against FP64 are recorded by the tests).
+//   persistent: 1 = the whole closed-shell build of the split-BF16 default
+//               as one persistent launch over a queue of every phase's items
+//               (k_build_persist; dependencies by per-group counters), 0 =
+//               one launch per phase.  Bitwise the same results.
+//   bucket_mhi / bucket_slice: a Fock slice per group size bucket: groups
+//               of at most bucket_mhi[i] padded functions (ascending; the
+//               first bucket that holds the group) take bucket_slice[i];
+//               larger groups, and every group when the lists are empty,
+//               take fock_slice.
 struct Tuning {
   int fock_slice = 1024;
   int density_variant = 8;
+  int persistent = 0;
+  std::vector<int> bucket_mhi, bucket_slice;
+  int slice_for(int m_pad) const {
+    for (size_t i = 0; i < bucket_mhi.size(); ++i)
+      if (m_pad <= bucket_mhi[i]) return bucket_slice[i];
+    return fock_slice;
+  }
 };
 
 struct SolveResult {

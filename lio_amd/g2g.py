@@ -17,7 +17,9 @@ The shared library must be built (``__graft_entry__.build()`` or
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
+import time
 
 import numpy as np
 
@@ -123,6 +125,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     L.xcq_reserve_slab.argtypes = [C.c_int, C.c_ulonglong]
     L.xcq_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
+    L.xcq_set_persistent.argtypes = [vp, C.c_int]
+    L.xcq_batch_set_persistent.argtypes = [vp, C.c_int]
+    L.xcq_batch_set_bucket_slices.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.xcq_batch_group_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.c_int,
+                                        C.POINTER(C.c_int)]
     _LIB = L
     return L
 
@@ -341,6 +348,11 @@ class Context:
         """Kernel schedule for the next build_grid (0 / -1 keep a knob)."""
         _check(self.L.xcq_set_schedule(self.h, fock_slice, density_variant), "xcq_set_schedule")
 
+    def set_persistent(self, on: bool):
+        """One persistent launch per build (split-BF16 default, closed shell)
+        from the next build_grid; bitwise the results of the phase launches."""
+        _check(self.L.xcq_set_persistent(self.h, int(bool(on))), "xcq_set_persistent")
+
     def release_device(self):
         """Frees the device memory of this molecule; build_grid restores it."""
         _check(self.L.xcq_release_device(self.h), "xcq_release_device")
@@ -554,6 +566,11 @@ class Batch:
         _check(self.L.xcq_batch_set_schedule(self.h, fock_slice, density_variant),
                "xcq_batch_set_schedule")
 
+    def set_persistent(self, on: bool):
+        """One persistent launch per build (split-BF16 default, closed shell);
+        reloads.  Bitwise the results of the phase launches."""
+        _check(self.L.xcq_batch_set_persistent(self.h, int(bool(on))), "xcq_batch_set_persistent")
+
     def tune(self, d_rho: int, d_fock: int, slices=(256, 512, 1024, 2048), variants=None,
              reps: int = 5):
         """Autotunes the schedule on these buffers over the candidates
@@ -573,6 +590,109 @@ class Batch:
         keys = [(int(a), None if va is None else int(b))
                 for a, b in zip(sl, va if va is not None else sl)]
         return keys[best.value], dict(zip(keys, ms.tolist()))
+
+    def group_sizes(self):
+        """(padded function count, padded point count) of every group."""
+        n = C.c_int(0)
+        _check(self.L.xcq_batch_group_sizes(self.h, None, None, 0, C.byref(n)),
+               "xcq_batch_group_sizes")
+        m = np.zeros(n.value, np.int32)
+        p = np.zeros(n.value, np.int64)
+        _check(self.L.xcq_batch_group_sizes(self.h, m.ctypes.data_as(C.POINTER(C.c_int)),
+                                            p.ctypes.data_as(C.POINTER(C.c_longlong)), n.value,
+                                            C.byref(n)), "xcq_batch_group_sizes")
+        return m, p
+
+    def set_bucket_slices(self, m_hi, slices):
+        """Fock slice per group size bucket (groups of at most m_hi[i] padded
+        functions take slices[i]); reloads."""
+        mh = np.ascontiguousarray(m_hi, np.int32)
+        sl = np.ascontiguousarray(slices, np.int32)
+        if len(mh) != len(sl):
+            raise ValueError("one slice per bucket")
+        _check(self.L.xcq_batch_set_bucket_slices(self.h, len(mh),
+                                                  mh.ctypes.data_as(C.POINTER(C.c_int)),
+                                                  sl.ctypes.data_as(C.POINTER(C.c_int))),
+               "xcq_batch_set_bucket_slices")
+
+    def _time_builds(self, d_rho: int, d_fock: int, reps: int) -> float:
+        self.solve_device(d_rho, d_fock)  # warm-up
+        self.energies()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.solve_device(d_rho, d_fock)
+        self.energies()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    def autotune(self, d_rho: int, d_fock: int, path: str | None = None,
+                 buckets=(64, 128, 192, 256, 384), slices=(256, 512, 1024, 2048),
+                 reps: int = 3, persistent=(False, True)) -> dict:
+        """Schedule of this batch per group size bucket, persisted as JSON.
+
+        With ``path`` naming an existing schedule file written for the same
+        bucket bounds and precision, the schedule is applied as it is.
+        Otherwise: the Fock slice of each bucket (groups of at most
+        buckets[i] padded functions; larger groups form the last bucket) is
+        chosen by timing whole builds on these buffers, one bucket at a time,
+        heaviest bucket (sum of n m^2) first, the others held at their best so
+        far; then, in split-BF16 mode, one persistent launch against one
+        launch per phase.  The result is written to ``path`` (when given) and
+        returned: bucket bounds and slices, the launch form, the ms per build
+        of every candidate timed."""
+        bounds = [int(b) for b in buckets]
+        if path and os.path.exists(path):
+            with open(path) as f:
+                sched = json.load(f)
+            if sched.get("bucket_mhi", [])[:len(bounds)] == bounds and \
+                    sched.get("precision") == self.precision:
+                self.set_bucket_slices(sched["bucket_mhi"], sched["bucket_slice"])
+                self.set_persistent(bool(sched.get("persistent")))
+                sched["loaded"] = True
+                return sched
+        m, p = self.group_sizes()
+        nb = len(bounds) + 1
+        edges = bounds + [int(max(m.max(initial=0), bounds[-1] + 32))]
+        which = np.searchsorted(np.array(edges), m, side="left")
+        work = np.zeros(nb)
+        np.add.at(work, which, p.astype(np.float64) * m.astype(np.float64) ** 2)
+        choice = [1024] * nb
+        table = {}
+        self.set_persistent(False)
+        self.set_bucket_slices(edges, choice)
+        best = self._time_builds(d_rho, d_fock, reps)
+        table["start"] = best
+        for b in np.argsort(-work):
+            if work[b] == 0:
+                continue
+            for sl in slices:
+                if sl == choice[b]:
+                    continue
+                trial = list(choice)
+                trial[b] = int(sl)
+                self.set_bucket_slices(edges, trial)
+                ms = self._time_builds(d_rho, d_fock, reps)
+                table["bucket<=%d slice %d" % (edges[b], sl)] = ms
+                if ms < best:
+                    best, choice = ms, trial
+        self.set_bucket_slices(edges, choice)
+        use_persistent = False
+        if self.precision == "bf16x6" and len(persistent) > 1:
+            self.set_persistent(True)
+            ms = self._time_builds(d_rho, d_fock, reps)
+            table["persistent"] = ms
+            if ms < best:
+                best, use_persistent = ms, True
+            else:
+                self.set_persistent(False)
+        sched = {"bucket_mhi": edges, "bucket_slice": [int(c) for c in choice],
+                 "persistent": use_persistent, "precision": self.precision, "ms": best,
+                 "bucket_groups": np.bincount(which, minlength=nb).tolist(),
+                 "bucket_work": work.tolist(), "table": table, "loaded": False}
+        if path:
+            os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+            with open(path, "w") as f:
+                json.dump(sched, f, indent=1)
+        return sched
 
     def close(self):
         if getattr(self, "h", None):
