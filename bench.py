@@ -56,8 +56,8 @@ DENSITY_KERNEL = "k_density_bf2<4, 64, false, 1>"
 # HBM bytes per launch of DENSITY_KERNEL on (H2O)32: rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0 in KiB (copy-kernel calibrated), measured on the tree
 # named in DENSITY_TRAFFIC_TREE (profiles/xc_fock_h2o32.md gives the command)
-DENSITY_TRAFFIC = None
-DENSITY_TRAFFIC_TREE = None
+DENSITY_TRAFFIC = (3861543 * 2.0 + 178912 * 1.0) * 1024
+DENSITY_TRAFFIC_TREE = "34b9bd7"
 
 
 def log(*a):
