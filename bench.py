@@ -52,12 +52,15 @@ MIXED = "bf16x6"
 # BF16 MFMA products per tile product, averaged over the density launch's four
 # products: Y_0 takes ten (four-piece operands), Y_x, Y_y, Y_z six each
 BF16_PRODUCTS = (10 + 3 * 6) / 4
-DENSITY_KERNEL = "k_density_bf2<4, 64, false, 1>"
-# HBM bytes per launch of DENSITY_KERNEL on (H2O)32: rocprofv3 FETCH_SIZE x 2.0
-# + WRITE_SIZE x 1.0 in KiB (copy-kernel calibrated), measured on the tree
-# named in DENSITY_TRAFFIC_TREE (profiles/xc_fock_h2o32.md gives the command)
-DENSITY_TRAFFIC = (3861543 * 2.0 + 178912 * 1.0) * 1024
-DENSITY_TRAFFIC_TREE = "34b9bd7"
+# the density phase of the split-BF16 default (variant 10): the chunk kernel
+# and its finishing launch (sums in chunk order, functional; 2 % of the phase)
+DENSITY_KERNEL = "k_density_cs<4, false> + k_density_cs_finish<4>"
+# HBM bytes per density phase on (H2O)32: rocprofv3 FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 in KiB (copy-kernel calibrated) of both launches, measured
+# on the tree named in DENSITY_TRAFFIC_TREE (profiles/xc_fock_h2o32.md gives
+# the command); None until measured on this kernel
+DENSITY_TRAFFIC = None
+DENSITY_TRAFFIC_TREE = None
 
 
 def log(*a):
@@ -502,8 +505,9 @@ def main():
                      "traffic": DENSITY_TRAFFIC, "traffic_tree": DENSITY_TRAFFIC_TREE,
                      "algorithmic_bytes": rmx["dens_bytes"], "kernel_ms": rmx["phases"][1],
                      "note": "achieved = algorithmic bytes (42 B per point-function pair, 8 B per "
-                             "P_g element, 16 B per point) / HIP-event duration of the launch; "
-                             "traffic = TCC FETCH_SIZE + WRITE_SIZE per launch, calibrated "
+                             "P_g element, 16 B per point) / HIP-event duration of the density "
+                             "phase (both launches); "
+                             "traffic = TCC FETCH_SIZE + WRITE_SIZE per phase, calibrated "
                              "against a plain copy kernel, measured on traffic_tree "
                              "(profiles/xc_fock_h2o32.md)",
                      "mfma_tflops_executed": mfma_tflops,

@@ -230,10 +230,13 @@ int xcq_batch_load_ms(xcq_batch* b, double ms[4]);
  * 128- and 64-function chunks with LDS-staged density pieces, 3 and 4:
  * row-split 16-point blocks for groups above 64 functions, 5: 2 in the
  * radial-Hessian layout, 4 value planes instead of 10, 6 and 7: the basis
- * evaluated in the density kernel, 8 (default): 2 with phi's FP32 remainder
- * plane and four-piece Y_0 products, 9: 2 with the remainder added in the
- * epilogue; 0-5 and 9 are the faster, less accurate forms: |dN| near 1e-6
- * at a few hundred electrons) -- on these buffers
+ * evaluated in the density kernel, 8: 2 with phi's FP32 remainder plane and
+ * four-piece Y_0 products, 9: 2 with the remainder added in the epilogue,
+ * 10 (default): 8 with one block per (64-point tile, 64-function row chunk),
+ * the blocks of a tile consecutive on one XCD so that its points' values
+ * are fetched into that XCD's L2 once for all chunks, 11: 10 with phi's
+ * pieces split once per grid; 0-5 and 9 are the faster, less accurate
+ * forms: |dN| near 1e-6 at a few hundred electrons) -- on these buffers
  * (rebuilding the work lists for each), keeps the fastest and returns the
  * per-candidate ms in ms_out; *best is its index.  A captured graph is
  * dropped. */
@@ -245,13 +248,17 @@ int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps,
  * reloads at once.  No reference counterpart. */
 int xcq_set_schedule(xcq_context* ctx, int fock_slice, int density_variant);
 int xcq_batch_set_schedule(xcq_batch* b, int fock_slice, int density_variant);
-/* on = 1: the whole closed-shell build with the split-BF16 default density
- * kernel runs as one persistent launch over an ordered queue of every phase's
- * items (expansion, density and functional, Fock tile slices, tile sums,
- * packing, energies), dependencies tracked by per-group counters; on = 0
- * (default): one launch per phase.  The results are bitwise the same.  Takes
- * effect at the next xcq_build_grid (a batch reloads at once).  Other modes
- * (FP64, FP32, open shell, other density variants) keep the phase launches. */
+/* The batch's schedule knobs (any pointer may be NULL).  No reference
+ * counterpart. */
+int xcq_batch_get_schedule(xcq_batch* b, int* fock_slice, int* density_variant, int* persistent);
+/* on = 1: a closed-shell split-BF16 build with density variant 8 runs as one
+ * persistent launch over an ordered queue of every phase's items (expansion,
+ * density and functional, Fock tile slices, tile sums, packing, energies),
+ * dependencies tracked by per-group counters; on = 0 (default): one launch
+ * per phase.  The results are bitwise those of variant 8's phase launches.
+ * Takes effect at the next xcq_build_grid (a batch reloads at once).  Other
+ * modes (FP64, FP32, open shell, other density variants, the default 10
+ * included) keep the phase launches. */
 int xcq_set_persistent(xcq_context* ctx, int on);
 /* Fock slice per group size bucket (the autotuner's schedule): groups of at
  * most m_hi[i] padded functions (m_hi ascending; the first bucket that holds

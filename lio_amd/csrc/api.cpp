@@ -604,6 +604,15 @@ int xcq_batch_set_persistent(xcq_batch* b, int on) {
   });
 }
 
+int xcq_batch_get_schedule(xcq_batch* b, int* fock_slice, int* density_variant, int* persistent) {
+  return guarded([&] {
+    const xcq::Tuning& t = b->engine->tuning();
+    if (fock_slice) *fock_slice = t.fock_slice;
+    if (density_variant) *density_variant = t.density_variant;
+    if (persistent) *persistent = t.persistent;
+  });
+}
+
 int xcq_batch_tune(xcq_batch* b, const double* d_rho, double* d_fock, int reps, const int* slices,
                    const int* variants, int ncand, double* ms_out, int* best) {
   return guarded([&] {

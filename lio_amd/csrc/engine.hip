@@ -59,6 +59,7 @@ constexpr int kFnAlign = 16;     // function padding (one MFMA row tile)
 constexpr int kDensPts = 64;     // points per k_density block (16 per wave)
 constexpr int kRowSplitMin = 64; // split-BF16 row-split density above this m_pad
 constexpr int kFnChunk = 64;     // functions per density pass / Fock tile side
+constexpr int kXcds = 8;         // MI355X: XCDs, each with its own L2 (block b runs on b % 8)
 constexpr int kSliceDefault = 1024;  // points per Fock slice (split-K over blocks)
 constexpr int kPackMapMax = 16384;  // groups per molecule the packing kernel maps in LDS
 constexpr int kEvalPts = 256;    // threads per evaluation block
@@ -577,6 +578,17 @@ __global__ __launch_bounds__(256, 2) void k_fock(const FockItem* __restrict__ it
 // split-BF16 (BF16X6) contractions
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f8v __attribute__((ext_vector_type(8)));
+
+// Eight BF16 pieces from their FP32 values (four v_cvt_pk_bf16_f32).  Every
+// piece of the splits below has at most 8 significant bits, so the
+// round-to-nearest conversion is exact: the upper half of the FP32 bits.
+__device__ inline bf8 pack_bf8(const float (&v)[8]) {
+  f8v x;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) x[q] = v[q];
+  return __builtin_convertvector(x, bf8);
+}
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
@@ -585,8 +597,8 @@ typedef float fl4 __attribute__((ext_vector_type(4)));
 // 4 lanes l, l^16, l^32, l^48 that hold the same row or point): they share
 // one exponent e (2^(e-1) <= max |x| < 2^e) and each value becomes integers
 // a0 + a1 2^-8 + a2 2^-16 (|a0| <= 256, |a1|, |a2| <= 128, round to nearest)
-// in units of 2^(e-8), kept as the BF16 pieces a_s 2^(e-8-8s) (exact: at most
-// 9 significant bits).  The error is below 2^-24 of the block maximum.  With
+// in units of 2^(e-8), kept as the BF16 pieces a_s 2^(e-8-8s) (exact: an
+// integer of magnitude at most 256 has at most 8 significant bits).  The error is below 2^-24 of the block maximum.  With
 // both operands split this way every product of one MFMA output is an
 // integer multiple of one unit and their sum (below 2^21 units) is exact in
 // the matrix unit, whose adder otherwise truncates products that lie far
@@ -604,30 +616,20 @@ __device__ inline void split3(const float (&x)[8], bf8& h, bf8& m, bf8& l) {
   // to the nearest multiple of 2^(e-8) (ties to even): the sum lies in
   // [2^(e+15), 2^(e+16)), whose spacing is 2^(e-8).  The same with
   // 1.5 * 2^(e+7) and 1.5 * 2^(e-1) takes the remainders to multiples of
-  // 2^(e-16) and 2^(e-24).  Every piece then has at most 9 significant bits,
+  // 2^(e-16) and 2^(e-24).  Every piece then has at most 8 significant bits,
   // so its BF16 form is the upper half of its FP32 bits.
   const float c0 = ldexpf(1.5f, e + 15), c1 = ldexpf(1.5f, e + 7), c2 = ldexpf(1.5f, e - 1);
-  unsigned hb[8], mb[8], lb[8];
+  float vh[8], vm[8], vl[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const float vh = (x[q] + c0) - c0;
-    const float r = x[q] - vh;  // exact
-    const float vm = (r + c1) - c1;
-    const float vl = ((r - vm) + c2) - c2;
-    hb[q] = __builtin_bit_cast(unsigned, vh);
-    mb[q] = __builtin_bit_cast(unsigned, vm);
-    lb[q] = __builtin_bit_cast(unsigned, vl);
+    vh[q] = (x[q] + c0) - c0;
+    const float r = x[q] - vh[q];  // exact
+    vm[q] = (r + c1) - c1;
+    vl[q] = ((r - vm[q]) + c2) - c2;
   }
-  u4 H, Mi, L;
-#pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    H[p] = (hb[2 * p + 1] & 0xffff0000u) | (hb[2 * p] >> 16);
-    Mi[p] = (mb[2 * p + 1] & 0xffff0000u) | (mb[2 * p] >> 16);
-    L[p] = (lb[2 * p + 1] & 0xffff0000u) | (lb[2 * p] >> 16);
-  }
-  h = __builtin_bit_cast(bf8, H);
-  m = __builtin_bit_cast(bf8, Mi);
-  l = __builtin_bit_cast(bf8, L);
+  h = pack_bf8(vh);
+  m = pack_bf8(vm);
+  l = pack_bf8(vl);
 }
 
 // 16x16x32 BF16 MFMA (measured on the MI355X): lane l supplies
@@ -1271,6 +1273,285 @@ __global__ __launch_bounds__(256, ROWS == 128 ? 1 : 2) void k_density_bf2(const 
   density_bf2_item<NT, ROWS, RAD, LO>(blockIdx.x, items, groups, pg3, vals, pw, factor, epart, iexch, spin, factor_b, sums, sstride, px, py, pz, fmeta, rad, ybuf, lo16, sm);
 }
 
+// phi's BF16 pieces of density variant 11, once per grid: for every point
+// and 32-function K block, phi's pair (FP32 plane, BF16 remainder) in four
+// pieces (split4f), in the lanes of the density kernels (a wave holds 16
+// points, lane group lk the functions 8 lk .. 8 lk + 7 of a K block), so the
+// pieces are those the density kernel would make, bit for bit.  Layout per
+// group: four piece planes of m_pad x npts_pad BF16 (point-major, functions
+// contiguous) at 4 x phi_off.
+__global__ __launch_bounds__(256) void k_split_pm(const DensityItem* __restrict__ items,
+                                                  const DevGroup* __restrict__ groups,
+                                                  const float* __restrict__ vals,
+                                                  const unsigned short* __restrict__ lo16,
+                                                  unsigned short* __restrict__ pcs) {
+  const DensityItem it = items[blockIdx.x];
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  if (p0 >= g.npts_pad) return;  // whole waves (the splits exchange across lanes)
+  const int np = g.npts_pad, mp = g.m_pad, pt = p0 + lrow;
+  const long long plane = (long long)mp * np;
+  const float* Vp = vals + g.val_off + (long long)pt * mp;
+  const unsigned short* Vl = lo16 + g.phi_off + (long long)pt * mp;
+  unsigned short* Pc = pcs + 4 * g.phi_off + (long long)pt * mp;
+  for (int k0 = 0; k0 < mp; k0 += 32) {
+    const int kk = k0 + 8 * lk;
+    float x[8], xl[8];
+    load8(Vp + kk, x);
+    bf16x8_to_float(*reinterpret_cast<const u4*>(Vl + kk), xl);
+    bf8 b4[4];
+    split4f(x, xl, b4);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      *reinterpret_cast<u4*>(Pc + j * plane + kk) = __builtin_bit_cast(u4, b4[j]);
+  }
+}
+
+// Density row-dot sums on split-BF16 MFMA, one (point tile, row chunk) per
+// block (density variant 10): the products and epilogue of variant 8 for the
+// 64 functions c0 .. c0 + 63 of a 64-point tile.  The blocks of one tile are
+// consecutive in their XCD's dispatch order (the list is built so), so they
+// run side by side and the B operand of the tile's points (read for every
+// chunk) comes from HBM once and from the XCD's L2 for the other chunks.
+// Each block leaves its ten row-dot sums per point in cpart (one slot per
+// chunk) and k_density_cs_finish adds them in chunk order, then evaluates
+// the functional with one lane per point (the density blocks hold each point
+// on four lanes, a quarter of the wave).  (A last-block-finishes form, with
+// an agent-scope counter per tile, needs agent-scope fences in every block;
+// on this GPU they write back and invalidate the XCD's L2, which undid the
+// sharing: 9.0 ms against 1.4 ms.)
+struct ChunkItem {
+  int group, p0, c0;
+  int tile;   // the point tile's density item: its energy slot
+  int pbase;  // first cpart slot of the tile (one per chunk)
+  int nch;    // chunks of the group
+};
+
+// PS (density variant 11): phi's four BF16 pieces come split once per grid
+// (k_split_pm, the same pieces bit for bit), so the K loop loads them (four
+// 16-byte loads a step, one step ahead) instead of the FP32 plane and the
+// remainder and splits only the gradient planes.  (All thirteen B pieces
+// split once per grid leave the loop with almost no VALU work but read 26
+// bytes per point and function for every row chunk instead of 18, and the
+// launch turned bandwidth-bound: 1.62 ms against 1.38 ms at (H2O)32.)
+template <int NT, bool PS>
+__global__ __launch_bounds__(256, 2) void k_density_cs(const ChunkItem* __restrict__ items,
+                                                       const DevGroup* __restrict__ groups,
+                                                       const unsigned short* __restrict__ pg4,
+                                                       const float* __restrict__ vals,
+                                                       float* __restrict__ ybuf,
+                                                       const unsigned short* __restrict__ lo16,
+                                                       double* __restrict__ cpart,
+                                                       const unsigned short* __restrict__ pcs) {
+  constexpr int NS = (NT == 4) ? 10 : 1;
+  constexpr int ROWS = 64, RT = 4, NPC = 4;
+  constexpr int UNITS = NPC * ROWS * 4;  // 16-byte units of one staged P_g tile
+  __shared__ u4 sA[2][UNITS];            // [buf][piece][row][k octet]
+  const ChunkItem it = items[blockIdx.x];
+  if (it.group < 0) return;  // padding of a short XCD list
+  const DevGroup g = groups[it.group];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int p0 = it.p0 + 16 * wave;
+  const bool active = p0 < g.npts_pad;
+  const int np = g.npts_pad, mp = g.m_pad, c0 = it.c0;
+  const long long plane = (long long)mp * np;
+  const long long psz = (long long)mp * mp;
+  const int pt = active ? p0 + lrow : it.p0;  // inactive waves still stage tiles
+  const float* Vp = vals + g.val_off + (long long)pt * mp;
+  const unsigned short* Vl = lo16 + g.phi_off + (long long)pt * mp;
+  const unsigned short* Pp = pg4 + NPC * g.pg_off;
+  // P_g tile of a K step: 16-byte unit c = threadIdx.x + 256 i is piece
+  // c / (4 ROWS), row (c / 4) % ROWS, k octet c % 4, loaded into registers
+  // before the step's products and written to LDS after them.  (Copied
+  // straight into LDS instead, the copy's completion has to be awaited
+  // before the next LDS instruction -- the split's cross-lane exchange --
+  // which left its latency exposed at every step.)  Rows past the group's
+  // functions are never read (their address is clamped to the last row).
+  u4 pa[UNITS / 256];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) {
+      const int c = threadIdx.x + 256 * i;
+      const int sp = c / (4 * ROWS), row = (c >> 2) & (ROWS - 1), oc = c & 3;
+      const int j = min(c0 + row, mp - 1);
+      pa[i] = *reinterpret_cast<const u4*>(Pp + sp * psz + (long long)j * mp + k0 + 8 * oc);
+    }
+  };
+  auto lstore = [&](int b) {
+#pragma unroll
+    for (int i = 0; i < UNITS / 256; ++i) sA[b][threadIdx.x + 256 * i] = pa[i];
+  };
+  const int nrt = min(RT, (mp - c0) / 16);
+  f4 acc[RT][NT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[rt][t] = f4{0, 0, 0, 0};
+  const unsigned short* Pc = pcs + 4 * g.phi_off + (long long)pt * mp;  // PS
+  float xb[NT][8];      // the next step's FP32 values (plane 0 unused with PS)
+  u4 xlp = u4{0, 0, 0, 0};
+  u4 nb[PS ? 4 : 1];    // the next step's phi pieces (PS)
+  if constexpr (PS) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) nb[j] = *reinterpret_cast<const u4*>(Pc + j * plane + 8 * lk);
+  } else {
+    load8(Vp + 8 * lk, xb[0]);
+    xlp = *reinterpret_cast<const u4*>(Vl + 8 * lk);
+  }
+#pragma unroll
+  for (int t = 1; t < NT; ++t) load8(Vp + t * plane + 8 * lk, xb[t]);
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int k0 = 0, buf = 0; k0 < mp; k0 += 32, buf ^= 1) {
+    const bool more = k0 + 32 < mp;
+    const int kk = k0 + 8 * lk;
+    bf8 bh[NT], bm[NT], bl[NT];
+    bf8 b4[4];
+    if constexpr (PS) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b4[j] = __builtin_bit_cast(bf8, nb[j]);
+    } else {
+      float xl[8];
+      bf16x8_to_float(xlp, xl);
+      split4f(xb[0], xl, b4);
+    }
+#pragma unroll
+    for (int t = 1; t < NT; ++t) split3(xb[t], bh[t], bm[t], bl[t]);
+    if (more) {
+      gload(k0 + 32);
+      if constexpr (PS) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nb[j] = *reinterpret_cast<const u4*>(Pc + j * plane + kk + 32);
+      } else {
+        load8(Vp + kk + 32, xb[0]);
+        xlp = *reinterpret_cast<const u4*>(Vl + kk + 32);
+      }
+#pragma unroll
+      for (int t = 1; t < NT; ++t) load8(Vp + t * plane + kk + 32, xb[t]);
+    }
+    if (active) {
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        if (rt < nrt) {
+          const int u = (16 * rt + lrow) * 4 + lk;
+          const bf8 ah = __builtin_bit_cast(bf8, sA[buf][u]);
+          const bf8 am = __builtin_bit_cast(bf8, sA[buf][ROWS * 4 + u]);
+          const bf8 al = __builtin_bit_cast(bf8, sA[buf][2 * ROWS * 4 + u]);
+          const bf8 a4[4] = {ah, am, al, __builtin_bit_cast(bf8, sA[buf][3 * ROWS * 4 + u])};
+          f4 hi, lo;
+          mma10(a4, b4, hi, lo);
+          acc[rt][0] += hi + lo;
+#pragma unroll
+          for (int t = 1; t < NT; ++t) acc[rt][t] += mma6(ah, am, al, bh[t], bm[t], bl[t]);
+        }
+      }
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+  }
+  double s[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) s[k] = 0.0;
+  if (active) {
+    if (ybuf) {  // Y = P_g phi of this point for the force pass (exact FP32 sums)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+        if (rt < nrt)
+          *reinterpret_cast<fl4*>(ybuf + g.phi_off + (long long)pt * mp + c0 + 16 * rt + 4 * lk) =
+              fl4{acc[rt][0][0], acc[rt][0][1], acc[rt][0][2], acc[rt][0][3]};
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      if (rt >= nrt) continue;
+      const float* vj = Vp + c0 + 16 * rt + 4 * lk;  // functions j .. j+3 of this point
+      const fl4 vlo = bf16x4_to_float(Vl + c0 + 16 * rt + 4 * lk);
+      if constexpr (NT == 4) {
+        fl4 v[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) v[q] = *reinterpret_cast<const fl4*>(vj + q * plane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double y0 = acc[rt][0][i], yx = acc[rt][1][i], yy = acc[rt][2][i],
+                       yz = acc[rt][3][i];
+          const double v1 = v[1][i], v2 = v[2][i], v3 = v[3][i];
+          s[0] += ((double)v[0][i] + (double)vlo[i]) * y0;
+          s[1] += v1 * y0;
+          s[2] += v2 * y0;
+          s[3] += v3 * y0;
+          s[4] += (double)v[4][i] * y0 + v1 * yx;
+          s[5] += (double)v[5][i] * y0 + v2 * yy;
+          s[6] += (double)v[6][i] * y0 + v3 * yz;
+          s[7] += (double)v[7][i] * y0 + v1 * yy;
+          s[8] += (double)v[8][i] * y0 + v1 * yz;
+          s[9] += (double)v[9][i] * y0 + v2 * yz;
+        }
+      } else {
+        const fl4 v0 = *reinterpret_cast<const fl4*>(vj);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s[0] += ((double)v0[i] + (double)vlo[i]) * (double)acc[rt][0][i];
+      }
+    }
+    for (int k = 0; k < NS; ++k) {  // the 4 lane groups holding the same point
+      s[k] += __shfl_xor(s[k], 16);
+      s[k] += __shfl_xor(s[k], 32);
+    }
+  }
+  if (active && lk == 0) {
+    double* my = cpart + (long long)(it.pbase + c0 / ROWS) * NS * kDensPts + 16 * wave + lrow;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) my[k * kDensPts] = s[k];
+  }
+}
+
+// Density variant 10, after k_density_cs: one wave per point tile, lane l
+// its point l; the chunks' row-dot sums added in chunk order, then the
+// functional and the tile's energy partials in the order of the one-block
+// form (16-point groups by a butterfly, then the groups in order).
+template <int NT>
+__global__ __launch_bounds__(256) void k_density_cs_finish(const ChunkItem* __restrict__ fin, int nfin,
+                                                           const DevGroup* __restrict__ groups,
+                                                           const double* __restrict__ pw,
+                                                           double* __restrict__ factor,
+                                                           double* __restrict__ epart, int iexch,
+                                                           int spin, double* __restrict__ factor_b,
+                                                           double* __restrict__ sums,
+                                                           long long sstride,
+                                                           const double* __restrict__ cpart) {
+  constexpr int NS = (NT == 4) ? 10 : 1;
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (w >= nfin) return;  // whole waves
+  const ChunkItem it = fin[w];
+  const DevGroup g = groups[it.group];
+  const bool active = it.p0 + lane < g.npts_pad;
+  double e_w = 0.0, n_w = 0.0;
+  if (active) {
+    const double* ps = cpart + (long long)it.pbase * NS * kDensPts + lane;
+    double s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      double t = ps[k * kDensPts];
+      for (int c = 1; c < it.nch; ++c) t += ps[((long long)c * NS + k) * kDensPts];
+      s[k] = t;
+    }
+    point_stage<NT>(s, g.pt_off + it.p0 + lane, iexch, spin, pw, factor, factor_b, sums, sstride,
+                    e_w, n_w);
+  }
+  for (int o = 8; o >= 1; o >>= 1) {
+    e_w += __shfl_xor(e_w, o);
+    n_w += __shfl_xor(n_w, o);
+  }
+  const double e1 = __shfl(e_w, 16), e2 = __shfl(e_w, 32), e3 = __shfl(e_w, 48);
+  const double n1 = __shfl(n_w, 16), n2 = __shfl(n_w, 32), n3 = __shfl(n_w, 48);
+  if (lane == 0) {
+    epart[2 * it.tile] = ((e_w + e1) + e2) + e3;
+    epart[2 * it.tile + 1] = ((n_w + n1) + n2) + n3;
+  }
+}
+
 // Density + functional on split-BF16 MFMA, row-split form for groups of
 // more than 64 functions: a block owns 16 points and its four waves split the
 // function row tiles (wave w takes tiles w, w + 4, ..., RTW of them a pass).
@@ -1589,7 +1870,6 @@ __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__
 // ends this step, so each step runs [radial sums of K shells, DMA in flight]
 // -> barrier -> [K operand, MFMAs] -> barrier.  Two blocks share a CU.
 
-constexpr int kEvShK = 32;  // shells one K step (32 functions) touches at most
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1597,53 +1877,14 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Block fixed-point split of 8 FP64 values (a lane's 8 K values of one
-// point; the same 4-lane block as split3) into four BF16 pieces in units
-// 2^(e-8), 2^(e-16), 2^(e-24), 2^(e-32), 2^(e-1) <= max |x| < 2^e.  Adding
-// and removing 1.5 * 2^(e+44) rounds a double of magnitude below 2^e to the
-// nearest multiple of 2^(e-8) (the sum lies in [2^(e+44), 2^(e+45)), whose
-// spacing is 2^(e-8)); the remainders are exact.  Every piece is an integer
-// of magnitude at most 256 times its unit, exact in FP32, and its BF16 form
-// is the upper half of the FP32 bits.  All 64 lanes must call it.
-__device__ inline void split4d(const double (&x)[8], bf8 (&pc)[4]) {
-  double mx = 0.0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) mx = fmax(mx, fabs(x[q]));
-  mx = fmax(mx, __shfl_xor(mx, 16));
-  mx = fmax(mx, __shfl_xor(mx, 32));
-  int e;
-  (void)frexp(mx, &e);
-  const double c0 = ldexp(1.5, e + 44), c1 = ldexp(1.5, e + 36), c2 = ldexp(1.5, e + 28),
-               c3 = ldexp(1.5, e + 20);
-  unsigned b[4][8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double v0 = (x[q] + c0) - c0;
-    double r = x[q] - v0;
-    const double v1 = (r + c1) - c1;
-    r -= v1;
-    const double v2 = (r + c2) - c2;
-    r -= v2;
-    const double v3 = (r + c3) - c3;
-    b[0][q] = __builtin_bit_cast(unsigned, (float)v0);
-    b[1][q] = __builtin_bit_cast(unsigned, (float)v1);
-    b[2][q] = __builtin_bit_cast(unsigned, (float)v2);
-    b[3][q] = __builtin_bit_cast(unsigned, (float)v3);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    u4 u;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) u[p] = (b[k][2 * p + 1] & 0xffff0000u) | (b[k][2 * p] >> 16);
-    pc[k] = __builtin_bit_cast(bf8, u);
-  }
-}
-
-// The same split from an FP32 pair (hi + lo, |lo| <= ulp(hi) / 2: a double
-// rounded to FP32 and its remainder): the first three pieces come from hi
-// as in split3, the fourth from hi's remainder plus lo (their FP32 sum is
-// off by at most 2^-48 of the block maximum, far below the piece's unit
-// 2^(e-32)).  Same pieces as split4d of hi + lo up to that rounding.
+// Block fixed-point split of an FP32 pair (hi + lo, |lo| <= ulp(hi) / 2: a
+// double rounded to FP32 and its remainder; a lane's 8 K values of one
+// point, the same 4-lane block as split3) into four BF16 pieces in units
+// 2^(e-8), 2^(e-16), 2^(e-24), 2^(e-32), 2^(e-1) <= max |hi| < 2^e: the
+// first three pieces come from hi as in split3, the fourth from hi's
+// remainder plus lo (their FP32 sum is off by at most 2^-48 of the block
+// maximum, far below the piece's unit 2^(e-32)), so the pieces are those of
+// the double hi + lo up to that rounding.  All 64 lanes must call it.
 __device__ inline void split4f(const float (&hi)[8], const float (&lo)[8], bf8 (&pc)[4]) {
   float mx = 0.f;
 #pragma unroll
@@ -1654,28 +1895,19 @@ __device__ inline void split4f(const float (&hi)[8], const float (&lo)[8], bf8 (
   (void)frexpf(mx, &e);
   const float c0 = ldexpf(1.5f, e + 15), c1 = ldexpf(1.5f, e + 7), c2 = ldexpf(1.5f, e - 1),
               c3 = ldexpf(1.5f, e - 9);
-  unsigned b[4][8];
+  float v[4][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const float v0 = (hi[q] + c0) - c0;
-    float r = hi[q] - v0;
-    const float v1 = (r + c1) - c1;
-    r -= v1;
-    const float v2 = (r + c2) - c2;
-    r = (r - v2) + lo[q];
-    const float v3 = (r + c3) - c3;
-    b[0][q] = __builtin_bit_cast(unsigned, v0);
-    b[1][q] = __builtin_bit_cast(unsigned, v1);
-    b[2][q] = __builtin_bit_cast(unsigned, v2);
-    b[3][q] = __builtin_bit_cast(unsigned, v3);
+    v[0][q] = (hi[q] + c0) - c0;
+    float r = hi[q] - v[0][q];
+    v[1][q] = (r + c1) - c1;
+    r -= v[1][q];
+    v[2][q] = (r + c2) - c2;
+    r = (r - v[2][q]) + lo[q];
+    v[3][q] = (r + c3) - c3;
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    u4 u;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) u[p] = (b[k][2 * p + 1] & 0xffff0000u) | (b[k][2 * p] >> 16);
-    pc[k] = __builtin_bit_cast(bf8, u);
-  }
+  for (int k = 0; k < 4; ++k) pc[k] = pack_bf8(v[k]);
 }
 
 // The ten piece products of weight >= 2^-24 of four-piece operands, as two
@@ -3284,7 +3516,7 @@ struct DeviceGrid {
   int M = 0, natom = 0, iexch = 9, nval = 10;  // M, natom: whole batch
   int nmol = 1, max_M = 0, max_ng = 0, max_mpad = 0;
   int fock_slice = kSliceDefault;
-  int density_variant = 8;
+  int density_variant = 10;
   int nrowsplit = 0;  // leading density items on the row-split kernel
   long long packed = 0;                 // packed entries of the whole batch
   std::vector<long long> mol_packed;    // per molecule offset, then the total
@@ -3341,7 +3573,15 @@ struct DeviceGrid {
   double* fout = nullptr;         // forces, natom x 3 (atom-major)
   bool radial = false;            // radial-Hessian layout (split-BF16 variant 5)
   bool inkernel = false;          // basis evaluated in the density kernel (variants 6, 7)
-  bool lo = false;                // phi's remainder (lo16), four-piece Y_0 (variants 8, 9)
+  bool lo = false;                // phi's remainder (lo16), four-piece Y_0 (variants 8-10)
+  bool chunked = false;           // one density block per (point tile, row chunk) (variants 10, 11)
+  bool presplit = false;          // phi's pieces split once per grid (variant 11)
+  unsigned short* pcs = nullptr;  // those pieces: 4 planes of phi_elems BF16
+  ChunkItem* chitems = nullptr;   // its blocks in dispatch order (XCD-interleaved)
+  int nchunk = 0;
+  double* cpart = nullptr;        // per (point tile, chunk): 10 x 64 row-dot sums
+  ChunkItem* finitems = nullptr;  // per point tile (k_density_cs_finish)
+  int nfin = 0;
   unsigned short* lo16 = nullptr; // phi's remainder in BF16, point-major (phi_elems)
   bool persistent = false;        // one persistent launch per build (k_build_persist)
   PersistArgs pargs{};            // its queue and tables (device pointers)
@@ -3578,7 +3818,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->fock_slice = tuning_.fock_slice;
   dg->density_variant = tuning_.density_variant;
 
-  if (dg->density_variant < 0 || dg->density_variant > 9)
+  if (dg->density_variant < 0 || dg->density_variant > 11)
     throw std::runtime_error("g2g: unknown density kernel variant");
   dg->iexch = mols[0]->iexch;
   dg->gga = mols[0]->gga;
@@ -3592,7 +3832,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   if (dg->open && prec == Precision::BF16X6 && dg->density_variant != 1 &&
       dg->density_variant != 2 && dg->density_variant < 5) {
-    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2 or 5-9");
+    throw std::runtime_error("g2g: open shell in split-BF16 needs density variant 1, 2 or 5-11");
   }
   const int nspin = dg->open ? 2 : 1;
   // LDA keeps the gradient too: the XC forces need it.  The radial-Hessian
@@ -3601,8 +3841,11 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   dg->radial = prec == Precision::BF16X6 && tuning_.density_variant == 5;
   dg->inkernel = prec == Precision::BF16X6 &&
                  (tuning_.density_variant == 6 || tuning_.density_variant == 7);
-  dg->lo = prec == Precision::BF16X6 &&
-           (tuning_.density_variant == 8 || tuning_.density_variant == 9);
+  dg->lo = prec == Precision::BF16X6 && tuning_.density_variant >= 8;
+  // variant 10: variant 8's products, one block per (point tile, row chunk);
+  // 11: 10 with phi's four pieces split once per grid
+  dg->chunked = prec == Precision::BF16X6 && tuning_.density_variant >= 10;
+  dg->presplit = prec == Precision::BF16X6 && tuning_.density_variant == 11;
   // in-kernel evaluation keeps no value planes (phi alone, function-major,
   // for the Fock kernel); variant 8 one more plane (phi's remainder)
   dg->nval = dg->inkernel ? 0 : (dg->gga && !dg->radial) ? 10 : 4;
@@ -3612,7 +3855,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   // split-BF16 K steps are 32 functions; its P_g holds three 2-byte pieces
   const int fn_align = bf ? 32 : kFnAlign;
   const size_t pbytes =
-      bf ? (dg->inkernel || dg->density_variant == 8 ? 4 : 3) * sizeof(unsigned short) : vbytes;
+      bf ? (dg->inkernel || dg->density_variant == 8 || dg->chunked ? 4 : 3) *
+               sizeof(unsigned short)
+         : vbytes;
 
   // batch numbering: molecule k owns functions [fn_base, fn_base + M) and
   // packed entries [rho_off, rho_off + M (M + 1) / 2)
@@ -3834,6 +4079,35 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       if (groups[ditems[i].group].mol == k) eidx.push_back(i);
     hmol[k].ee = (int)eidx.size();
   }
+  // variants 10, 11: every point tile's row chunks.  Block b runs on XCD
+  // b % kXcds, so the list is the XCDs' lists interleaved: each tile (heavy
+  // groups first) goes whole to the XCD with the least work so far, its
+  // chunks consecutive there, and the short lists are padded with empty
+  // items (a few).  Contiguous runs of tiles per XCD (a group's P_g in one
+  // L2) needed thousands of empty items for the uneven item counts and ran
+  // slower.
+  std::vector<ChunkItem> chitems, finitems;
+  long long npart = 0;
+  if (dg->chunked) {
+    std::vector<std::vector<ChunkItem>> lane(kXcds);
+    std::vector<long long> load(kXcds, 0);
+    for (int i = 0; i < (int)ditems.size(); ++i) {
+      const int mp = groups[ditems[i].group].m_pad, nch = (mp + kFnChunk - 1) / kFnChunk;
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += (long long)nch * (mp + 48);  // K steps and the epilogue
+      finitems.push_back(ChunkItem{ditems[i].group, ditems[i].p0, 0, i, (int)npart, nch});
+      for (int c = 0; c < nch; ++c)
+        lane[x].push_back(ChunkItem{ditems[i].group, ditems[i].p0, kFnChunk * c, i, (int)npart, nch});
+      npart += nch;
+    }
+    size_t len = 0;
+    for (const auto& l : lane) len = std::max(len, l.size());
+    chitems.assign(len * kXcds, ChunkItem{-1, 0, 0, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+      for (size_t j = 0; j < lane[x].size(); ++j) chitems[j * kXcds + x] = lane[x][j];
+  }
+  dg->nchunk = (int)chitems.size();
+  dg->nfin = (int)finitems.size();
   // Fock partial k belongs to (group, tile pair, slice) in slot order
   std::vector<FockItem> fitems(fock_base);
   std::vector<int2> gtiles;
@@ -3942,6 +4216,10 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->kbsh = c.take<int2>(kbsh.size() + 1);
     dg->sfn = c.take<int2>(sfn.size() + 1);
     dg->lo16 = c.take<unsigned short>(dg->lo ? fo : 0);
+    dg->chitems = c.take<ChunkItem>(chitems.size());
+    dg->pcs = c.take<unsigned short>(dg->presplit ? 4 * fo : 0);
+    dg->cpart = c.take<double>((size_t)npart * 10 * kDensPts);
+    dg->finitems = c.take<ChunkItem>(finitems.size());
     PersistArgs& pa = dg->pargs;
     dg->dpargs = c.take<PersistArgs>(1);
     pa.sync = c.take<int>(dg->nsync + 1);
@@ -4057,6 +4335,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->sfn, sfn.data(), sfn.size() * sizeof(int2));
   up(dg->afn_off, afn_off.data(), afn_off.size() * sizeof(int));
   up(dg->afn, afn.data(), afn.size() * sizeof(int));
+  up(dg->chitems, chitems.data(), chitems.size() * sizeof(ChunkItem));
+  up(dg->finitems, finitems.data(), finitems.size() * sizeof(ChunkItem));
   if (dg->persistent) {
     PersistArgs& pa = dg->pargs;
     up(const_cast<int*>(pa.gneed), gneed.data(), gneed.size() * sizeof(int));
@@ -4159,6 +4439,9 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
           hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
       }
+      if (dg->presplit && dg->ndensity > 0)
+        hipLaunchKernelGGL(k_split_pm, dim3(dg->ndensity), dim3(256), 0, s, dg->ditems,
+                           dg->groups, v, dg->lo16, dg->pcs);
       if (dg->nrad > 0)
         hipLaunchKernelGGL(k_eval_radial, dim3(dg->nrad), block, 0, s, dg->ritems, dg->groups,
                            dg->shells, dg->bas, dg->px, dg->py, dg->pz, dg->rad);
@@ -4724,7 +5007,7 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
     return;
   }
   if (g.lo) {  // variants 8 and 9: phi's remainder plane (8: four-piece P_g)
-    const bool four = g.density_variant == 8;
+    const bool four = g.density_variant == 8 || g.chunked;
     if (g.ngather > 0) {
       if (four)
         hipLaunchKernelGGL(k_gather_density_bf<4>, dim3(g.ngather), dim3(256), 0, s, g.gitems,
@@ -4736,7 +5019,32 @@ static void launch_density_bf(DeviceGrid& g, hipStream_t s, const double* d_rho,
     if (spin < 2) mark(ev, 1, s);
     float* y = yb ? yb + (spin == 2 ? g.phi_elems : 0) : nullptr;
 
-    if (g.ndensity > 0) {
+    if (g.chunked && g.nchunk > 0) {
+      const dim3 fgrid((g.nfin + 3) / 4);
+      if (g.gga) {
+        if (g.presplit)
+          hipLaunchKernelGGL((k_density_cs<4, true>), dim3(g.nchunk), dim3(256), 0, s, g.chitems,
+                             g.groups, pg3, vals, y, g.lo16, g.cpart, g.pcs);
+        else
+          hipLaunchKernelGGL((k_density_cs<4, false>), dim3(g.nchunk), dim3(256), 0, s, g.chitems,
+                             g.groups, pg3, vals, y, g.lo16, g.cpart, g.pcs);
+        if (g.nfin > 0)
+          hipLaunchKernelGGL(k_density_cs_finish<4>, fgrid, dim3(256), 0, s, g.finitems, g.nfin,
+                             g.groups, g.pw, g.factor, g.epart, g.iexch, spin, g.factor_b, g.sums,
+                             g.npts_pad, g.cpart);
+      } else {
+        if (g.presplit)
+          hipLaunchKernelGGL((k_density_cs<1, true>), dim3(g.nchunk), dim3(256), 0, s, g.chitems,
+                             g.groups, pg3, vals, y, g.lo16, g.cpart, g.pcs);
+        else
+          hipLaunchKernelGGL((k_density_cs<1, false>), dim3(g.nchunk), dim3(256), 0, s, g.chitems,
+                             g.groups, pg3, vals, y, g.lo16, g.cpart, g.pcs);
+        if (g.nfin > 0)
+          hipLaunchKernelGGL(k_density_cs_finish<1>, fgrid, dim3(256), 0, s, g.finitems, g.nfin,
+                             g.groups, g.pw, g.factor, g.epart, g.iexch, spin, g.factor_b, g.sums,
+                             g.npts_pad, g.cpart);
+      }
+    } else if (g.ndensity > 0) {
 #define XCQ_LO(NT, L)                                                                           \
   hipLaunchKernelGGL((k_density_bf2<NT, 64, false, L>), dim3(g.ndensity), dim3(256), 0, s,        \
                      g.ditems, g.groups, pg3, vals, g.pw, g.factor, g.epart, g.iexch, spin,        \

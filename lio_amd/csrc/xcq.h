@@ -186,17 +186,25 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 //               planes instead of 10 in HBM, more arithmetic per point);
 //               6 and 7 = the basis evaluated inside the density kernel
 //               (128- and 64-function chunks; no value planes, FP64 values);
-//               8 (default) = 2 with one more plane holding phi's FP32
+//               8 = 2 with one more plane holding phi's FP32
 //               remainder, Y_0 = P_g phi from four-piece operands (ten piece
 //               products) and phi in FP64 in the epilogue; 9 = 2 with the
-//               remainder added to first order in the epilogue only.
+//               remainder added to first order in the epilogue only; 10
+//               (default) = 8 with one block per (64-point tile, 64-function
+//               row chunk): a tile's blocks run side by side on one XCD and
+//               share its points' values through that L2; a second kernel
+//               adds the chunks' row-dot sums in chunk order and evaluates
+//               the functional; 11 = 10 with phi's four pieces split once
+//               per grid (one more BF16 plane set, 8 bytes per point and
+//               function) instead of in every build.
 //               Variants 0-5 and 9 leave |dN| up to about 1e-6 at a few
-//               hundred electrons; 6-8 keep it near 1e-8 (measured errors
+//               hundred electrons; 6-8, 10 and 11 keep it near 1e-8 (measured errors
 //               against FP64 are recorded by the tests).
-//   persistent: 1 = the whole closed-shell build of the split-BF16 default
-//               as one persistent launch over a queue of every phase's items
-//               (k_build_persist; dependencies by per-group counters), 0 =
-//               one launch per phase.  Bitwise the same results.
+//   persistent: 1 = the whole closed-shell split-BF16 build of density
+//               variant 8 as one persistent launch over a queue of every
+//               phase's items (k_build_persist; dependencies by per-group
+//               counters), 0 = one launch per phase.  Bitwise the results of
+//               variant 8's phase launches; other variants ignore it.
 //   bucket_mhi / bucket_slice: a Fock slice per group size bucket: groups
 //               of at most bucket_mhi[i] padded functions (ascending; the
 //               first bucket that holds the group) take bucket_slice[i];
@@ -204,7 +212,7 @@ enum class Precision : int { FP64 = 0, FP32 = 1, BF16X6 = 2 };
 //               take fock_slice.
 struct Tuning {
   int fock_slice = 1024;
-  int density_variant = 8;
+  int density_variant = 10;
   int persistent = 0;
   std::vector<int> bucket_mhi, bucket_slice;
   int slice_for(int m_pad) const {

@@ -127,6 +127,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.xcq_batch_set_schedule.argtypes = [vp, C.c_int, C.c_int]
     L.xcq_set_persistent.argtypes = [vp, C.c_int]
     L.xcq_batch_set_persistent.argtypes = [vp, C.c_int]
+    L.xcq_batch_get_schedule.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
     L.xcq_batch_set_bucket_slices.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.xcq_batch_group_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.c_int,
                                         C.POINTER(C.c_int)]
@@ -567,9 +569,17 @@ class Batch:
                "xcq_batch_set_schedule")
 
     def set_persistent(self, on: bool):
-        """One persistent launch per build (split-BF16 default, closed shell);
-        reloads.  Bitwise the results of the phase launches."""
+        """One persistent launch per build (split-BF16 density variant 8,
+        closed shell; other variants ignore it); reloads.  Bitwise the
+        results of variant 8's phase launches."""
         _check(self.L.xcq_batch_set_persistent(self.h, int(bool(on))), "xcq_batch_set_persistent")
+
+    def schedule(self) -> dict:
+        """The batch's schedule knobs: fock_slice, density_variant, persistent."""
+        a, b, c = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(self.L.xcq_batch_get_schedule(self.h, C.byref(a), C.byref(b), C.byref(c)),
+               "xcq_batch_get_schedule")
+        return {"fock_slice": a.value, "density_variant": b.value, "persistent": bool(c.value)}
 
     def tune(self, d_rho: int, d_fock: int, slices=(256, 512, 1024, 2048), variants=None,
              reps: int = 5):
@@ -635,10 +645,11 @@ class Batch:
         buckets[i] padded functions; larger groups form the last bucket) is
         chosen by timing whole builds on these buffers, one bucket at a time,
         heaviest bucket (sum of n m^2) first, the others held at their best so
-        far; then, in split-BF16 mode, one persistent launch against one
-        launch per phase.  The result is written to ``path`` (when given) and
-        returned: bucket bounds and slices, the launch form, the ms per build
-        of every candidate timed."""
+        far; then, in split-BF16 mode, one persistent launch (density variant
+        8) against one launch per phase with the batch's density variant.
+        The result is written to ``path`` (when given) and returned: bucket
+        bounds and slices, the launch form and density variant, the ms per
+        build of every candidate timed."""
         bounds = [int(b) for b in buckets]
         if path and os.path.exists(path):
             with open(path) as f:
@@ -646,6 +657,8 @@ class Batch:
             if sched.get("bucket_mhi", [])[:len(bounds)] == bounds and \
                     sched.get("precision") == self.precision:
                 self.set_bucket_slices(sched["bucket_mhi"], sched["bucket_slice"])
+                if "density_variant" in sched:
+                    self.set_schedule(density_variant=int(sched["density_variant"]))
                 self.set_persistent(bool(sched.get("persistent")))
                 sched["loaded"] = True
                 return sched
@@ -658,6 +671,7 @@ class Batch:
         choice = [1024] * nb
         table = {}
         self.set_persistent(False)
+        variant = self.schedule()["density_variant"]
         self.set_bucket_slices(edges, choice)
         best = self._time_builds(d_rho, d_fock, reps)
         table["start"] = best
@@ -677,15 +691,18 @@ class Batch:
         self.set_bucket_slices(edges, choice)
         use_persistent = False
         if self.precision == "bf16x6" and len(persistent) > 1:
+            self.set_schedule(density_variant=8)
             self.set_persistent(True)
             ms = self._time_builds(d_rho, d_fock, reps)
             table["persistent"] = ms
             if ms < best:
-                best, use_persistent = ms, True
+                best, use_persistent, variant = ms, True, 8
             else:
                 self.set_persistent(False)
+                self.set_schedule(density_variant=variant)
         sched = {"bucket_mhi": edges, "bucket_slice": [int(c) for c in choice],
-                 "persistent": use_persistent, "precision": self.precision, "ms": best,
+                 "persistent": use_persistent, "density_variant": variant,
+                 "precision": self.precision, "ms": best,
                  "bucket_groups": np.bincount(which, minlength=nb).tolist(),
                  "bucket_work": work.tolist(), "table": table, "loaded": False}
         if path:

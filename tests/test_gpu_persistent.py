@@ -1,10 +1,11 @@
 """One persistent launch per build, size-bucket schedules and the autotuner,
 on the GPU.
 
-* The split-BF16 default build as one persistent launch over the ordered
-  queue of every phase's items gives bitwise the results of the phase
-  launches: single molecules, a batch, forces, and every replay of a
-  captured graph (with eager work on other buffers between replays).
+* The split-BF16 build of density variant 8 as one persistent launch over
+  the ordered queue of every phase's items gives bitwise the results of
+  variant 8's phase launches: single molecules, a batch, forces, and every
+  replay of a captured graph (with eager work on other buffers between
+  replays).
 * A Fock slice per group size bucket changes only the order of the slice
   sums: within 1e-11 of the single-slice schedule in FP64.
 * The autotuner writes its schedule as JSON and a second batch applies the
@@ -50,6 +51,7 @@ def test_persistent_build_equals_phase_launches(name):
     out = []
     for on in (False, True):
         b = g2g.Batch([s], precision="bf16x6")
+        b.set_schedule(density_variant=8)
         b.set_persistent(on)
         out.append(_solve(b, torch) + _solve(b, torch, forces=True))
         b.close()
@@ -63,9 +65,11 @@ def test_persistent_batch_and_graph_replays():
     from lio_amd import g2g, systems
     sys_ = systems.molecule_batch(6, seed=23, atoms=(3, 30))
     b0 = g2g.Batch(sys_, precision="bf16x6")
+    b0.set_schedule(density_variant=8)
     E0, N0, F0, _ = _solve(b0, torch)
     b0.close()
     b = g2g.Batch(sys_, precision="bf16x6")
+    b.set_schedule(density_variant=8)
     b.set_persistent(True)
     E1, N1, F1, _ = _solve(b, torch)
     assert np.array_equal(E0, E1) and np.array_equal(N0, N1) and np.array_equal(F0, F1)
@@ -120,6 +124,7 @@ def test_autotune_writes_and_reapplies_its_schedule(tmp_path):
     again = b2.autotune(rho.data_ptr(), F.data_ptr(), path=path)
     assert again["loaded"] and again["bucket_slice"] == sched["bucket_slice"]
     assert again["persistent"] == sched["persistent"]
+    assert b2.schedule()["density_variant"] == sched["density_variant"]
     E2, N2, F2, _ = _solve(b2, torch)
     b2.close()
     assert np.array_equal(E1, E2) and np.array_equal(F1, F2)

@@ -36,7 +36,8 @@ def _run(b, torch, static):
     return E, N, F.cpu().numpy(), f.cpu().numpy()
 
 
-@pytest.mark.parametrize("precision,variant", [("bf16x6", 2), ("bf16x6", 7), ("fp64", None)])
+@pytest.mark.parametrize("precision,variant", [("bf16x6", 2), ("bf16x6", 7), ("bf16x6", 10),
+                                               ("fp64", None)])
 def test_static_forces_equal_allocating_forces(precision, variant):
     torch = _torch()
     from lio_amd import g2g, systems
