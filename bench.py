@@ -58,9 +58,9 @@ DENSITY_KERNEL = "k_density_cs<4, false> + k_density_cs_finish<4>"
 # HBM bytes per density phase on (H2O)32: rocprofv3 FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 in KiB (copy-kernel calibrated) of both launches, measured
 # on the tree named in DENSITY_TRAFFIC_TREE (profiles/xc_fock_h2o32.md gives
-# the command); None until measured on this kernel
-DENSITY_TRAFFIC = None
-DENSITY_TRAFFIC_TREE = None
+# the command)
+DENSITY_TRAFFIC = ((2499086 + 56179) * 2.0 + (106171 + 5543) * 1.0) * 1024
+DENSITY_TRAFFIC_TREE = "e42f99e"
 
 
 def log(*a):
