@@ -335,7 +335,7 @@ __device__ inline void point_stage(const double* s, int q, int iexch, int spin,
     const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
     const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
     const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
-    xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+    xc::gga_point_x(dens, gr, h1, h2, iexch, ex, ec, y2a);
   } else {
     xc::lda_point(dens, iexch, ex, ec, y2a);
   }
@@ -952,7 +952,7 @@ __global__ __launch_bounds__(256, 2) void k_density_bf(const DensityItem* __rest
         const double gr[3] = {2.0 * s[1], 2.0 * s[2], 2.0 * s[3]};
         const double h1[3] = {2.0 * s[4], 2.0 * s[5], 2.0 * s[6]};
         const double h2[3] = {2.0 * s[7], 2.0 * s[8], 2.0 * s[9]};
-        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+        xc::gga_point_x(dens, gr, h1, h2, iexch, ex, ec, y2a);
       } else {
         xc::lda_point(dens, iexch, ex, ec, y2a);
       }
@@ -1689,7 +1689,7 @@ __global__ __launch_bounds__(256, 2) void k_density_bf3(const DensityItem* __res
         const double gr[3] = {2.0 * t[1], 2.0 * t[2], 2.0 * t[3]};
         const double h1[3] = {2.0 * t[4], 2.0 * t[5], 2.0 * t[6]};
         const double h2[3] = {2.0 * t[7], 2.0 * t[8], 2.0 * t[9]};
-        xc::gga_point(dens, gr, h1, h2, ex, ec, y2a);
+        xc::gga_point_x(dens, gr, h1, h2, iexch, ex, ec, y2a);
       } else {
         xc::lda_point(dens, iexch, ex, ec, y2a);
       }
@@ -1831,13 +1831,20 @@ __device__ __forceinline__ void fock_bf_item(const int bi, const FockItem* __res
   for (int e = threadIdx.x; e < kFnChunk * kFnChunk; e += 256) out[e] = sF[e];
 }
 
+// Block b takes item order[b] (its partial keeps the item's slot): the tile
+// pairs of one group and slice are consecutive in one XCD's dispatch order,
+// so the function rows they share come from that XCD's L2; -1 pads a short
+// XCD list.
 __global__ __launch_bounds__(256, 2) void k_fock_bf(const FockItem* __restrict__ items,
                                                   const DevGroup* __restrict__ groups,
                                                   const float* __restrict__ phiT,
                                                   const double* __restrict__ factor,
-                                                  double* __restrict__ fpart) {
+                                                  double* __restrict__ fpart,
+                                                  const int* __restrict__ order) {
   __shared__ double sF[kFnChunk * kFnChunk];
-  fock_bf_item(blockIdx.x, items, groups, phiT, factor, fpart, sF);
+  const int it = order[blockIdx.x];
+  if (it < 0) return;
+  fock_bf_item(it, items, groups, phiT, factor, fpart, sF);
 }
 
 // ---------------------------------------------------------------------------
@@ -3581,6 +3588,8 @@ struct DeviceGrid {
   int nchunk = 0;
   double* cpart = nullptr;        // per (point tile, chunk): 10 x 64 row-dot sums
   ChunkItem* finitems = nullptr;  // per point tile (k_density_cs_finish)
+  int* forder = nullptr;          // split-BF16 Fock blocks: item per block (XCD-interleaved)
+  int nfock_order = 0;
   int nfin = 0;
   unsigned short* lo16 = nullptr; // phi's remainder in BF16, point-major (phi_elems)
   bool persistent = false;        // one persistent launch per build (k_build_persist)
@@ -4107,6 +4116,28 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       for (size_t j = 0; j < lane[x].size(); ++j) chitems[j * kXcds + x] = lane[x][j];
   }
   dg->nchunk = (int)chitems.size();
+  // split-BF16 Fock blocks: the tile pairs of each (group, slice), heavy
+  // groups first, go whole to the XCD with the fewest blocks so far
+  std::vector<int> forder;
+  if (bf) {
+    std::vector<std::vector<int>> lane(kXcds);
+    for (int g : order) {
+      const DevGroup& d = groups[g];
+      const int pairs = d.ntile * (d.ntile + 1) / 2;
+      for (int sl = 0; sl < d.nslice; ++sl) {
+        size_t x = 0;
+        for (int q = 1; q < kXcds; ++q)
+          if (lane[q].size() < lane[x].size()) x = q;
+        for (int pr = 0; pr < pairs; ++pr) lane[x].push_back(d.fock_base + pr * d.nslice + sl);
+      }
+    }
+    size_t len = 0;
+    for (const auto& l : lane) len = std::max(len, l.size());
+    forder.assign(len * kXcds, -1);
+    for (int x = 0; x < kXcds; ++x)
+      for (size_t j = 0; j < lane[x].size(); ++j) forder[j * kXcds + x] = lane[x][j];
+  }
+  dg->nfock_order = (int)forder.size();
   dg->nfin = (int)finitems.size();
   // Fock partial k belongs to (group, tile pair, slice) in slot order
   std::vector<FockItem> fitems(fock_base);
@@ -4220,6 +4251,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     dg->pcs = c.take<unsigned short>(dg->presplit ? 4 * fo : 0);
     dg->cpart = c.take<double>((size_t)npart * 10 * kDensPts);
     dg->finitems = c.take<ChunkItem>(finitems.size());
+    dg->forder = c.take<int>(forder.size());
     PersistArgs& pa = dg->pargs;
     dg->dpargs = c.take<PersistArgs>(1);
     pa.sync = c.take<int>(dg->nsync + 1);
@@ -4337,6 +4369,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   up(dg->afn, afn.data(), afn.size() * sizeof(int));
   up(dg->chitems, chitems.data(), chitems.size() * sizeof(ChunkItem));
   up(dg->finitems, finitems.data(), finitems.size() * sizeof(ChunkItem));
+  up(dg->forder, forder.data(), forder.size() * sizeof(int));
   if (dg->persistent) {
     PersistArgs& pa = dg->pargs;
     up(const_cast<int*>(pa.gneed), gneed.data(), gneed.size() * sizeof(int));
@@ -5140,8 +5173,8 @@ static void launch_fock(DeviceGrid& g, hipStream_t s, const double* factor) {
     hipLaunchKernelGGL(k_fock<float>, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups,
                        static_cast<const float*>(g.vals), factor, g.fpart);
   else
-    hipLaunchKernelGGL(k_fock_bf, dim3(g.nfock), dim3(256), 0, s, g.fitems, g.groups, g.phiT,
-                       factor, g.fpart);
+    hipLaunchKernelGGL(k_fock_bf, dim3(g.nfock_order), dim3(256), 0, s, g.fitems, g.groups,
+                       g.phiT, factor, g.fpart, g.forder);
   hipLaunchKernelGGL(k_fock_group, dim3(16 * g.ngroup_tiles), dim3(256), 0, s, g.groups, g.gtiles,
                      g.fpart, g.fg);
   XCQ_HIP(hipGetLastError());

@@ -194,9 +194,6 @@ uint64_t fnv1a(const std::vector<int>& v) {
 Molecule capture(const ProblemView& pv, unsigned grid_type) {
   if (grid_type > 2) throw std::runtime_error("g2g: unknown grid type");
   if (pv.iexch < 1 || pv.iexch > 9) throw std::runtime_error("g2g: Iexch must be in 1..9");
-  if (pv.iexch >= 4 && pv.iexch <= 8)
-    throw std::runtime_error("g2g: Iexch 4-8 (Perdew86, Becke88, LYP GGAs) are not provided by this "
-                             "build; use 9 (PBE) or 1-3 (LDA)");
   Molecule mol;
   mol.natom = (int)pv.natom;
   mol.M = (int)pv.M;

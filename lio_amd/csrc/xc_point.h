@@ -14,7 +14,8 @@
 //     rho^2 > 1e-18 and correlation rho >= 1e-12; |grad rho|^2 is floored at
 //     DBL_MIN.
 // LDA (iexch 1..3): Slater exchange with no correlation, Gunnarsson-Lundqvist
-// or VWN correlation (g2g/pointxc/calc_ldaCS.h:32-102).
+// or VWN correlation (g2g/pointxc/calc_ldaCS.h:32-102).  The other GGAs of
+// iexch 4-8 follow further below (gga_point_x).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -180,6 +181,200 @@ __device__ __host__ inline void lda_point(double rho, int iexch, double& ex, dou
   y2a = v0 + vcl;
 }
 
+
+// ---------------------------------------------------------------------------
+// The other closed-shell GGAs of LIO (iexch 4-8, g2g/pointxc/calc_ggaCS.h:
+// 68-238): exchange Perdew-Wang 86 (iexch 4, 8; PRB 33, 8800 (1986)) or
+// Becke 88 (5, 6, 7; PRA 38, 3098 (1988)); correlation VWN (6), VWN plus
+// Perdew's 1986 gradient correction (4, 5; PRB 33, 8822 (1986)) or
+// Lee-Yang-Parr (7, 8; PRB 37, 785 (1988)).  Each part returns its energy per
+// particle and its share of the Laplacian-form potential y2a, which are
+// summed.  The expressions keep the reference's arithmetic where it changes
+// results: its truncated decimal constants (1.33333333333 for 4/3 and the
+// like), the literals it writes in single precision (the P86 and LYP
+// coefficients, -1/3, 7/6, 8/3: widened here from their float values), the
+// VWN fit evaluated from its four parameters (not the LDA path's tabulated
+// quotients) with the reference's Q * Q2 in the potential, and pow where it
+// calls pow.  Inputs: rho, y = rho^(1/3), |grad rho|^2 (floored at DBL_MIN),
+// |grad rho|, the Laplacian and u0 = grad rho . (Hessian grad rho) / |grad rho|.
+// ---------------------------------------------------------------------------
+namespace gga2 {
+constexpr double kAlpha = -0.738558766382022447;  // Slater exchange, -(3/4)(3/pi)^(1/3)
+constexpr double kRsC = 0.620350490899400087;     // rs = kRsC / rho^(1/3)
+
+// Perdew-Wang 86 exchange: F(s) = (1 + 1.296 s^2 + 14 s^4 + 0.2 s^6)^(1/15).
+__device__ __host__ inline void x_pw86(double rho, double y, double ag, double lap, double u0,
+                                       double& ex, double& vx) {
+  const double kf = 3.0936677 * y;  // (3 pi^2 rho)^(1/3)
+  const double s = ag / (2.0 * kf * rho);
+  const double s2 = s * s, s3 = s2 * s;
+  const double g0 = 1.0 + 1.296 * s2 + 14.0 * pow(s, 4) + 0.2 * pow(s, 6);
+  const double fx = 1.0 / 15.0;
+  const double F = pow(g0, fx);
+  ex = kAlpha * F * y;
+  const double t = lap / (rho * 4.0 * (kf * kf));
+  const double u = u0 / (pow(2.0 * kf, 3) * (rho * rho));
+  const double g2 = 2.592 * s + 56.0 * s3 + 1.2 * pow(s, 5);  // dg0/ds
+  const double g3 = 2.592 + 56.0 * s2 + 1.2 * pow(s, 4);      // g2 / s
+  const double g4 = 112.0 * s + 4.8 * s3;                     // dg3/ds
+  const double dF = fx * F / g0 * g2;
+  const double dsF = fx * F / g0 * (-14.0 * fx * g3 * g2 / g0 + g4);
+  vx = kAlpha * y * (1.33333333333 * F - t / s * dF - (u - 1.3333333333 * s3) * dsF);
+}
+
+// Becke 88 exchange for the closed shell (each spin channel holds rho / 2);
+// its potential needs the gradient of |grad rho| (from the Hessian).
+__device__ __host__ inline void x_b88(double rho, double y, const double g[3], double ag,
+                                      const double h1[3], const double h2[3], double lap,
+                                      double& ex, double& vx) {
+  const double beta = 0.0042;
+  const double e0 = kAlpha * y;
+  const double half = rho / 2.0;
+  const double r13 = cbrt(half), r43 = cbrt(pow(half, 4));
+  const double xs = ag / (2.0 * r43);  // reduced gradient of one spin channel
+  const double ash = asinh(xs);
+  const double dn = 1.0 + 6.0 * beta * xs * ash;
+  ex = e0 + (-2.0 * beta * r43 * xs * xs / (dn * rho));
+  const double v0 = 1.33333333333333 * e0;
+  const double fb = 1.0 / dn;
+  const double xa = xs / sqrt(1.0 + xs * xs);
+  const double dn1 = 1.0 + fb * (1.0 - 6.0 * beta * xs * xa);
+  const double dn2 = 1.0 / (1.0 + xs * xs) + 2.0 * fb * (2.0 - 6.0 * beta * xs * xa);
+  const double dn3 = ash * (1.0 + 2.0 * fb) + xa * dn2;
+  const double de1 = 1.33333333333333 / cbrt(pow(rho, 7));
+  const double c2 = cbrt(2.0);
+  // grad |grad rho| components, then the gradient of the channel's x
+  const double dgx = (g[0] * h1[0] + g[1] * h2[0] + g[2] * h2[1]) / ag;
+  const double dgy = (g[0] * h2[0] + g[1] * h1[1] + g[2] * h2[2]) / ag;
+  const double dgz = (g[0] * h2[1] + g[1] * h2[2] + g[2] * h1[2]) / ag;
+  const double inv = 1.0 / (rho * y);
+  const double gxx = c2 * (inv * dgx - de1 * g[0] * ag);
+  const double gxy = c2 * (inv * dgy - de1 * g[1] * ag);
+  const double gxz = c2 * (inv * dgz - de1 * g[2] * ag);
+  const double dn4 = 6.0 * beta * fb * (g[0] / 2.0 * gxx + g[1] / 2.0 * gxy + g[2] / 2.0 * gxz);
+  const double dn5 = 1.33333333333333 * r43 * r13 * xs * xs;
+  const double tot = dn5 - lap / 2.0 * dn1 + dn4 * dn3;
+  vx = v0 + (-beta * fb / r43 * tot);
+}
+
+// VWN correlation as the GGA branch writes it (parameters A, b, c, x0, Q).
+__device__ __host__ inline void c_vwn(double y, double& ec, double& vc) {
+  const double A = 0.03109205, b = 3.72744, c = 12.9352, x0 = -0.10498,
+               Q = 6.15199066246304849, A16 = 0.005182008333, Q2 = 4.7309269;
+  const double rs = kRsC / y, x = sqrt(rs);
+  const double X = rs + b * x + c;
+  const double X0 = x0 * x0 + b * x0 + c;
+  const double t1 = 2.0 * x + b, lX = log(X), at = atan(Q / t1);
+  const double bx = b * x0 / X0;
+  ec = A * (2.0 * log(x) - lX + 2.0 * b / Q * at -
+            bx * (2.0 * log(x - x0) - lX + 2.0 * (b + 2.0 * x0) / Q * at));
+  const double t5 = (b * x + 2.0 * c) / x;
+  vc = ec - A16 * x * (t5 / X - 4.0 * b / (t1 * t1 + Q * Q2) * (1.0 - x0 / X0 * (b - 2.0 * x0)) -
+                       bx * (2.0 / (x - x0) - t1 / X));
+}
+
+// Perdew 86 gradient correction on top of VWN: ec and vc are VWN's on entry.
+__device__ __host__ inline void c_p86(double rho, double y, double g2, double ag, double lap,
+                                      double u0, double& ec, double& vc) {
+  const double alf = 0.023266, bet = 7.389, gam = 8.723, del = 0.472;
+  const double rs = kRsC / y, rs2 = rs * rs;
+  const double n1 = (double)0.002568f + alf * rs + bet * rs2;
+  const double d1 = (double)1.0f + gam * rs + del * rs2 + 1.0e4 * bet * (rs * rs * rs);
+  const double C = 0.001667 + n1 / d1;
+  const double n1p = alf + (double)2.0f * bet * rs;
+  const double d1p = gam + (double)2.0f * del * rs + 3.0e4 * bet * rs2;
+  double dC = n1p / d1 - n1 / (d1 * d1) * d1p;
+  dC = (double)-0.333333333333333f * dC * kRsC / (y * rho);
+  const double phi = (double)0.0008129082f / C * ag / pow(rho, (double)(7.0f / 6.0f));
+  const double ep = exp(-phi);
+  const double e0 = ep * C;
+  ec = ec + e0 * g2 / (y * (rho * rho));
+  const double D1 = ((double)2.0f - phi) * lap / rho;
+  const double phi2 = phi * phi;
+  double D2 = (double)1.33333333333333333f - (double)3.666666666666666666f * phi +
+              (double)1.166666666666666f * phi2;
+  D2 = D2 * g2 / (rho * rho);
+  const double D3 = phi * (phi - (double)3.0f) * u0 / (rho * ag);
+  const double D4 = ep * g2 / (y * rho) * (phi2 - phi - (double)1.0f) * dC;
+  vc = vc - 1.0 * (e0 / y * (D1 - D2 + D3) - D4);
+}
+
+// Lee-Yang-Parr correlation in the Laplacian form of the reference.
+__device__ __host__ inline void c_lyp(double rho, double g2, double lap, double& ec, double& vc) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, c3 = 0.0844333333, d = 0.349,
+               d3 = 0.116333333, cf = 2.87123400018819;
+  const double rm13 = 1 / cbrt(rho);
+  const double r53 = cbrt(pow(rho, 5));
+  const double ecr = exp(-c * rm13);
+  const double f1 = (double)1.0f / ((double)1.0f + d * rm13);
+  const double tw = (double)(1.0f / 8.0f) * (g2 / rho - lap);  // Weizsaecker term / 8
+  double term = (tw / (double)9.0f + lap / (double)18.0f) - (double)2.0f * tw + cf * r53;
+  term = rho + b * (rm13 * rm13) * ecr * term;
+  ec = -a * f1 * term / rho;
+  const double h1 = ecr / r53;
+  const double g1 = f1 * h1;
+  const double tm1 = d3 * (rm13 / rho);
+  const double fp1 = tm1 * (f1 * f1);
+  const double tm2 = (double)-1.666666666f + c3 * rm13;
+  const double hp1 = h1 * tm2 / rho;
+  const double gp1 = fp1 * h1 + hp1 * f1;
+  const double fp2 = tm1 * (double)2.0f * f1 * (fp1 - (double)0.6666666666f * f1 / rho);
+  const double tm3 = (double)1.6666666666f - c3 * (double)1.3333333333f * rm13;
+  const double hp2 = hp1 * tm2 / rho + h1 * tm3 / (rho * rho);
+  const double gp2 = fp2 * h1 + (double)2.0f * fp1 * hp1 + hp2 * f1;
+  const double term3 = -a * (fp1 * rho + f1) -
+                       a * b * cf * (gp1 * rho + (double)(8.0f / 3.0f) * g1) * r53;
+  const double term4 = (gp2 * rho * g2 + gp1 * ((double)3.0f * g2 + (double)2.0f * rho * lap) +
+                        (double)4.0f * g1 * lap) * a * b / (double)4.0f;
+  const double term5 = ((double)3.0f * gp2 * rho * g2 +
+                        gp1 * ((double)5.0f * g2 + (double)6.0f * rho * lap) +
+                        (double)4.0f * g1 * lap) * a * b / (double)72.0f;
+  vc = term3 - term4 - term5;
+}
+}  // namespace gga2
+
+// iexch 4-8 point (not inlined: the density kernels that evaluate the
+// functional keep their register allocation for the PBE default).
+__device__ __host__ __noinline__ void gga_point_other(double rho, const double g[3],
+                                                      const double h1[3], const double h2[3],
+                                                      int iexch, double& ex, double& ec,
+                                                      double& y2a) {
+  if (rho < 1e-18) {
+    ex = ec = y2a = 0.0;
+    return;
+  }
+  const double y = cbrt(rho);
+  double g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  if (g2 < DBL_MIN) g2 = DBL_MIN;
+  const double ag = sqrt(g2);
+  const double lap = h1[0] + h1[1] + h1[2];
+  const double u0 = ((g[0] * g[0]) * h1[0] + 2.0 * g[0] * g[1] * h2[0] + 2.0 * g[1] * g[2] * h2[2] +
+                     2.0 * g[0] * g[2] * h2[1] + (g[1] * g[1]) * h1[1] + (g[2] * g[2]) * h1[2]) /
+                    ag;
+  double vx, vc;
+  if (iexch == 4 || iexch == 8)
+    gga2::x_pw86(rho, y, ag, lap, u0, ex, vx);
+  else
+    gga2::x_b88(rho, y, g, ag, h1, h2, lap, ex, vx);
+  if (iexch <= 6) {
+    gga2::c_vwn(y, ec, vc);
+    if (iexch != 6) gga2::c_p86(rho, y, g2, ag, lap, u0, ec, vc);
+  } else {
+    gga2::c_lyp(rho, g2, lap, ec, vc);
+  }
+  y2a = vx + vc;
+}
+
+// Closed-shell GGA point for LIO's iexch: 4-8 as above, PBE for the others
+// (the reference's GGA branch sends iexch 0-3 and 9 to PBE).
+__device__ __host__ inline void gga_point_x(double rho, const double g[3], const double h1[3],
+                                            const double h2[3], int iexch, double& ex,
+                                            double& ec, double& y2a) {
+  if (iexch >= 4 && iexch <= 8)
+    gga_point_other(rho, g, h1, h2, iexch, ex, ec, y2a);
+  else
+    gga_point(rho, g, h1, h2, ex, ec, y2a);
+}
 
 // ---------------------------------------------------------------------------
 // Open shell (spin-polarised) PBE in the same Laplacian potential form

@@ -590,8 +590,7 @@ void pbe(double rho, double agrad, double delgrad, double lap, double& ex, doubl
 }
 
 // GGA point functional from rho, grad rho, (xx,yy,zz) and (xy,xz,yz) second
-// derivatives (calc_ggaCS.h:35-160).  Only PBE is restated; iexch 4..8 are
-// rejected at setup.
+// derivatives (calc_ggaCS.h:35-160): the PBE branch (iexch 0-3 and 9).
 void gga_point(double rho, const double g[3], const double h1[3], const double h2[3],
                double& ex, double& ec, double& y2a) {
   if (rho < 1e-18) {
@@ -768,6 +767,164 @@ void point(double ra, const double ga[3], const double h1a[3], const double h2a[
 
 // LDA: Slater exchange plus (1) nothing, (2) Gunnarsson-Lundqvist or (3) VWN
 // correlation (calc_ldaCS.h:32-102).
+// The GGA branches of iexch 4-8 (calc_ggaCS.h:43-236), step by step in the
+// reference's order of operations, its constants (calc_ggaCS.h:10-33) and its
+// single-precision literals widened to double:
+//   exchange     4, 8: Perdew-Wang 86 (calc_ggaCS.h:67-90)
+//                5-7:  Becke 88 (calc_ggaCS.h:91-135)
+//   correlation  4-6:  VWN (calc_ggaCS.h:161-186), plus for 4 and 5 Perdew's
+//                      1986 gradient term (calc_ggaCS.h:188-215)
+//                7, 8: Lee-Yang-Parr (calc_ggaCS.h:217-253)
+void gga_point_other(double rho, const double g[3], const double h1[3], const double h2[3],
+                     int iexch, double& ex, double& ec, double& y2a) {
+  if (rho < 1e-18) {  // calc_ggaCS.h:43-52 (FULL_DOUBLE)
+    ex = ec = y2a = 0.0;
+    return;
+  }
+  const double alpha = -0.738558766382022447, gl = 0.620350490899400087;
+  const double y = std::cbrt(rho);
+  double grad2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  if (grad2 < std::numeric_limits<double>::min()) grad2 = std::numeric_limits<double>::min();
+  const double dgrad = std::sqrt(grad2);
+  const double d0 = h1[0] + h1[1] + h1[2];
+  const double u0 = ((g[0] * g[0]) * h1[0] + 2.0 * g[0] * g[1] * h2[0] + 2.0 * g[1] * g[2] * h2[2] +
+                     2.0 * g[0] * g[2] * h2[1] + (g[1] * g[1]) * h1[1] + (g[2] * g[2]) * h1[2]) /
+                    dgrad;
+  double vx;
+  if (iexch == 4 || iexch == 8) {  // Perdew-Wang 86 exchange
+    const double rho2 = rho * rho;
+    const double ckf = 3.0936677 * y;
+    const double sr = dgrad / (2.0 * ckf * rho);
+    const double fx = 1.0 / 15.0;
+    const double sr2 = sr * sr, sr3 = sr * sr * sr;
+    const double g0 = 1.0 + 1.296 * sr2 + 14.0 * std::pow(sr, 4) + 0.2 * std::pow(sr, 6);
+    const double F = std::pow(g0, fx);
+    ex = alpha * F * y;
+    const double t = d0 / (rho * 4.0 * (ckf * ckf));
+    const double u = u0 / (std::pow(2.0 * ckf, 3) * rho2);
+    const double g2 = 2.592 * sr + 56.0 * sr3 + 1.2 * std::pow(sr, 5);
+    const double g3 = 2.592 + 56.0 * sr2 + 1.2 * std::pow(sr, 4);
+    const double g4 = 112.0 * sr + 4.8 * sr3;
+    const double dF = fx * F / g0 * g2;
+    const double dsF = fx * F / g0 * (-14.0 * fx * g3 * g2 / g0 + g4);
+    vx = alpha * y * (1.33333333333 * F - t / sr * dF - (u - 1.3333333333 * sr3) * dsF);
+  } else {  // Becke 88 exchange
+    const double beta = 0.0042;
+    const double e0 = alpha * y;
+    const double hr = rho / 2.0;
+    const double r13 = std::cbrt(hr);
+    const double r43 = std::cbrt(std::pow(hr, 4));
+    const double xs = dgrad / (2.0 * r43);
+    const double as = std::asinh(xs);
+    const double DN = 1.0 + 6.0 * beta * xs * as;
+    const double ect = -2.0 * beta * r43 * xs * xs / (DN * rho);
+    ex = e0 + ect;
+    const double v0 = 1.33333333333333 * e0;
+    const double Fb = 1.0 / DN;
+    const double XA1 = xs / std::sqrt(1.0 + xs * xs);
+    const double DN1 = 1.0 + Fb * (1.0 - 6.0 * beta * xs * XA1);
+    const double DN2 = 1.0 / (1.0 + xs * xs) + 2.0 * Fb * (2.0 - 6.0 * beta * xs * XA1);
+    const double DN3 = as * (1.0 + 2.0 * Fb) + XA1 * DN2;
+    const double D02 = d0 / 2.0;
+    const double de1 = 1.33333333333333 / std::cbrt(std::pow(rho, 7));
+    const double c2 = std::cbrt(2.0);
+    const double dgx = (g[0] * h1[0] + g[1] * h2[0] + g[2] * h2[1]) / dgrad;
+    const double dgy = (g[0] * h2[0] + g[1] * h1[1] + g[2] * h2[2]) / dgrad;
+    const double dgz = (g[0] * h2[1] + g[1] * h2[2] + g[2] * h1[2]) / dgrad;
+    const double gx = c2 * (1.0 / (rho * y) * dgx - de1 * g[0] * dgrad);
+    const double gy = c2 * (1.0 / (rho * y) * dgy - de1 * g[1] * dgrad);
+    const double gz = c2 * (1.0 / (rho * y) * dgz - de1 * g[2] * dgrad);
+    const double DN4 = 6.0 * beta * Fb * (g[0] / 2.0 * gx + g[1] / 2.0 * gy + g[2] / 2.0 * gz);
+    const double DN5 = 1.33333333333333 * r43 * r13 * xs * xs;
+    const double TOT2 = DN5 - D02 * DN1 + DN4 * DN3;
+    vx = v0 + (-beta * Fb / r43 * TOT2);
+  }
+  double vc;
+  if (iexch <= 6) {  // VWN, then Perdew 86 for 4 and 5
+    const double A1 = 0.03109205, B1 = 3.72744, C1 = 12.9352, X0 = -0.10498,
+                 Q = 6.15199066246304849, A16 = 0.005182008333, Q2 = 4.7309269;
+    const double rho2 = rho * rho;
+    const double rs = gl / y;
+    const double x1 = std::sqrt(rs);
+    const double Xx = rs + B1 * x1 + C1;
+    const double Xxo = (X0 * X0) + B1 * X0 + C1;
+    const double t1 = 2.0 * x1 + B1;
+    const double t2 = std::log(Xx);
+    const double t3 = std::atan(Q / t1);
+    const double t4 = B1 * X0 / Xxo;
+    ec = A1 * (2.0 * std::log(x1) - t2 + 2.0 * B1 / Q * t3 -
+               t4 * (2.0 * std::log(x1 - X0) - t2 + 2.0 * (B1 + 2.0 * X0) / Q * t3));
+    const double t5 = (B1 * x1 + 2.0 * C1) / x1;
+    const double t6 = X0 / Xxo;
+    vc = ec - A16 * x1 *
+                  (t5 / Xx - 4.0 * B1 / ((t1 * t1) + (Q * Q2)) * (1.0 - t6 * (B1 - 2.0 * X0)) -
+                   t4 * (2.0 / (x1 - X0) - t1 / Xx));
+    if (iexch != 6) {
+      const double alf = 0.023266, bet = 7.389, gam = 8.723, del = 0.472;
+      const double rs2 = rs * rs;
+      const double Cx1 = double(0.002568f) + alf * rs + bet * rs2;
+      const double Cx2 = double(1.0f) + gam * rs + del * rs2 + 1.0e4 * bet * (rs * rs * rs);
+      const double C = 0.001667 + Cx1 / Cx2;
+      const double Cx3 = alf + double(2.0f) * bet * rs;
+      const double Cx4 = gam + double(2.0f) * del * rs + 3.0e4 * bet * rs2;
+      double dC = Cx3 / Cx2 - Cx1 / (Cx2 * Cx2) * Cx4;
+      dC = double(-0.333333333333333f) * dC * gl / (y * rho);
+      const double phi = double(0.0008129082f) / C * dgrad / std::pow(rho, double(7.0f / 6.0f));
+      const double expo = std::exp(-phi);
+      const double ex0 = expo * C;
+      ec = ec + ex0 * grad2 / (y * rho2);
+      const double D1 = (double(2.0f) - phi) * d0 / rho;
+      const double phi2 = phi * phi;
+      double D2 = double(1.33333333333333333f) - double(3.666666666666666666f) * phi +
+                  double(1.166666666666666f) * phi2;
+      D2 = D2 * grad2 / rho2;
+      const double D3 = phi * (phi - double(3.0f)) * u0 / (rho * dgrad);
+      const double D4 = expo * grad2 / (y * rho) * (phi2 - phi - double(1.0f)) * dC;
+      vc = vc - 1.0 * (ex0 / y * (D1 - D2 + D3) - D4);
+    }
+  } else {  // Lee-Yang-Parr
+    const double AL = 0.04918, BL = 0.132, CL = 0.2533, CL3 = 0.0844333333, DL = 0.349,
+                 DL3 = 0.116333333, CF = 2.87123400018819;
+    const double rom13 = 1 / std::cbrt(rho);
+    const double rom53 = std::cbrt(std::pow(rho, 5));
+    const double ecro = std::exp(-CL * rom13);
+    const double f1 = double(1.0f) / (double(1.0f) + DL * rom13);
+    const double tw = double(1.0f / 8.0f) * (grad2 / rho - d0);
+    double term = (tw / double(9.0f) + d0 / double(18.0f)) - double(2.0f) * tw + CF * rom53;
+    term = rho + BL * (rom13 * rom13) * ecro * term;
+    ec = -AL * f1 * term / rho;
+    const double hh1 = ecro / rom53;
+    const double gg1 = f1 * hh1;
+    const double tm1 = DL3 * (rom13 / rho);
+    const double fp1 = tm1 * (f1 * f1);
+    const double tm2 = double(-1.666666666f) + CL3 * rom13;
+    const double hp1 = hh1 * tm2 / rho;
+    const double gp1 = fp1 * hh1 + hp1 * f1;
+    const double fp2 = tm1 * double(2.0f) * f1 * (fp1 - double(0.6666666666f) * f1 / rho);
+    const double tm3 = double(1.6666666666f) - CL3 * double(1.3333333333f) * rom13;
+    const double hp2 = hp1 * tm2 / rho + hh1 * tm3 / (rho * rho);
+    const double gp2 = fp2 * hh1 + double(2.0f) * fp1 * hp1 + hp2 * f1;
+    const double term3 = -AL * (fp1 * rho + f1) -
+                         AL * BL * CF * (gp1 * rho + double(8.0f / 3.0f) * gg1) * rom53;
+    const double term4 = (gp2 * rho * grad2 + gp1 * (double(3.0f) * grad2 + double(2.0f) * rho * d0) +
+                          double(4.0f) * gg1 * d0) * AL * BL / double(4.0f);
+    const double term5 = (double(3.0f) * gp2 * rho * grad2 +
+                          gp1 * (double(5.0f) * grad2 + double(6.0f) * rho * d0) +
+                          double(4.0f) * gg1 * d0) * AL * BL / double(72.0f);
+    vc = term3 - term4 - term5;
+  }
+  y2a = vx + vc;
+}
+
+// The GGA point of LIO's closed-shell path for any iexch of the GGA branch.
+void gga_point_lio(double rho, const double g[3], const double h1[3], const double h2[3],
+                   int iexch, double& ex, double& ec, double& y2a) {
+  if (iexch >= 4 && iexch <= 8)
+    gga_point_other(rho, g, h1, h2, iexch, ex, ec, y2a);
+  else
+    gga_point(rho, g, h1, h2, ex, ec, y2a);
+}
+
 void lda_point(double rho, int iexch, double& ex, double& ec, double& y2a) {
   if (rho == 0) {
     ex = ec = y2a = 0;
@@ -886,7 +1043,7 @@ void solve_group(const Problem& pb, const Group& g, const double* rho, double& e
         d2z += gy[i] * wz + gz[i] * wy + hyz[i] * w + w2z * fi;
       }
       const double gr[3] = {dx, dy, dz}, h1[3] = {d1x, d1y, d1z}, h2[3] = {d2x, d2y, d2z};
-      gga_point(dens, gr, h1, h2, ex, ec, y2a);
+      gga_point_lio(dens, gr, h1, h2, pb.iexch, ex, ec, y2a);
     }
     const double wp = g.w[p];
     energy += (dens * wp) * (ex + ec);
@@ -1104,10 +1261,6 @@ void* oracle_setup(const XcInput* in) {
   try {
     auto* h = new Handle;
     load(h->pb, in);
-    if (h->pb.gga && h->pb.iexch != 9) {
-      delete h;
-      return nullptr;  // only PBE is restated among the GGAs
-    }
     const double t0 = now();
     build_partition(*h);
     h->grid_seconds = now() - t0;
@@ -1210,8 +1363,7 @@ void oracle_free(void* hp) { delete static_cast<Handle*>(hp); }
 
 void oracle_gga_point(double rho, const double* g, const double* h1, const double* h2,
                       int iexch, double* ex, double* ec, double* y2a) {
-  (void)iexch;
-  gga_point(rho, g, h1, h2, *ex, *ec, *y2a);
+  gga_point_lio(rho, g, h1, h2, iexch, *ex, *ec, *y2a);
 }
 
 void oracle_lda_point(double rho, int iexch, double* ex, double* ec, double* y2a) {
@@ -1239,6 +1391,13 @@ int oracle_basis_values(const XcInput* in, int npts, const double* pts, int M, d
 void oracle_gga_points(int n, const double* rho, const double* grad, const double* h1,
                        const double* h2, double* ex, double* ec, double* y2a) {
   for (int i = 0; i < n; ++i) gga_point(rho[i], grad + 3 * i, h1 + 3 * i, h2 + 3 * i, ex[i], ec[i], y2a[i]);
+}
+
+// The same for any iexch of the GGA branch (4-8, PBE otherwise).
+void oracle_gga_points_x(int n, const double* rho, const double* grad, const double* h1,
+                         const double* h2, int iexch, double* ex, double* ec, double* y2a) {
+  for (int i = 0; i < n; ++i)
+    gga_point_lio(rho[i], grad + 3 * i, h1 + 3 * i, h2 + 3 * i, iexch, ex[i], ec[i], y2a[i]);
 }
 
 // Spin-polarised PBE points: per point the two channels' rho, gradient (3),

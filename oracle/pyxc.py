@@ -199,6 +199,8 @@ class Oracle(_Checker):
         L.oracle_basis_values.restype = C.c_int
         L.oracle_gga_points.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_gga_points.restype = None
+        L.oracle_gga_points_x.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.oracle_gga_points_x.restype = None
         L.oracle_gga_points_open.argtypes = [C.c_int] + [_dp] * 12
         L.oracle_gga_points_open.restype = None
         L.oracle_lda_points.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, _dp]
@@ -300,16 +302,17 @@ class Oracle(_Checker):
                                   C.byref(v))
         return ex.value, ec.value, v.value
 
-    def gga_points(self, rho, grad, hess1, hess2):
+    def gga_points(self, rho, grad, hess1, hess2, iexch=9):
         rho = np.ascontiguousarray(rho, np.float64)
         n = len(rho)
         g = np.ascontiguousarray(grad, np.float64).reshape(n, 3)
         h1 = np.ascontiguousarray(hess1, np.float64).reshape(n, 3)
         h2 = np.ascontiguousarray(hess2, np.float64).reshape(n, 3)
         ex, ec, v = np.zeros(n), np.zeros(n), np.zeros(n)
-        self.lib.oracle_gga_points(n, _ptr(rho, C.c_double), _ptr(g, C.c_double),
-                                   _ptr(h1, C.c_double), _ptr(h2, C.c_double),
-                                   _ptr(ex, C.c_double), _ptr(ec, C.c_double), _ptr(v, C.c_double))
+        self.lib.oracle_gga_points_x(n, _ptr(rho, C.c_double), _ptr(g, C.c_double),
+                                     _ptr(h1, C.c_double), _ptr(h2, C.c_double), iexch,
+                                     _ptr(ex, C.c_double), _ptr(ec, C.c_double),
+                                     _ptr(v, C.c_double))
         return ex, ec, v
 
     def gga_points_open(self, ra, ga, h1a, h2a, rb, gb, h1b, h2b):

@@ -30,21 +30,35 @@ from lio_amd import systems  # noqa: E402
 MOLECULES = [(3, 101), (10, 102), (25, 103), (45, 104), (58, 105), (60, 106), (195, 107)]
 
 
+# the other closed-shell GGAs (iexch 4-8): water, and two molecules with
+# Becke 88 + LYP (7) and Perdew-Wang 86 + Perdew 86 (4)
+GGA_MOLECULES = [(25, 103, 7), (10, 102, 4)]
+
+
+def _with_iexch(s, ix):
+    s.iexch = ix
+    return s
+
+
 def cases():
     yield "c0_pbe", lambda: systems.config0()
     for ix in (1, 2, 3):
-        def b(ix=ix):
-            s = systems.config0()
-            s.iexch = ix
-            return s
-        yield "c0_lda%d" % ix, b
+        yield "c0_lda%d" % ix, (lambda ix=ix: _with_iexch(systems.config0(), ix))
+    for ix in (4, 5, 6, 7, 8):
+        yield "c0_gga%d" % ix, (lambda ix=ix: _with_iexch(systems.config0(), ix))
     yield "c1_pbe", lambda: systems.config1()
     for n, seed in MOLECULES:
         yield "mol%d_s%d_pbe" % (n, seed), (lambda n=n, seed=seed: systems.random_molecule(n, seed))
+    for n, seed, ix in GGA_MOLECULES:
+        yield "mol%d_s%d_gga%d" % (n, seed, ix), \
+            (lambda n=n, seed=seed, ix=ix: _with_iexch(systems.random_molecule(n, seed), ix))
 
 
 # open shell (two more alpha than beta electrons), PBE
+# LIO's open-shell GGA is PBE whatever iexch says (calc_ggaOS.h takes none):
+# c0_gga7_open2 records that
 OPEN_CASES = [("c0_pbe_open2", lambda: systems.open_shell(systems.config0())),
+              ("c0_gga7_open2", lambda: systems.open_shell(_with_iexch(systems.config0(), 7))),
               ("mol10_s102_pbe_open2", lambda: systems.open_shell(systems.random_molecule(10, 102))),
               ("mol25_s103_pbe_open2", lambda: systems.open_shell(systems.random_molecule(25, 103)))]
 

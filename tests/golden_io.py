@@ -10,19 +10,26 @@ GROUP_FIELDS = ("kind", "npoints", "nfuncs", "s", "p", "d", "nnuc", "wsum", "xsu
 
 
 def system_for(name: str):
+    """The input system of a fixture: c0_* (water; _lda1-3 and _gga4-8 set
+    iexch), c1_pbe ((H2O)32), mol{n}_s{seed}_pbe / _gga{iexch} (seeded
+    random molecules), any of them with _open2 (two unpaired electrons)."""
     if name.endswith("_open2"):
         return systems.open_shell(system_for(name[:-len("_open2")]))
-    if name.startswith("c0_"):
-        s = systems.config0()
-        if name.startswith("c0_lda"):
-            s.iexch = int(name[len("c0_lda")])
-        return s
     if name == "c1_pbe":
         return systems.config1()
-    if name.startswith("mol"):
-        n, seed = name[3:].split("_pbe")[0].split("_s")
-        return systems.random_molecule(int(n), int(seed))
-    raise KeyError(name)
+    base, kind = name.rsplit("_", 1)
+    if base == "c0":
+        s = systems.config0()
+    elif base.startswith("mol"):
+        n, seed = base[3:].split("_s")
+        s = systems.random_molecule(int(n), int(seed))
+    else:
+        raise KeyError(name)
+    if kind.startswith("lda") or kind.startswith("gga"):
+        s.iexch = int(kind[3:])
+    elif kind != "pbe":
+        raise KeyError(name)
+    return s
 
 
 def load(name: str):

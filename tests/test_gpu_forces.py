@@ -36,7 +36,8 @@ def _forces(s, precision="fp64"):
 
 
 @pytest.mark.parametrize("name", ["c0_pbe", "c0_lda1", "c0_lda2", "c0_lda3", "mol3_s101_pbe",
-                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe"])
+                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe",
+                                  "c0_gga5", "c0_gga8", "mol10_s102_gga4", "mol25_s103_gga7"])
 def test_forces_match_golden(name):
     _gpu()
     gold = load(name)

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-8
 TOL_MIXED = 1e-6
-CASES = ["c0_pbe_open2", "mol10_s102_pbe_open2", "mol25_s103_pbe_open2"]
+CASES = ["c0_pbe_open2", "mol10_s102_pbe_open2", "mol25_s103_pbe_open2", "c0_gga7_open2"]
 
 
 def _gpu():

@@ -23,7 +23,8 @@ def _gpu():
 
 
 @pytest.mark.parametrize("name", ["c0_pbe", "c0_lda1", "c0_lda2", "c0_lda3", "mol3_s101_pbe",
-                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe"])
+                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe",
+                                  "c0_gga4", "c0_gga5", "c0_gga6", "c0_gga7", "c0_gga8", "mol10_s102_gga4", "mol25_s103_gga7"])
 def test_context_matches_golden(name):
     _gpu()
     from lio_amd import g2g

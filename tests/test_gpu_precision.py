@@ -37,7 +37,8 @@ def _gpu():
 
 
 CASES = [(n, p) for p in ("fp32", "bf16x6")
-         for n in ("c0_pbe", "c0_lda3", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe")]
+         for n in ("c0_pbe", "c0_lda3", "mol25_s103_pbe", "mol45_s104_pbe", "c1_pbe",
+                   "c0_gga4", "c0_gga6", "mol10_s102_gga4", "mol25_s103_gga7")]
 
 
 @pytest.mark.parametrize("name,precision", CASES)

@@ -29,7 +29,8 @@ def _compare_forces(o, gold):
 
 
 @pytest.mark.parametrize("name", ["c0_pbe", "c0_lda1", "c0_lda2", "c0_lda3", "mol3_s101_pbe",
-                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe"])
+                                  "mol10_s102_pbe", "mol25_s103_pbe", "mol45_s104_pbe",
+                                  "c0_gga4", "c0_gga5", "c0_gga6", "c0_gga7", "c0_gga8", "mol10_s102_gga4", "mol25_s103_gga7"])
 def test_oracle_matches_golden(name):
     gold = load(name)
     s = system_for(name)
@@ -55,10 +56,12 @@ def test_oracle_matches_golden_water_cluster():
     _compare_forces(o, gold)
 
 
-@pytest.mark.parametrize("name", ["c0_pbe_open2", "mol10_s102_pbe_open2", "mol25_s103_pbe_open2"])
+@pytest.mark.parametrize("name", ["c0_pbe_open2", "mol10_s102_pbe_open2", "mol25_s103_pbe_open2",
+                                  "c0_gga7_open2"])
 def test_oracle_open_shell_matches_golden(name):
-    """Open shell (two unpaired electrons, PBE): energy, both spins' Fock
-    matrices, electron count and forces of LIO's CPU path."""
+    """Open shell (two unpaired electrons, PBE -- LIO's open-shell GGA is PBE
+    whatever iexch says): energy, both spins' Fock matrices, electron count
+    and forces of LIO's CPU path."""
     gold = load(name)
     s = system_for(name)
     check_inputs(s, gold)
