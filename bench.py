@@ -235,8 +235,9 @@ def bench_batch(precision, nmol=64, seed=3, reps=5):
     t0 = time.perf_counter()
     sched = b.autotune(rho.data_ptr(), F.data_ptr(), path=sched_path)
     t_tune = time.perf_counter() - t0
-    if not sched["loaded"] and os.path.isdir(os.path.join(ROOT, "gpurun_out")):
-        with open(os.path.join(ROOT, "gpurun_out", os.path.basename(sched_path)), "w") as f:
+    out_dir = os.environ.get("LIO_AMD_RESULTS_DIR", "")  # keep a freshly tuned schedule
+    if not sched["loaded"] and out_dir and os.path.isdir(out_dir):
+        with open(os.path.join(out_dir, os.path.basename(sched_path)), "w") as f:
             json.dump(sched, f, indent=1)
     b.solve_device(rho.data_ptr(), F.data_ptr())
     b.energies()

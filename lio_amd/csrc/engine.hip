@@ -2788,7 +2788,7 @@ __global__ __launch_bounds__(1024) void k_energy(const double* __restrict__ epar
 }
 
 // ---------------------------------------------------------------------------
-// One persistent launch per build (split-BF16 default density, closed shell)
+// One persistent launch per build (split-BF16 density variant 8, closed shell)
 // ---------------------------------------------------------------------------
 // Every phase of a build -- density expansion, density and functional, Fock
 // tiles, tile sums over slices, packing, energies -- as items of one queue,

@@ -13,8 +13,8 @@ molecules of 3-60 atoms in one batch) and the served workload (QM regions of
 * Three replicas of 150-200 atoms served from a device slab under a budget
   that forces evictions: every served step equals the same step alone.
 
-The measured errors are printed and, when gpurun_out/ exists, appended to
-gpurun_out/scale_errors.jsonl.
+The measured errors are printed and, when the directory named by
+LIO_AMD_RESULTS_DIR exists, appended to scale_errors.jsonl there.
 """
 import json
 import os
@@ -29,7 +29,7 @@ from golden_io import check_inputs, fock_errors, load, system_for
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-OUT = os.path.join(ROOT, "gpurun_out")
+OUT = os.environ.get("LIO_AMD_RESULTS_DIR", "")  # where to append the error records, if set
 GOLD_BATCH = ["mol58_s105_pbe", "mol60_s106_pbe"]
 TOL = {"fp64": 1e-8, "bf16x6": 1e-6}
 
@@ -43,7 +43,7 @@ def _torch():
 
 def _record(**rec):
     print(json.dumps(rec), flush=True)
-    if os.path.isdir(OUT):
+    if OUT and os.path.isdir(OUT):
         with open(os.path.join(OUT, "scale_errors.jsonl"), "a") as f:
             f.write(json.dumps(rec) + "\n")
 
