@@ -3052,34 +3052,77 @@ __device__ inline double becke_cell_w(double dj, double dk, double rjk, double a
   return 0.5 * (1.0 - u);
 }
 
-// One staged point per thread, one block per 256 points of one staged
-// group: weight *= P_own / sum_j P_j over the group's nuclei, the own atom's
-// cell taken separately when it is not among them.  The group is uniform in
-// the block, so its nuclei, their coordinates and the pair tables are
-// wave-uniform loads.  Distances are recomputed instead of stored per point
-// (the host keeps them in an array), with identical values.
-//
-// Each P_j is first tested against the point's nearest nucleus n: when
-// s(mu_jn) is exactly zero the product is zero whatever the order of its
-// other factors (all lie in [0, 1]), and a wave runs the k-term product only
-// for nuclei whose cells reach one of its points.  Zero terms added to the
-// sum change nothing, and a zero own cell gives weight 0 however far the sum
-// got, so the weight is bitwise the plain double loop's
-// (g2g/cpu/weight.cpp:13-112 without the early exit).
-__global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ groups,
-                                               const int2* __restrict__ items,
-                                               const StMol* __restrict__ mols,
-                                               const int* __restrict__ gatoms,
-                                               const int* __restrict__ owner,
-                                               const double* __restrict__ ax,
-                                               const double* __restrict__ ay,
-                                               const double* __restrict__ az,
-                                               const double* __restrict__ aw,
-                                               const double* __restrict__ rij,
-                                               const double* __restrict__ sx,
-                                               const double* __restrict__ sy,
-                                               const double* __restrict__ sz,
-                                               double* __restrict__ sw) {
+// Staged points per Becke block: one wave, so that a block's cost follows
+// its own points only.
+constexpr int kBeckePts = 64;
+
+// The near-first order of every staged group's nuclei: the nuclei by
+// distance from the group's first staged point, ties in list order (the
+// rank of q counts the nuclei strictly nearer, or as near and earlier), the
+// order the host's becke_order forms with the same operations; gord holds the
+// nucleus indices in that order.  One block per group, a thread per nucleus.
+__global__ __launch_bounds__(256) void k_becke_order(const StGroup* __restrict__ groups,
+                                                     const StMol* __restrict__ mols,
+                                                     const int* __restrict__ gatoms,
+                                                     const double* __restrict__ ax,
+                                                     const double* __restrict__ ay,
+                                                     const double* __restrict__ az,
+                                                     const double* __restrict__ sx,
+                                                     const double* __restrict__ sy,
+                                                     const double* __restrict__ sz,
+                                                     int* __restrict__ gord) {
+  const StGroup G = groups[blockIdx.x];
+  if (G.npts == 0) return;
+  const StMol m = mols[G.mol];
+  const int* at = gatoms + G.atom_off;
+  const int k = G.natoms;
+  const double* A = ax + m.atom_base;
+  const double* B = ay + m.atom_base;
+  const double* C = az + m.atom_base;
+  const double x = sx[G.pt_off], y = sy[G.pt_off], z = sz[G.pt_off];
+  for (int q = threadIdx.x; q < k; q += blockDim.x) {
+    const double dq = dist3(x, y, z, A[at[q]], B[at[q]], C[at[q]]);
+    int rank = 0;
+    for (int r = 0; r < k; ++r) {
+      const double dr = dist3(x, y, z, A[at[r]], B[at[r]], C[at[r]]);
+      rank += (dr < dq || (dr == dq && r < q)) ? 1 : 0;
+    }
+    gord[G.atom_off + rank] = at[q];
+  }
+}
+
+// One staged point per thread, one block per kBeckePts points of one staged
+// group: weight *= P_own / sum_j P_j over the group's nuclei,
+// P_j = prod_{q != j} s(mu_jq), every factor in [0, 1].  The group is
+// uniform in the block, so its nuclei, their coordinates and the pair tables
+// are wave-uniform loads; distances are recomputed instead of stored per
+// point, with the host's values.  The host's weigh (grid.cpp) does the same
+// arithmetic, so the weights are bitwise its weights:
+// P_own is formed as the reference forms it (a zero test against the point's
+// nearest nucleus n, then the nuclei in list order), so a weight is zero
+// exactly when the reference's is, and a zero P_own ends the point.  Every
+// other P_j runs over the nuclei nearest-first (gord, from the group's first
+// point) and is dropped once its running product, an upper bound of P_j,
+// falls below 2^-60 P_own: the sum holds P_own, so a dropped term lies far
+// below half an ulp of it.  A point whose own nucleus is not among the
+// group's keeps the plain double loop (g2g/cpu/weight.cpp:13-112).  The
+// loops over j are wave-uniform: P_own is formed at the j of each owner a
+// lane of the wave has.
+__global__ __launch_bounds__(kBeckePts) void k_becke(const StGroup* __restrict__ groups,
+                                                     const int2* __restrict__ items,
+                                                     const StMol* __restrict__ mols,
+                                                     const int* __restrict__ gatoms,
+                                                     const int* __restrict__ gord,
+                                                     const int* __restrict__ owner,
+                                                     const double* __restrict__ ax,
+                                                     const double* __restrict__ ay,
+                                                     const double* __restrict__ az,
+                                                     const double* __restrict__ aw,
+                                                     const double* __restrict__ rij,
+                                                     const double* __restrict__ sx,
+                                                     const double* __restrict__ sy,
+                                                     const double* __restrict__ sz,
+                                                     double* __restrict__ sw) {
 #pragma clang fp contract(off)
   const int2 it = items[blockIdx.x];
   const int gi = __builtin_amdgcn_readfirstlane(it.x);
@@ -3088,6 +3131,7 @@ __global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ group
   if (p >= G.pt_off + G.npts) return;
   const StMol m = mols[G.mol];
   const int* at = gatoms + G.atom_off;
+  const int* ord = gord + G.atom_off;
   const int k = G.natoms, own = owner[p], na = m.natom;
   const double* A = ax + m.atom_base;
   const double* B = ay + m.atom_base;
@@ -3095,25 +3139,22 @@ __global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ group
   const double* W = aw + m.rij_off;
   const double* D = rij + m.rij_off;
   const double x = sx[p], y = sy[p], z = sz[p];
-  int an = -1;
+  int an = -1, qo = -1;
   double dn = 0.0;
-  bool own_in = false;
   for (int q = 0; q < k; ++q) {
     const int aq = at[q];
     const double dq = dist3(x, y, z, A[aq], B[aq], C[aq]);
-    own_in |= (aq == own);
+    if (aq == own) qo = q;
     if (an < 0 || dq < dn) {
       dn = dq;
       an = aq;
     }
   }
-  double total = 0.0, mine = 0.0;
-  for (int j = 0; j < k; ++j) {
-    const int aj = at[j];
-    const double dj = dist3(x, y, z, A[aj], B[aj], C[aj]);
+  // the reference's P_j of nucleus j (list position), at distance dj
+  auto exact = [&](int aj, double dj) {
     const double* Dj = D + (long long)aj * na;
     const double* Wj = W + (long long)aj * na;
-    if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
+    if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) return 0.0;
     double prod = 1.0;
     for (int q = 0; q < k; ++q) {
       const int ak = at[q];
@@ -3121,15 +3162,52 @@ __global__ __launch_bounds__(256) void k_becke(const StGroup* __restrict__ group
       prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
       if (prod == 0.0) break;
     }
-    if (aj == own) mine = prod;
-    total += prod;
-  }
-  if (!own_in && k > 0) {
+    return prod;
+  };
+  double total = 0.0, mine = 0.0;
+  if (qo >= 0) {
+    for (int j = 0; j < k; ++j) {  // P_own at the owners of the wave's points
+      if (j != qo) continue;
+      const int aj = at[j];
+      mine = exact(aj, dist3(x, y, z, A[aj], B[aj], C[aj]));
+    }
+    if (mine == 0.0) {
+      sw[p] = 0.0;
+      return;
+    }
+    const double thr = ldexp(mine, -60);
+    for (int j = 0; j < k; ++j) {
+      if (j == qo) {
+        total += mine;
+        continue;
+      }
+      const int aj = at[j];
+      const double dj = dist3(x, y, z, A[aj], B[aj], C[aj]);
+      const double* Dj = D + (long long)aj * na;
+      const double* Wj = W + (long long)aj * na;
+      if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
+      double prod = 1.0;
+      for (int t = 0; t < k; ++t) {
+        const int ak = ord[t];
+        if (ak == aj) continue;
+        prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
+        if (prod < thr) {
+          prod = 0.0;
+          break;
+        }
+      }
+      total += prod;
+    }
+  } else {
+    for (int j = 0; j < k; ++j) {
+      const int aj = at[j];
+      total += exact(aj, dist3(x, y, z, A[aj], B[aj], C[aj]));
+    }
     const double down = dist3(x, y, z, A[own], B[own], C[own]);
     const double* Do = D + (long long)own * na;
     const double* Wo = W + (long long)own * na;
     mine = 1.0;
-    if (becke_cell_w(down, dn, Do[an], Wo[an]) == 0.0)
+    if (k > 0 && becke_cell_w(down, dn, Do[an], Wo[an]) == 0.0)
       mine = 0.0;
     else
       for (int q = 0; q < k; ++q) {
@@ -4510,7 +4588,7 @@ struct StageDev {
   StGroup* groups;
   StMol* mols;
   int2* bitems;
-  int *owner, *gatoms, *count;
+  int *owner, *gatoms, *gord, *count;
   double *ax, *ay, *az, *aw, *rij, *x, *y, *z, *w;
 };
 
@@ -4539,6 +4617,7 @@ void carve_stage(Carver& c, const StageHost& h, long long n_points, StageDev& d)
   d.bitems = c.take<int2>(h.bitems.size() + 1);
   d.owner = c.take<int>(n_points + 1);
   d.gatoms = c.take<int>(h.gatoms.size() + 1);
+  d.gord = c.take<int>(h.gatoms.size() + 1);
   d.count = c.take<int>(h.sg.size() + 1);
   d.ax = c.take<double>(h.ax.size() + 1);
   d.ay = c.take<double>(h.ay.size() + 1);
@@ -4580,17 +4659,26 @@ std::vector<GridPartition> Engine::weigh_and_load(const std::vector<const Molecu
   };
   up(d.groups, h.sg.data(), h.sg.size() * sizeof(StGroup));
   up(d.mols, h.sm.data(), h.sm.size() * sizeof(StMol));
-  up(d.bitems, h.bitems.data(), h.bitems.size() * sizeof(int2));
+  // the Becke blocks heaviest first (cost ~ natoms^2), so that the longest
+  // ones do not start last
+  std::vector<int2> bitems = h.bitems;
+  std::stable_sort(bitems.begin(), bitems.end(), [&](const int2& a, const int2& b) {
+    return h.sg[a.x].natoms > h.sg[b.x].natoms;
+  });
+  up(d.bitems, bitems.data(), bitems.size() * sizeof(int2));
   up(d.gatoms, h.gatoms.data(), h.gatoms.size() * sizeof(int));
   up(d.ax, h.ax.data(), h.ax.size() * sizeof(double));
   up(d.ay, h.ay.data(), h.ay.size() * sizeof(double));
   up(d.az, h.az.data(), h.az.size() * sizeof(double));
   up(d.aw, h.aw.data(), h.aw.size() * sizeof(double));
   up(d.rij, h.rij.data(), h.rij.size() * sizeof(double));
+  if (nst > 0)
+    hipLaunchKernelGGL(k_becke_order, dim3(nst), dim3(256), 0, s, d.groups, d.mols, d.gatoms,
+                       d.ax, d.ay, d.az, d.x, d.y, d.z, d.gord);
   if (!h.bitems.empty())
-    hipLaunchKernelGGL(k_becke, dim3((unsigned)h.bitems.size()), dim3(256), 0, s, d.groups,
-                       d.bitems, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.aw, d.rij, d.x,
-                       d.y, d.z, d.w);
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)h.bitems.size()), dim3(kBeckePts), 0, s, d.groups,
+                       d.bitems, d.mols, d.gatoms, d.gord, d.owner, d.ax, d.ay, d.az, d.aw, d.rij,
+                       d.x, d.y, d.z, d.w);
   const int drop = opt.remove_zero_weights ? 1 : 0;
   if (nst > 0)
     hipLaunchKernelGGL(k_group_count, dim3(nst), dim3(256), 0, s, d.groups, d.w, drop, d.count);
@@ -4675,7 +4763,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
       sz.insert(sz.end(), gp.z.begin() + b, gp.z.begin() + b + G.npts);
       sw.insert(sw.end(), gp.w.begin() + b, gp.w.begin() + b + G.npts);
       owner.insert(owner.end(), st.owner.begin() + b, st.owner.begin() + b + G.npts);
-      for (int o = 0; o < G.npts; o += 256)
+      for (int o = 0; o < G.npts; o += kBeckePts)
         h.bitems.push_back(make_int2((int)h.sg.size(), G.pt_off + o));
       h.sg.push_back(G);
     }
@@ -4798,7 +4886,7 @@ std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecu
   bound.sg.resize(nbins);
   bound.sm = h.sm;
   bound.gatoms.resize((size_t)naslot);
-  bound.bitems.resize((size_t)(nraw / 256 + nbins));
+  bound.bitems.resize((size_t)(nraw / kBeckePts + nbins));
   bound.ax = h.ax;
   bound.ay = h.ay;
   bound.az = h.az;
@@ -4954,7 +5042,7 @@ std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecu
     G.mol = k;
     G.pad_ = 0;
     h.gatoms.insert(h.gatoms.end(), pa.begin() + it.aoff, pa.begin() + it.aoff + bi.nnuc);
-    for (int o2 = 0; o2 < G.npts; o2 += 256)
+    for (int o2 = 0; o2 < G.npts; o2 += kBeckePts)
       h.bitems.push_back(make_int2((int)h.sg.size(), G.pt_off + o2));
     h.sg.push_back(G);
   }

@@ -85,6 +85,24 @@ void select(const Molecule& mol, const Options& opt, const GridFrame& fr,
     if (owner[a]) s.atoms.push_back(a);
 }
 
+// The group's atoms nearest-first from (x, y, z) (its first staged point):
+// list positions sorted by distance, ties in list order.  The device forms
+// the same order with the same operations (k_becke_order).
+std::vector<int> becke_order(const Molecule& mol, const std::vector<int>& atoms, double x,
+                             double y, double z) {
+  const int k = (int)atoms.size();
+  std::vector<double> d(k);
+  for (int q = 0; q < k; ++q) {
+    const int a = atoms[q];
+    const double dx = x - mol.x[a], dy = y - mol.y[a], dz = z - mol.z[a];
+    d[q] = std::sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  std::vector<int> ord(k);
+  for (int q = 0; q < k; ++q) ord[q] = q;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+  return ord;
+}
+
 // Becke cell function s(mu_jk) with the size adjustment of the atom pair.
 inline double cell(double dj, double dk, double rjk, double rmj, double rmk) {
   double u = (dj - dk) / rjk;
@@ -98,13 +116,23 @@ inline double cell(double dj, double dk, double rjk, double rmj, double rmk) {
 }
 
 // Multiplies the staged weights by the Becke partition over the group's own
-// atoms and drops points whose weight became exactly zero.  P_j is tested
-// first against the point's nearest atom n: a zero s(mu_jn) makes the product
-// zero whatever the order of its other factors (all in [0, 1]); otherwise the
-// product runs over all atoms in the plain order.
+// atoms and drops points whose weight became exactly zero:
+// w *= P_own / sum_j P_j, P_j = prod_{q != j} s(mu_jq), every factor in [0, 1].
+// P_own is formed exactly as the reference forms it (a zero test against the
+// point's nearest atom n, then the atoms in list order), so a weight is zero
+// exactly when the reference's is, and a zero P_own ends the point.  Every
+// other P_j runs over the group's atoms nearest-first (becke_order: the same
+// order for all points of the group) and is dropped once its running product
+// -- an upper bound of P_j -- falls below 2^-60 P_own: the sum holds P_own, so
+// a dropped term is far below half an ulp of it.  The sum differs from the
+// reference's by a few ulps at most (the kept P_j are products in another
+// order).  A point whose own atom is not among the group's keeps the plain
+// loop.
 void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& rij,
            const std::vector<int>& atoms, Staging& st) {
   const int n = (int)st.x.size(), k = (int)atoms.size(), na = mol.natom;
+  const std::vector<int> ord = becke_order(mol, atoms, n > 0 ? st.x[0] : 0.0,
+                                           n > 0 ? st.y[0] : 0.0, n > 0 ? st.z[0] : 0.0);
 #pragma omp parallel for schedule(dynamic, 64) if (n > 256)
   for (int p = 0; p < n; ++p) {
     const int own = st.atom[p];
@@ -116,35 +144,62 @@ void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& r
       d = big.data();
     }
     bool own_in = false;
-    int qn = -1;
+    int qn = -1, qo = -1;
     for (int q = 0; q < k; ++q) {
       const int a = atoms[q];
       const double dx = st.x[p] - mol.x[a], dy = st.y[p] - mol.y[a], dz = st.z[p] - mol.z[a];
       d[q] = std::sqrt(dx * dx + dy * dy + dz * dz);
-      own_in |= (a == own);
+      if (a == own) {
+        own_in = true;
+        qo = q;
+      }
       if (qn < 0 || d[q] < d[qn]) qn = q;
     }
     const int an = qn < 0 ? -1 : atoms[qn];
-    double total = 0.0, mine = 0.0;
-    for (int j = 0; j < k; ++j) {
+    // the reference's P_j (zero test against the nearest atom, list order)
+    auto exact = [&](int j) {
       const int aj = atoms[j];
-      double prod = 1.0;
       if (j != qn && cell(d[j], d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0)
-        prod = 0.0;
-      else
-        for (int q = 0; q < k; ++q) {
-          const int ak = atoms[q];
+        return 0.0;
+      double prod = 1.0;
+      for (int q = 0; q < k; ++q) {
+        const int ak = atoms[q];
+        if (ak == aj) continue;
+        prod *= cell(d[j], d[q], rij[(size_t)aj * na + ak], mol.rm[aj], mol.rm[ak]);
+        if (prod == 0.0) break;
+      }
+      return prod;
+    };
+    double total = 0.0, mine = 0.0;
+    if (own_in) {
+      mine = exact(qo);
+      if (mine == 0.0) {
+        st.w[p] = 0.0;
+        continue;
+      }
+      const double thr = std::ldexp(mine, -60);
+      for (int j = 0; j < k; ++j) {
+        if (j == qo) {
+          total += mine;
+          continue;
+        }
+        const int aj = atoms[j];
+        if (j != qn && cell(d[j], d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0)
+          continue;
+        double prod = 1.0;
+        for (int t = 0; t < k; ++t) {
+          const int q = ord[t], ak = atoms[q];
           if (ak == aj) continue;
           prod *= cell(d[j], d[q], rij[(size_t)aj * na + ak], mol.rm[aj], mol.rm[ak]);
-          if (prod == 0.0) break;
+          if (prod < thr) {
+            prod = 0.0;
+            break;
+          }
         }
-      if (aj == own) {
-        mine = prod;
-        if (mine == 0.0) break;
+        total += prod;
       }
-      total += prod;
-    }
-    if (!own_in) {
+    } else {
+      for (int j = 0; j < k; ++j) total += exact(j);
       const double dx = st.x[p] - mol.x[own], dy = st.y[p] - mol.y[own], dz = st.z[p] - mol.z[own];
       const double down = std::sqrt(dx * dx + dy * dy + dz * dz);
       mine = 1.0;
