@@ -3052,8 +3052,8 @@ __device__ inline double becke_cell_w(double dj, double dk, double rjk, double a
   return 0.5 * (1.0 - u);
 }
 
-// Staged points per Becke block: one wave, so that a block's cost follows
-// its own points only.
+// Staged points per Becke block; the block has kBeckeLanes waves, one per
+// class of j, over the same points.
 constexpr int kBeckePts = 64;
 
 // The near-first order of every staged group's nuclei: the nuclei by
@@ -3091,38 +3091,23 @@ __global__ __launch_bounds__(256) void k_becke_order(const StGroup* __restrict__
   }
 }
 
-// One staged point per thread, one block per kBeckePts points of one staged
-// group: weight *= P_own / sum_j P_j over the group's nuclei,
-// P_j = prod_{q != j} s(mu_jq), every factor in [0, 1].  The group is
+// The exact terms of the Becke weights, one staged point per lane: the
+// point's nearest nucleus n (first in list order at the least distance) and
+// own nucleus (list positions, -1 when it is not among the group's), P_own
+// and P_n formed as the reference forms them (a zero test against n unless
+// j = n, then the nuclei in list order; g2g/cpu/weight.cpp:13-112), so a
+// weight is zero exactly when the reference's is.  A point whose own nucleus
+// is not among the group's forms P_own over all of them.  The group is
 // uniform in the block, so its nuclei, their coordinates and the pair tables
-// are wave-uniform loads; distances are recomputed instead of stored per
-// point, with the host's values.  The host's weigh (grid.cpp) does the same
-// arithmetic, so the weights are bitwise its weights:
-// P_own is formed as the reference forms it (a zero test against the point's
-// nearest nucleus n, then the nuclei in list order), so a weight is zero
-// exactly when the reference's is, and a zero P_own ends the point.  Every
-// other P_j runs over the nuclei nearest-first (gord, from the group's first
-// point) and is dropped once its running product, an upper bound of P_j,
-// falls below 2^-60 P_own: the sum holds P_own, so a dropped term lies far
-// below half an ulp of it.  A point whose own nucleus is not among the
-// group's keeps the plain double loop (g2g/cpu/weight.cpp:13-112).  The
-// loops over j are wave-uniform: P_own is formed at the j of each owner a
-// lane of the wave has.
-__global__ __launch_bounds__(kBeckePts) void k_becke(const StGroup* __restrict__ groups,
-                                                     const int2* __restrict__ items,
-                                                     const StMol* __restrict__ mols,
-                                                     const int* __restrict__ gatoms,
-                                                     const int* __restrict__ gord,
-                                                     const int* __restrict__ owner,
-                                                     const double* __restrict__ ax,
-                                                     const double* __restrict__ ay,
-                                                     const double* __restrict__ az,
-                                                     const double* __restrict__ aw,
-                                                     const double* __restrict__ rij,
-                                                     const double* __restrict__ sx,
-                                                     const double* __restrict__ sy,
-                                                     const double* __restrict__ sz,
-                                                     double* __restrict__ sw) {
+// are wave-uniform loads, and P_own and P_n are formed in wave-uniform loops
+// at the j of each owner and nearest nucleus a lane of the wave has.
+__global__ __launch_bounds__(kBeckePts) void k_becke_exact(
+    const StGroup* __restrict__ groups, const int2* __restrict__ items,
+    const StMol* __restrict__ mols, const int* __restrict__ gatoms,
+    const int* __restrict__ owner, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const double* __restrict__ aw, const double* __restrict__ rij,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, double2* __restrict__ bterm, int2* __restrict__ bnuc) {
 #pragma clang fp contract(off)
   const int2 it = items[blockIdx.x];
   const int gi = __builtin_amdgcn_readfirstlane(it.x);
@@ -3131,7 +3116,6 @@ __global__ __launch_bounds__(kBeckePts) void k_becke(const StGroup* __restrict__
   if (p >= G.pt_off + G.npts) return;
   const StMol m = mols[G.mol];
   const int* at = gatoms + G.atom_off;
-  const int* ord = gord + G.atom_off;
   const int k = G.natoms, own = owner[p], na = m.natom;
   const double* A = ax + m.atom_base;
   const double* B = ay + m.atom_base;
@@ -3139,7 +3123,7 @@ __global__ __launch_bounds__(kBeckePts) void k_becke(const StGroup* __restrict__
   const double* W = aw + m.rij_off;
   const double* D = rij + m.rij_off;
   const double x = sx[p], y = sy[p], z = sz[p];
-  int an = -1, qo = -1;
+  int an = -1, qo = -1, qn = -1;
   double dn = 0.0;
   for (int q = 0; q < k; ++q) {
     const int aq = at[q];
@@ -3148,9 +3132,9 @@ __global__ __launch_bounds__(kBeckePts) void k_becke(const StGroup* __restrict__
     if (an < 0 || dq < dn) {
       dn = dq;
       an = aq;
+      qn = q;
     }
   }
-  // the reference's P_j of nucleus j (list position), at distance dj
   auto exact = [&](int aj, double dj) {
     const double* Dj = D + (long long)aj * na;
     const double* Wj = W + (long long)aj * na;
@@ -3164,59 +3148,119 @@ __global__ __launch_bounds__(kBeckePts) void k_becke(const StGroup* __restrict__
     }
     return prod;
   };
-  double total = 0.0, mine = 0.0;
-  if (qo >= 0) {
-    for (int j = 0; j < k; ++j) {  // P_own at the owners of the wave's points
-      if (j != qo) continue;
-      const int aj = at[j];
-      mine = exact(aj, dist3(x, y, z, A[aj], B[aj], C[aj]));
-    }
-    if (mine == 0.0) {
-      sw[p] = 0.0;
-      return;
-    }
-    const double thr = ldexp(mine, -60);
-    for (int j = 0; j < k; ++j) {
-      if (j == qo) {
-        total += mine;
-        continue;
+  double mine = 0.0, pn = 0.0;
+  if (k > 0) {
+    if (qo >= 0) {
+      for (int j = 0; j < k; ++j) {
+        if (j != qo) continue;
+        const int aj = at[j];
+        mine = exact(aj, dist3(x, y, z, A[aj], B[aj], C[aj]));
       }
-      const int aj = at[j];
-      const double dj = dist3(x, y, z, A[aj], B[aj], C[aj]);
-      const double* Dj = D + (long long)aj * na;
-      const double* Wj = W + (long long)aj * na;
-      if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
-      double prod = 1.0;
-      for (int t = 0; t < k; ++t) {
-        const int ak = ord[t];
-        if (ak == aj) continue;
-        prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
-        if (prod < thr) {
-          prod = 0.0;
-          break;
-        }
-      }
-      total += prod;
+    } else {
+      mine = exact(own, dist3(x, y, z, A[own], B[own], C[own]));
     }
-  } else {
-    for (int j = 0; j < k; ++j) {
-      const int aj = at[j];
-      total += exact(aj, dist3(x, y, z, A[aj], B[aj], C[aj]));
-    }
-    const double down = dist3(x, y, z, A[own], B[own], C[own]);
-    const double* Do = D + (long long)own * na;
-    const double* Wo = W + (long long)own * na;
-    mine = 1.0;
-    if (k > 0 && becke_cell_w(down, dn, Do[an], Wo[an]) == 0.0)
-      mine = 0.0;
-    else
-      for (int q = 0; q < k; ++q) {
-        const int ak = at[q];
-        mine *= becke_cell_w(down, dist3(x, y, z, A[ak], B[ak], C[ak]), Do[ak], Wo[ak]);
-        if (mine == 0.0) break;
+    pn = mine;
+    if (mine != 0.0 && qn != qo)
+      for (int j = 0; j < k; ++j) {
+        if (j != qn) continue;
+        pn = exact(an, dn);
       }
   }
-  sw[p] *= (total == 0.0 ? 0.0 : mine / total);
+  bterm[p] = make_double2(mine, pn);
+  bnuc[p] = make_int2(qo, qn);
+}
+
+// Becke weights of kBeckePts staged points of one group: weight *=
+// P_own / sum_j P_j over the group's nuclei, P_j = prod_{q != j} s(mu_jq),
+// every factor in [0, 1], with P_own and P_n from k_becke_exact.  Lane l of
+// every wave takes point l; wave c sums the j of class c (j mod kBeckeLanes)
+// in list order, and the first wave adds the classes' sums in order.  Every
+// P_j other than P_own and P_n runs over the nuclei nearest-first (gord,
+// from the group's first point) and is dropped once its running product, an
+// upper bound of P_j, falls below 2^-60 of the larger exact term of the sum
+// (P_own, P_n): a dropped term lies far below half an ulp of the sum.  A
+// point whose own nucleus is not among the group's leaves P_own out of the
+// sum and the bound; a zero P_own ends the point.  The group is uniform in
+// the block, so the loops over j and the nuclei's loads are wave-uniform;
+// distances are recomputed instead of stored per point, with the host's
+// values.  The host's weigh (grid.cpp) does the same arithmetic, so the
+// weights are bitwise its weights.
+__global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
+    const StGroup* __restrict__ groups, const int2* __restrict__ items,
+    const StMol* __restrict__ mols, const int* __restrict__ gatoms,
+    const int* __restrict__ gord, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const double* __restrict__ aw, const double* __restrict__ rij,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, const double2* __restrict__ bterm,
+    const int2* __restrict__ bnuc, double* __restrict__ sw) {
+#pragma clang fp contract(off)
+  __shared__ double part[kBeckeLanes][kBeckePts];
+  const int2 it = items[blockIdx.x];
+  const int gi = __builtin_amdgcn_readfirstlane(it.x);
+  const int lane = (int)threadIdx.x % kBeckePts;
+  const int cls = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kBeckePts);
+  const int p = __builtin_amdgcn_readfirstlane(it.y) + lane;
+  const StGroup G = groups[gi];
+  const bool live = p < G.pt_off + G.npts;
+  const int k = G.natoms;
+  double mine = 0.0, sum = 0.0;
+  if (live && k > 0) {
+    const StMol m = mols[G.mol];
+    const int* at = gatoms + G.atom_off;
+    const int* ord = gord + G.atom_off;
+    const int na = m.natom;
+    const double* A = ax + m.atom_base;
+    const double* B = ay + m.atom_base;
+    const double* C = az + m.atom_base;
+    const double* W = aw + m.rij_off;
+    const double* D = rij + m.rij_off;
+    const double x = sx[p], y = sy[p], z = sz[p];
+    const double2 tm = bterm[p];
+    const int2 nq = bnuc[p];
+    const int qo = nq.x, qn = nq.y, an = at[qn];
+    const double dn = dist3(x, y, z, A[an], B[an], C[an]);
+    const double pn = tm.y;
+    mine = tm.x;
+    if (mine != 0.0) {
+      const double thr = ldexp(qo >= 0 ? fmax(mine, pn) : pn, -60);
+      for (int j = cls; j < k; j += kBeckeLanes) {
+        if (j == qo) {
+          sum += mine;
+          continue;
+        }
+        if (j == qn) {
+          sum += pn;
+          continue;
+        }
+        const int aj = at[j];
+        const double dj = dist3(x, y, z, A[aj], B[aj], C[aj]);
+        const double* Dj = D + (long long)aj * na;
+        const double* Wj = W + (long long)aj * na;
+        if (becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
+        double prod = 1.0;
+        for (int t = 0; t < k; ++t) {
+          const int ak = ord[t];
+          if (ak == aj) continue;
+          prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
+          if (prod < thr) {
+            prod = 0.0;
+            break;
+          }
+        }
+        sum += prod;
+      }
+    }
+  }
+  part[cls][lane] = sum;
+  __syncthreads();
+  if (cls != 0 || !live) return;
+  double total = 0.0;
+#pragma unroll
+  for (int c = 0; c < kBeckeLanes; ++c) total += part[c][lane];
+  if (k == 0 || mine == 0.0)
+    sw[p] = 0.0;
+  else
+    sw[p] *= (total == 0.0 ? 0.0 : mine / total);
 }
 
 // Points a staged group keeps (all, or those of non-zero weight).
@@ -4590,6 +4634,8 @@ struct StageDev {
   int2* bitems;
   int *owner, *gatoms, *gord, *count;
   double *ax, *ay, *az, *aw, *rij, *x, *y, *z, *w;
+  double2* bterm;  // P_own, P_n of every staged point
+  int2* bnuc;      // own and nearest nucleus (list positions) of every staged point
 };
 
 namespace {
@@ -4628,6 +4674,8 @@ void carve_stage(Carver& c, const StageHost& h, long long n_points, StageDev& d)
   d.y = c.take<double>(n_points + 1);
   d.z = c.take<double>(n_points + 1);
   d.w = c.take<double>(n_points + 1);
+  d.bterm = c.take<double2>(n_points + 1);
+  d.bnuc = c.take<int2>(n_points + 1);
 }
 
 }  // namespace
@@ -4675,10 +4723,14 @@ std::vector<GridPartition> Engine::weigh_and_load(const std::vector<const Molecu
   if (nst > 0)
     hipLaunchKernelGGL(k_becke_order, dim3(nst), dim3(256), 0, s, d.groups, d.mols, d.gatoms,
                        d.ax, d.ay, d.az, d.x, d.y, d.z, d.gord);
-  if (!h.bitems.empty())
-    hipLaunchKernelGGL(k_becke, dim3((unsigned)h.bitems.size()), dim3(kBeckePts), 0, s, d.groups,
-                       d.bitems, d.mols, d.gatoms, d.gord, d.owner, d.ax, d.ay, d.az, d.aw, d.rij,
-                       d.x, d.y, d.z, d.w);
+  if (!h.bitems.empty()) {
+    hipLaunchKernelGGL(k_becke_exact, dim3((unsigned)h.bitems.size()), dim3(kBeckePts), 0, s,
+                       d.groups, d.bitems, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.aw,
+                       d.rij, d.x, d.y, d.z, d.bterm, d.bnuc);
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)h.bitems.size()), dim3(kBeckePts * kBeckeLanes), 0,
+                       s, d.groups, d.bitems, d.mols, d.gatoms, d.gord, d.ax, d.ay, d.az, d.aw,
+                       d.rij, d.x, d.y, d.z, d.bterm, d.bnuc, d.w);
+  }
   const int drop = opt.remove_zero_weights ? 1 : 0;
   if (nst > 0)
     hipLaunchKernelGGL(k_group_count, dim3(nst), dim3(256), 0, s, d.groups, d.w, drop, d.count);

@@ -120,14 +120,18 @@ inline double cell(double dj, double dk, double rjk, double rmj, double rmk) {
 // w *= P_own / sum_j P_j, P_j = prod_{q != j} s(mu_jq), every factor in [0, 1].
 // P_own is formed exactly as the reference forms it (a zero test against the
 // point's nearest atom n, then the atoms in list order), so a weight is zero
-// exactly when the reference's is, and a zero P_own ends the point.  Every
-// other P_j runs over the group's atoms nearest-first (becke_order: the same
-// order for all points of the group) and is dropped once its running product
-// -- an upper bound of P_j -- falls below 2^-60 P_own: the sum holds P_own, so
-// a dropped term is far below half an ulp of it.  The sum differs from the
-// reference's by a few ulps at most (the kept P_j are products in another
-// order).  A point whose own atom is not among the group's keeps the plain
-// loop.
+// exactly when the reference's is, and a zero P_own ends the point; so is P_n
+// (no zero test), a term of the sum and in general its largest.  Every other
+// P_j runs over the group's atoms nearest-first (becke_order: the same order
+// for all points of the group) and is dropped once its running product -- an
+// upper bound of P_j -- falls below 2^-60 of the larger of P_own and P_n:
+// the sum holds that term, so a dropped one lies far below half an ulp of it.  The sum runs over
+// kBeckeLanes interleaved classes of j (j mod kBeckeLanes, each in list
+// order from 0.0), which are then added in class order: the device gives each
+// class to its own wave.  The sum differs from the reference's by a few ulps
+// at most.  A point whose own atom is not among the group's does the same
+// with P_own formed over all of the group's atoms, left out of the sum and
+// out of the bound.
 void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& rij,
            const std::vector<int>& atoms, Staging& st) {
   const int n = (int)st.x.size(), k = (int)atoms.size(), na = mol.natom;
@@ -143,49 +147,62 @@ void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& r
       big.resize(k + 1);
       d = big.data();
     }
-    bool own_in = false;
     int qn = -1, qo = -1;
     for (int q = 0; q < k; ++q) {
       const int a = atoms[q];
       const double dx = st.x[p] - mol.x[a], dy = st.y[p] - mol.y[a], dz = st.z[p] - mol.z[a];
       d[q] = std::sqrt(dx * dx + dy * dy + dz * dz);
-      if (a == own) {
-        own_in = true;
-        qo = q;
-      }
+      if (a == own) qo = q;
       if (qn < 0 || d[q] < d[qn]) qn = q;
     }
-    const int an = qn < 0 ? -1 : atoms[qn];
-    // the reference's P_j (zero test against the nearest atom, list order)
-    auto exact = [&](int j) {
-      const int aj = atoms[j];
-      if (j != qn && cell(d[j], d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0)
+    if (k == 0) {
+      st.w[p] = 0.0;  // no atoms: the sum is empty
+      continue;
+    }
+    const int an = atoms[qn];
+    // the reference's P_j of the atom at distance dj (zero test against the
+    // nearest atom unless j is the nearest, then the atoms in list order)
+    auto exact = [&](int aj, double dj) {
+      if (aj != an && cell(dj, d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0)
         return 0.0;
       double prod = 1.0;
       for (int q = 0; q < k; ++q) {
         const int ak = atoms[q];
         if (ak == aj) continue;
-        prod *= cell(d[j], d[q], rij[(size_t)aj * na + ak], mol.rm[aj], mol.rm[ak]);
+        prod *= cell(dj, d[q], rij[(size_t)aj * na + ak], mol.rm[aj], mol.rm[ak]);
         if (prod == 0.0) break;
       }
       return prod;
     };
-    double total = 0.0, mine = 0.0;
-    if (own_in) {
-      mine = exact(qo);
-      if (mine == 0.0) {
-        st.w[p] = 0.0;
-        continue;
-      }
-      const double thr = std::ldexp(mine, -60);
-      for (int j = 0; j < k; ++j) {
+    double down = 0.0;
+    if (qo >= 0) {
+      down = d[qo];
+    } else {
+      const double dx = st.x[p] - mol.x[own], dy = st.y[p] - mol.y[own], dz = st.z[p] - mol.z[own];
+      down = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    const double mine = exact(own, down);
+    if (mine == 0.0) {
+      st.w[p] = 0.0;
+      continue;
+    }
+    const double pn = qn == qo ? mine : exact(an, d[qn]);
+    // a bound below the sum: the largest of its exact terms
+    const double thr = std::ldexp(qo >= 0 ? std::max(mine, pn) : pn, -60);
+    double total = 0.0;
+    for (int c = 0; c < kBeckeLanes; ++c) {
+      double sum = 0.0;
+      for (int j = c; j < k; j += kBeckeLanes) {
         if (j == qo) {
-          total += mine;
+          sum += mine;
+          continue;
+        }
+        if (j == qn) {
+          sum += pn;
           continue;
         }
         const int aj = atoms[j];
-        if (j != qn && cell(d[j], d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0)
-          continue;
+        if (cell(d[j], d[qn], rij[(size_t)aj * na + an], mol.rm[aj], mol.rm[an]) == 0.0) continue;
         double prod = 1.0;
         for (int t = 0; t < k; ++t) {
           const int q = ord[t], ak = atoms[q];
@@ -196,21 +213,9 @@ void weigh(const Molecule& mol, const Options& opt, const std::vector<double>& r
             break;
           }
         }
-        total += prod;
+        sum += prod;
       }
-    } else {
-      for (int j = 0; j < k; ++j) total += exact(j);
-      const double dx = st.x[p] - mol.x[own], dy = st.y[p] - mol.y[own], dz = st.z[p] - mol.z[own];
-      const double down = std::sqrt(dx * dx + dy * dy + dz * dz);
-      mine = 1.0;
-      if (k > 0 && cell(down, d[qn], rij[(size_t)own * na + an], mol.rm[own], mol.rm[an]) == 0.0)
-        mine = 0.0;
-      else
-        for (int q = 0; q < k; ++q) {
-          const int ak = atoms[q];
-          mine *= cell(down, d[q], rij[(size_t)own * na + ak], mol.rm[own], mol.rm[ak]);
-          if (mine == 0.0) break;
-        }
+      total += sum;
     }
     st.w[p] *= (total == 0.0 ? 0.0 : mine / total);
   }

@@ -15,6 +15,10 @@
 namespace xcq {
 
 constexpr int kMaxContractions = 13;
+// Classes of j (j mod kBeckeLanes) the Becke sum sum_j P_j is split into:
+// each class is summed in list order, then the classes in order, on the host
+// and on the device (one wave per class) alike.
+constexpr int kBeckeLanes = 8;
 
 // g2g_set_options_ knobs and their defaults (g2g/init.cpp:394-406).
 struct Options {
