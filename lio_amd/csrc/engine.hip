@@ -3445,14 +3445,19 @@ __global__ __launch_bounds__(256) void k_gen_keys(const GenAtom* __restrict__ at
   vals[r] = (unsigned)r;
 }
 
-// Points per group key (integer counts: the order of the additions does not
-// matter).
-__global__ __launch_bounds__(256) void k_gen_count(const unsigned* __restrict__ keys, long long n,
-                                                   unsigned no_group, int* __restrict__ count) {
+// The run of every group key in the sorted keys: bounds[k] its first
+// position, bounds[nkeys + k] one past its last (both zero for a key with no
+// points), so its point count is their difference.  A thread per sorted
+// key writes the bounds where the key differs from a neighbour: no atomics.
+__global__ __launch_bounds__(256) void k_gen_bounds(const unsigned* __restrict__ keys,
+                                                    long long n, unsigned no_group,
+                                                    int* __restrict__ bounds) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const unsigned k = keys[i];
-  if (k < no_group) atomicAdd(&count[k], 1);
+  if (k >= no_group) return;
+  if (i == 0 || keys[i - 1] != k) bounds[k] = (int)i;
+  if (i == n - 1 || keys[i + 1] != k) bounds[no_group + k] = (int)(i + 1);
 }
 
 // Staged point arrays in sorted (group, generation) order: coordinates,
@@ -4961,7 +4966,7 @@ std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecu
     g.sh_len = c.take<int>(sh_len.size() + 1);
     g.sh_atom = c.take<int>(sh_atom.size() + 1);
     g.bin_mol = c.take<int>(nbins);
-    g.count = c.take<int>(nbins);
+    g.count = c.take<int>(2 * nbins);
     g.err = c.take<int>(1);
     g.slots = c.take<int>(nslot + 1);
     g.aslots = c.take<int>(naslot + 1);
@@ -4999,8 +5004,9 @@ std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecu
   up(g.sh_len, sh_len.data(), sh_len.size() * sizeof(int));
   up(g.sh_atom, sh_atom.data(), sh_atom.size() * sizeof(int));
   up(g.bin_mol, bin_mol.data(), bin_mol.size() * sizeof(int));
-  // counts and the error flag start at zero (a fill kernel, no memset)
-  hipLaunchKernelGGL(k_fill_int, dim3((nbins + 255) / 256), dim3(256), 0, s, g.count, nbins, 0);
+  // run bounds and the error flag start at zero (a fill kernel, no memset)
+  hipLaunchKernelGGL(k_fill_int, dim3((2 * nbins + 255) / 256), dim3(256), 0, s, g.count,
+                     2 * nbins, 0);
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s, g.err, 1, 0);
   const int natoms = (int)gatom.size();
   if (nraw > 0)
@@ -5011,21 +5017,23 @@ std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecu
   XCQ_HIP((hipError_t)radix_sort_pairs(g.sort, &sort_bytes, g.keys, g.skeys, g.vals, g.svals,
                                        (int)nraw, bits, stream_));
   if (nraw > 0)
-    hipLaunchKernelGGL(k_gen_count, dim3((unsigned)((nraw + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_gen_bounds, dim3((unsigned)((nraw + 255) / 256)), dim3(256), 0, s,
                        g.skeys, nraw, (unsigned)nbins, g.count);
   up(d.rij, h.rij.data(), h.rij.size() * sizeof(double));
   hipLaunchKernelGGL(k_gen_screen, dim3(nbins), dim3(256), 0, s, g.mols, g.bin_mol, g.atoms,
                      g.srad, d.rij, g.sh_first, g.sh_len, g.sh_atom, g.sh_a, g.sh_thr,
                      opt.assign_all_functions ? 1 : 0, g.slots, g.aslots, g.info);
   XCQ_HIP(hipGetLastError());
-  std::vector<int> count(nbins);
+  std::vector<int> count(2 * nbins);
   std::vector<BinInfo> info(nbins);
   int err = 0;
-  XCQ_HIP(hipMemcpyAsync(count.data(), g.count, nbins * sizeof(int), hipMemcpyDeviceToHost, s));
+  XCQ_HIP(hipMemcpyAsync(count.data(), g.count, 2 * nbins * sizeof(int), hipMemcpyDeviceToHost,
+                         s));
   XCQ_HIP(hipMemcpyAsync(info.data(), g.info, nbins * sizeof(BinInfo), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipMemcpyAsync(&err, g.err, sizeof(int), hipMemcpyDeviceToHost, s));
   XCQ_HIP(hipStreamSynchronize(s));
   if (err) throw std::runtime_error("grid point outside the cube lattice of the bounding prism");
+  for (int b = 0; b < nbins; ++b) count[b] = count[nbins + b] - count[b];
   // the staged groups, as the host keeps them: cubes with significant
   // functions and enough points, in lattice order, then the spheres
   std::vector<long long> start(nbins + 1, 0);
