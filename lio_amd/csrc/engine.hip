@@ -667,8 +667,10 @@ __device__ inline void load8(const float* p, float (&x)[8]) {
 // functions of one group: each wave evaluates one function at a time for its
 // 64 points (the function record is wave-uniform), the values are transposed
 // through LDS, and each thread stores 4 consecutive functions of one point
-// per plane in one 16-byte store.  The tile's phi values also go to the
-// function-major phi plane the Fock kernel reads (rows of 64 points).
+// per plane in one 16-byte store (NT = 512: each half of the block stores
+// every other plane).  The tile's phi values also go to the function-major
+// phi plane the Fock kernel reads (rows of 64 points).  Items of group -1 are
+// padding of the XCD-interleaved item list.
 constexpr int kPmPts = 64, kPmFns = 16;
 // LO: lo16 (point-major, the planes' layout, group blocks at phi_off) gets
 // phi's remainder phi - (double)(float)phi rounded to BF16 (nearest even).
@@ -678,8 +680,8 @@ __device__ inline unsigned short float_to_bf16(float f) {
   return (unsigned short)(u >> 16);
 }
 
-template <int NV, bool LO = false>
-__global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ items,
+template <int NV, bool LO = false, int NT = 256>
+__global__ __launch_bounds__(NT) void k_eval_pm(const EvalItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
                                                  const int* __restrict__ fnl,
                                                  const DevFn* __restrict__ bas,
@@ -692,12 +694,13 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
   constexpr int NP = NV + (LO ? 1 : 0);
   __shared__ float sT[NP][kPmFns][kPmPts + 1];
   const EvalItem it = items[blockIdx.x];
+  if (it.group < 0) return;
   const DevGroup g = groups[it.group];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = it.p0 + lane;
   const int q = g.pt_off + (p < g.npts_pad ? p : it.p0);
   const double x = px[q], y = py[q], z = pz[q];
-  for (int kk = wave; kk < kPmFns; kk += 4) {
+  for (int kk = wave; kk < kPmFns; kk += NT / 64) {
     const int f = fnl[g.fn_off + it.k + kk];
     double v[NV];
     if (f >= 0) {
@@ -709,20 +712,21 @@ __global__ __launch_bounds__(256) void k_eval_pm(const EvalItem* __restrict__ it
     if (LO) sT[NP - 1][kk][lane] = (float)(v[0] - (double)(float)v[0]);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < kPmFns * kPmPts; e += 256) {
+  for (int e = threadIdx.x; e < kPmFns * kPmPts; e += NT) {
     const int kk = e / kPmPts, pl = e % kPmPts;
     if (it.p0 + pl < g.npts_pad)
       phiT[g.phi_off + (long long)(it.k + kk) * g.npts_pad + it.p0 + pl] = sT[0][kk][pl];
   }
-  const int wp = threadIdx.x >> 2, wf = 4 * (threadIdx.x & 3);
+  const int t = threadIdx.x & 255, half = threadIdx.x >> 8;
+  const int wp = t >> 2, wf = 4 * (t & 3);
   const int pp = it.p0 + wp;
   if (pp >= g.npts_pad) return;
   const long long plane = (long long)g.m_pad * g.npts_pad;
   float* out = vals + g.val_off + (long long)pp * g.m_pad + it.k + wf;
-  for (int c = 0; c < NV; ++c)
+  for (int c = half; c < NV; c += NT / 256)
     *reinterpret_cast<fl4*>(out + c * plane) =
         fl4{sT[c][wf][wp], sT[c][wf + 1][wp], sT[c][wf + 2][wp], sT[c][wf + 3][wp]};
-  if (LO) {
+  if (LO && half == NT / 256 - 1) {
     unsigned short* o16 = lo16 + g.phi_off + (long long)pp * g.m_pad + it.k + wf;
     const unsigned a = float_to_bf16(sT[NP - 1][wf][wp]) |
                        ((unsigned)float_to_bf16(sT[NP - 1][wf + 1][wp]) << 16);
@@ -4314,6 +4318,18 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   dg->neval = (int)eitems.size();
   dg->ngather = (int)gitems.size();
+  if (!pmitems.empty()) {
+    // XCD-interleaved: block b runs on XCD b % 8 and takes item b / 8 of the
+    // list's b % 8-th eighth, so the function chunks of one point tile, whose
+    // 64-byte row pieces share cache lines, are written through one L2
+    const size_t n = pmitems.size(), L = (n + kXcds - 1) / kXcds;
+    std::vector<EvalItem> xi(kXcds * L, EvalItem{-1, 0, 0, 0});
+    for (size_t b = 0; b < xi.size(); ++b) {
+      const size_t i = (b % kXcds) * L + b / kXcds;
+      if (i < n) xi[b] = pmitems[i];
+    }
+    pmitems.swap(xi);
+  }
   dg->npm = (int)pmitems.size();
   dg->nrad = (int)ritems.size();
   dg->ndensity = (int)ditems.size();
@@ -4587,14 +4603,16 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       float* v = static_cast<float*>(dg->vals);
       if (dg->npm > 0) {
         if (dg->lo && dg->gga)
-          hipLaunchKernelGGL((k_eval_pm<10, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
-                             dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
+          hipLaunchKernelGGL((k_eval_pm<10, true, 512>), dim3(dg->npm), dim3(512), 0, s,
+                             dg->pmitems, dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v,
+                             dg->phiT, dg->lo16);
         else if (dg->lo)
           hipLaunchKernelGGL((k_eval_pm<4, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
                              dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
         else if (dg->gga && !dg->radial)
-          hipLaunchKernelGGL(k_eval_pm<10>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
-                             dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
+          hipLaunchKernelGGL((k_eval_pm<10, false, 512>), dim3(dg->npm), dim3(512), 0, s,
+                             dg->pmitems, dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v,
+                             dg->phiT, dg->lo16);
         else
           hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
