@@ -4187,19 +4187,46 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   std::vector<EvalItem> eitems;
   std::vector<GatherItem> gitems, ritems;
   std::vector<EvalItem> pmitems;
+  size_t pm_len = 0;
   // split-BF16 variant 3: groups of more than 64 functions run the row-split
   // kernel on 16-point blocks (listed first), the rest 64-point blocks
   const bool rowsplit = bf && (tuning_.density_variant == 3 || tuning_.density_variant == 4);
   std::vector<DensityItem> ditems, dsmall, citems;
+  // the function-major evaluation runs unless the point-major one does
+  const bool fm_eval = !bf || dg->inkernel, pm_eval = bf && !dg->inkernel;
+  {  // the lists' lengths first: millions of items for large molecules
+    size_t ne = 0, ng = 0, npm = 0, nt = 0;
+    for (int g = 0; g < G; ++g) {
+      const DevGroup& d = groups[g];
+      const size_t tiles = (d.npts_pad + kDensPts - 1) / kDensPts;
+      nt += tiles;
+      if (fm_eval) ne += (size_t)d.m_pad * ((d.npts_pad + kEvalSpan - 1) / kEvalSpan);
+      ng += ((size_t)d.m_pad * d.m_pad + 255) / 256;
+      if (pm_eval)
+        npm += (size_t)((d.npts_pad + kPmPts - 1) / kPmPts) * ((d.m_pad + kPmFns - 1) / kPmFns);
+    }
+    citems.reserve(nt);
+    dsmall.reserve(nt);
+    eitems.reserve(ne);
+    gitems.reserve(ng);
+    // the point-major items XCD-interleaved: block b runs on XCD b % 8 and
+    // takes item b / 8 of the list's (b % 8)-th eighth, so the function
+    // chunks of one point tile, whose 64-byte row pieces share cache lines,
+    // are written through one L2 (items of group -1 pad the eighths)
+    pm_len = (npm + kXcds - 1) / kXcds;
+    pmitems.assign(kXcds * pm_len, EvalItem{-1, 0, 0, 0});
+  }
+  size_t pm_next = 0;
   for (int g : order) {
     const DevGroup& d = groups[g];
     for (int p0 = 0; p0 < d.npts_pad; p0 += kDensPts) citems.push_back(DensityItem{g, p0});
-    for (int k = 0; k < d.m_pad; ++k)
+    for (int k = 0; fm_eval && k < d.m_pad; ++k)
       for (int p0 = 0; p0 < d.npts_pad; p0 += kEvalSpan) eitems.push_back(EvalItem{g, k, p0, 0});
     for (int e0 = 0; e0 < d.m_pad * d.m_pad; e0 += 256) gitems.push_back(GatherItem{g, e0});
-    if (bf && !dg->inkernel)
+    if (pm_eval)
       for (int p0 = 0; p0 < d.npts_pad; p0 += kPmPts)
-        for (int k0 = 0; k0 < d.m_pad; k0 += kPmFns) pmitems.push_back(EvalItem{g, k0, p0, 0});
+        for (int k0 = 0; k0 < d.m_pad; k0 += kPmFns, ++pm_next)
+          pmitems[kXcds * (pm_next % pm_len) + pm_next / pm_len] = EvalItem{g, k0, p0, 0};
     if (dg->radial)
       for (int e0 = 0; e0 < d.nsh_pad * d.npts_pad; e0 += kEvalSpan)
         ritems.push_back(GatherItem{g, e0});
@@ -4318,18 +4345,6 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   }
   dg->neval = (int)eitems.size();
   dg->ngather = (int)gitems.size();
-  if (!pmitems.empty()) {
-    // XCD-interleaved: block b runs on XCD b % 8 and takes item b / 8 of the
-    // list's b % 8-th eighth, so the function chunks of one point tile, whose
-    // 64-byte row pieces share cache lines, are written through one L2
-    const size_t n = pmitems.size(), L = (n + kXcds - 1) / kXcds;
-    std::vector<EvalItem> xi(kXcds * L, EvalItem{-1, 0, 0, 0});
-    for (size_t b = 0; b < xi.size(); ++b) {
-      const size_t i = (b % kXcds) * L + b / kXcds;
-      if (i < n) xi[b] = pmitems[i];
-    }
-    pmitems.swap(xi);
-  }
   dg->npm = (int)pmitems.size();
   dg->nrad = (int)ritems.size();
   dg->ndensity = (int)ditems.size();
@@ -4578,7 +4593,7 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
   hipEvent_t* ev = reinterpret_cast<hipEvent_t*>(events_);
   const double t_eval = wall();
   XCQ_HIP(hipEventRecord(ev[6], s));
-  if (dg->neval > 0) {
+  if (dg->neval > 0 || dg->npm > 0) {
     const dim3 grid(dg->neval), block(kEvalPts);
     if (prec == Precision::FP64) {
       double* v = static_cast<double*>(dg->vals);
