@@ -58,6 +58,20 @@ def test_host_partition_matches_reference_groups(name):
     assert same_partition(g2g.host_partition(s), gold["groups"])
 
 
+@pytest.mark.parametrize("name", ["c0_pbe", "mol25_s103_pbe", "c1_pbe", "mol195_s107_pbe"])
+def test_host_becke_weights_within_ulps_of_reference(name):
+    """The Becke sums leave LIO's order (the other P_j nearest-first with an
+    early exit below 2^-60 of the largest exact term, the sum in interleaved
+    classes of j); the weights must stay within a few ulps of LIO's: every
+    group's weight sum within 1e-14 relative of the golden inventory's (it
+    is 2-4e-16 at most on these systems), and the zero weights LIO's (the
+    same point counts)."""
+    gold = load(name)
+    mine = g2g.host_partition(system_for(name))
+    assert same_partition(mine, gold["groups"], wtol=1e-14)
+    assert sum(g["npoints"] for g in mine) == sum(g["npoints"] for g in gold["groups"])
+
+
 @pytest.mark.parametrize("opts", [dict(sphere_radius=0.0), dict(little_cube_size=5.0),
                                   dict(assign_all_functions=1), dict(remove_zero_weights=0),
                                   dict(min_points_per_cube=40, max_function_exponent=7)])
