@@ -3057,8 +3057,10 @@ __device__ inline double becke_cell_w(double dj, double dk, double rjk, double a
 }
 
 // Staged points per Becke block; the block has kBeckeLanes waves, one per
-// class of j, over the same points.
-constexpr int kBeckePts = 64;
+// class of j, over the same points.  The distances of the block's points to
+// the first kBeckeNear nuclei of the near-first order are formed once per
+// block, in LDS.
+constexpr int kBeckePts = 64, kBeckeNear = 64;
 
 // The near-first order of every staged group's nuclei: the nuclei by
 // distance from the group's first staged point, ties in list order (the
@@ -3182,7 +3184,9 @@ __global__ __launch_bounds__(kBeckePts) void k_becke_exact(
 // P_j other than P_own and P_n runs over the nuclei nearest-first (gord,
 // from the group's first point) and is dropped once its running product, an
 // upper bound of P_j, falls below 2^-60 of the larger exact term of the sum
-// (P_own, P_n): a dropped term lies far below half an ulp of the sum.  A
+// (P_own, P_n): a dropped term lies far below half an ulp of the sum.  The
+// distances to the first kBeckeNear nuclei of that order come from LDS
+// (formed once per block by the same function), the rest are recomputed.  A
 // point whose own nucleus is not among the group's leaves P_own out of the
 // sum and the bound; a zero P_own ends the point.  The group is uniform in
 // the block, so the loops over j and the nuclei's loads are wave-uniform;
@@ -3199,6 +3203,7 @@ __global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
     const int2* __restrict__ bnuc, double* __restrict__ sw) {
 #pragma clang fp contract(off)
   __shared__ double part[kBeckeLanes][kBeckePts];
+  __shared__ double dnear[kBeckeNear][kBeckePts];
   const int2 it = items[blockIdx.x];
   const int gi = __builtin_amdgcn_readfirstlane(it.x);
   const int lane = (int)threadIdx.x % kBeckePts;
@@ -3206,19 +3211,29 @@ __global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
   const int p = __builtin_amdgcn_readfirstlane(it.y) + lane;
   const StGroup G = groups[gi];
   const bool live = p < G.pt_off + G.npts;
-  const int k = G.natoms;
+  const int k = G.natoms, kn = min(k, kBeckeNear);
+  const StMol m = mols[G.mol];
+  const int* at = gatoms + G.atom_off;
+  const int* ord = gord + G.atom_off;
+  const double* A = ax + m.atom_base;
+  const double* B = ay + m.atom_base;
+  const double* C = az + m.atom_base;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (live) {
+    x = sx[p];
+    y = sy[p];
+    z = sz[p];
+    for (int t = cls; t < kn; t += kBeckeLanes) {
+      const int ak = ord[t];
+      dnear[t][lane] = dist3(x, y, z, A[ak], B[ak], C[ak]);
+    }
+  }
+  __syncthreads();
   double mine = 0.0, sum = 0.0;
   if (live && k > 0) {
-    const StMol m = mols[G.mol];
-    const int* at = gatoms + G.atom_off;
-    const int* ord = gord + G.atom_off;
     const int na = m.natom;
-    const double* A = ax + m.atom_base;
-    const double* B = ay + m.atom_base;
-    const double* C = az + m.atom_base;
     const double* W = aw + m.rij_off;
     const double* D = rij + m.rij_off;
-    const double x = sx[p], y = sy[p], z = sz[p];
     const double2 tm = bterm[p];
     const int2 nq = bnuc[p];
     const int qo = nq.x, qn = nq.y, an = at[qn];
@@ -3245,7 +3260,8 @@ __global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
         for (int t = 0; t < k; ++t) {
           const int ak = ord[t];
           if (ak == aj) continue;
-          prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
+          const double dk = t < kBeckeNear ? dnear[t][lane] : dist3(x, y, z, A[ak], B[ak], C[ak]);
+          prod *= becke_cell_w(dj, dk, Dj[ak], Wj[ak]);
           if (prod < thr) {
             prod = 0.0;
             break;
