@@ -3104,9 +3104,8 @@ __global__ __launch_bounds__(256) void k_becke_order(const StGroup* __restrict__
 // j = n, then the nuclei in list order; g2g/cpu/weight.cpp:13-112), so a
 // weight is zero exactly when the reference's is.  A point whose own nucleus
 // is not among the group's forms P_own over all of them.  The group is
-// uniform in the block, so its nuclei, their coordinates and the pair tables
-// are wave-uniform loads, and P_own and P_n are formed in wave-uniform loops
-// at the j of each owner and nearest nucleus a lane of the wave has.
+// uniform in the block, so its nuclei and their coordinates are wave-uniform
+// loads; the two products run in one pass over the nuclei.
 __global__ __launch_bounds__(kBeckePts) void k_becke_exact(
     const StGroup* __restrict__ groups, const int2* __restrict__ items,
     const StMol* __restrict__ mols, const int* __restrict__ gatoms,
@@ -3141,36 +3140,37 @@ __global__ __launch_bounds__(kBeckePts) void k_becke_exact(
       qn = q;
     }
   }
-  auto exact = [&](int aj, double dj) {
-    const double* Dj = D + (long long)aj * na;
-    const double* Wj = W + (long long)aj * na;
-    if (aj != an && becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) return 0.0;
-    double prod = 1.0;
-    for (int q = 0; q < k; ++q) {
-      const int ak = at[q];
-      if (ak == aj) continue;
-      prod *= becke_cell_w(dj, dist3(x, y, z, A[ak], B[ak], C[ak]), Dj[ak], Wj[ak]);
-      if (prod == 0.0) break;
-    }
-    return prod;
-  };
+  // P_own and P_n in one pass over the nuclei (each distance formed once),
+  // each with the reference's factor order and zero exit
   double mine = 0.0, pn = 0.0;
   if (k > 0) {
-    if (qo >= 0) {
-      for (int j = 0; j < k; ++j) {
-        if (j != qo) continue;
-        const int aj = at[j];
-        mine = exact(aj, dist3(x, y, z, A[aj], B[aj], C[aj]));
-      }
-    } else {
-      mine = exact(own, dist3(x, y, z, A[own], B[own], C[own]));
+    const double down = dist3(x, y, z, A[own], B[own], C[own]);
+    const double* Do = D + (long long)own * na;
+    const double* Wo = W + (long long)own * na;
+    const double* Dn = D + (long long)an * na;
+    const double* Wn = W + (long long)an * na;
+    double po = 1.0, pq = 1.0;
+    bool so = true, sn = qn != qo;  // products still running (P_n is not
+                                    // needed once P_own is zero)
+    if (own != an && becke_cell_w(down, dn, Do[an], Wo[an]) == 0.0) {
+      po = 0.0;
+      so = sn = false;
     }
-    pn = mine;
-    if (mine != 0.0 && qn != qo)
-      for (int j = 0; j < k; ++j) {
-        if (j != qn) continue;
-        pn = exact(an, dn);
+    for (int q = 0; q < k && (so || sn); ++q) {
+      const int ak = at[q];
+      const double dq = dist3(x, y, z, A[ak], B[ak], C[ak]);
+      if (so && ak != own) {
+        po *= becke_cell_w(down, dq, Do[ak], Wo[ak]);
+        so = po != 0.0;
+        sn = sn && so;
       }
+      if (sn && ak != an) {
+        pq *= becke_cell_w(dn, dq, Dn[ak], Wn[ak]);
+        sn = pq != 0.0;
+      }
+    }
+    mine = po;
+    pn = (mine == 0.0 || qn == qo) ? mine : pq;
   }
   bterm[p] = make_double2(mine, pn);
   bnuc[p] = make_int2(qo, qn);
