@@ -663,15 +663,19 @@ __device__ inline void load8(const float* p, float (&x)[8]) {
 }
 
 // Point-major FP32 basis planes [value][point][function] (functions
-// contiguous) for the split-BF16 path.  A block covers 64 points and 16
-// functions of one group: each wave evaluates one function at a time for its
-// 64 points (the function record is wave-uniform), the values are transposed
-// through LDS, and each thread stores 4 consecutive functions of one point
-// per plane in one 16-byte store (NT = 512: each half of the block stores
-// every other plane).  The tile's phi values also go to the function-major
-// phi plane the Fock kernel reads (rows of 64 points).  Items of group -1 are
-// padding of the XCD-interleaved item list.
-constexpr int kPmPts = 64, kPmFns = 16;
+// contiguous) for the split-BF16 path.  A block of eight waves covers 64
+// points and 32 functions of one group: each wave evaluates four functions
+// (one at a time, the function record wave-uniform) for its 64 points and
+// keeps their values in registers; plane by plane the values are transposed
+// through a small LDS tile (two buffers, one barrier a plane) and each
+// thread stores 4 consecutive functions of one point in one 16-byte store,
+// so a point's row piece of a plane is a whole 128-byte line (16-function
+// pieces, half lines, wrote at 3.1 TB/s against the function-major FP64
+// planes' 5.7).  The tile's phi values also go to the function-major phi
+// plane the Fock kernel reads (rows of 64 points).  Items of group -1 are
+// padding of the XCD-interleaved item list.  The split-BF16 function padding
+// is a multiple of 32, so a block never passes its group's functions.
+constexpr int kPmPts = 64, kPmFns = 32;
 // LO: lo16 (point-major, the planes' layout, group blocks at phi_off) gets
 // phi's remainder phi - (double)(float)phi rounded to BF16 (nearest even).
 __device__ inline unsigned short float_to_bf16(float f) {
@@ -680,8 +684,8 @@ __device__ inline unsigned short float_to_bf16(float f) {
   return (unsigned short)(u >> 16);
 }
 
-template <int NV, bool LO = false, int NT = 256>
-__global__ __launch_bounds__(NT) void k_eval_pm(const EvalItem* __restrict__ items,
+template <int NV, bool LO = false>
+__global__ __launch_bounds__(512) void k_eval_pm(const EvalItem* __restrict__ items,
                                                  const DevGroup* __restrict__ groups,
                                                  const int* __restrict__ fnl,
                                                  const DevFn* __restrict__ bas,
@@ -691,8 +695,8 @@ __global__ __launch_bounds__(NT) void k_eval_pm(const EvalItem* __restrict__ ite
                                                  float* __restrict__ vals,
                                                  float* __restrict__ phiT,
                                                  unsigned short* __restrict__ lo16) {
-  constexpr int NP = NV + (LO ? 1 : 0);
-  __shared__ float sT[NP][kPmFns][kPmPts + 1];
+  constexpr int NP = NV + (LO ? 1 : 0), FW = kPmFns / 8;  // functions per wave
+  __shared__ float sT[2][kPmFns][kPmPts + 1];
   const EvalItem it = items[blockIdx.x];
   if (it.group < 0) return;
   const DevGroup g = groups[it.group];
@@ -700,39 +704,46 @@ __global__ __launch_bounds__(NT) void k_eval_pm(const EvalItem* __restrict__ ite
   const int p = it.p0 + lane;
   const int q = g.pt_off + (p < g.npts_pad ? p : it.p0);
   const double x = px[q], y = py[q], z = pz[q];
-  for (int kk = wave; kk < kPmFns; kk += NT / 64) {
+  float o[FW][NP];
+#pragma unroll
+  for (int j = 0; j < FW; ++j) {
+    const int kk = wave + 8 * j;
     const int f = fnl[g.fn_off + it.k + kk];
     double v[NV];
     if (f >= 0) {
       eval_basis<NV>(bas[f], x, y, z, v);
     } else {
+#pragma unroll
       for (int c = 0; c < NV; ++c) v[c] = 0.0;
     }
-    for (int c = 0; c < NV; ++c) sT[c][kk][lane] = (float)v[c];
-    if (LO) sT[NP - 1][kk][lane] = (float)(v[0] - (double)(float)v[0]);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) o[j][c] = (float)v[c];
+    if (LO) o[j][NP - 1] = (float)(v[0] - (double)(float)v[0]);
+    if (p < g.npts_pad) phiT[g.phi_off + (long long)(it.k + kk) * g.npts_pad + p] = o[j][0];
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kPmFns * kPmPts; e += NT) {
-    const int kk = e / kPmPts, pl = e % kPmPts;
-    if (it.p0 + pl < g.npts_pad)
-      phiT[g.phi_off + (long long)(it.k + kk) * g.npts_pad + it.p0 + pl] = sT[0][kk][pl];
-  }
-  const int t = threadIdx.x & 255, half = threadIdx.x >> 8;
-  const int wp = t >> 2, wf = 4 * (t & 3);
+  const int wp = threadIdx.x >> 3, wf = 4 * (threadIdx.x & 7);
   const int pp = it.p0 + wp;
-  if (pp >= g.npts_pad) return;
   const long long plane = (long long)g.m_pad * g.npts_pad;
-  float* out = vals + g.val_off + (long long)pp * g.m_pad + it.k + wf;
-  for (int c = half; c < NV; c += NT / 256)
-    *reinterpret_cast<fl4*>(out + c * plane) =
-        fl4{sT[c][wf][wp], sT[c][wf + 1][wp], sT[c][wf + 2][wp], sT[c][wf + 3][wp]};
-  if (LO && half == NT / 256 - 1) {
-    unsigned short* o16 = lo16 + g.phi_off + (long long)pp * g.m_pad + it.k + wf;
-    const unsigned a = float_to_bf16(sT[NP - 1][wf][wp]) |
-                       ((unsigned)float_to_bf16(sT[NP - 1][wf + 1][wp]) << 16);
-    const unsigned b = float_to_bf16(sT[NP - 1][wf + 2][wp]) |
-                       ((unsigned)float_to_bf16(sT[NP - 1][wf + 3][wp]) << 16);
-    *reinterpret_cast<uint2*>(o16) = uint2{a, b};
+#pragma unroll
+  for (int c = 0; c < NP; ++c) {
+    const int b = c & 1;  // the buffer of plane c - 2 was read before plane c - 1's barrier
+#pragma unroll
+    for (int j = 0; j < FW; ++j) sT[b][wave + 8 * j][lane] = o[j][c];
+    __syncthreads();
+    if (pp < g.npts_pad) {
+      if (!LO || c < NV) {
+        float* out = vals + g.val_off + c * plane + (long long)pp * g.m_pad + it.k + wf;
+        *reinterpret_cast<fl4*>(out) =
+            fl4{sT[b][wf][wp], sT[b][wf + 1][wp], sT[b][wf + 2][wp], sT[b][wf + 3][wp]};
+      } else {
+        unsigned short* o16 = lo16 + g.phi_off + (long long)pp * g.m_pad + it.k + wf;
+        const unsigned a = float_to_bf16(sT[b][wf][wp]) |
+                           ((unsigned)float_to_bf16(sT[b][wf + 1][wp]) << 16);
+        const unsigned d = float_to_bf16(sT[b][wf + 2][wp]) |
+                           ((unsigned)float_to_bf16(sT[b][wf + 3][wp]) << 16);
+        *reinterpret_cast<uint2*>(o16) = uint2{a, d};
+      }
+    }
   }
 }
 
@@ -4227,8 +4238,8 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
     gitems.reserve(ng);
     // the point-major items XCD-interleaved: block b runs on XCD b % 8 and
     // takes item b / 8 of the list's (b % 8)-th eighth, so the function
-    // chunks of one point tile, whose 64-byte row pieces share cache lines,
-    // are written through one L2 (items of group -1 pad the eighths)
+    // chunks of one point tile, the rows of the same points, are written
+    // through one L2 (items of group -1 pad the eighths)
     pm_len = (npm + kXcds - 1) / kXcds;
     pmitems.assign(kXcds * pm_len, EvalItem{-1, 0, 0, 0});
   }
@@ -4634,18 +4645,18 @@ void Engine::load_batch_impl(const std::vector<const Molecule*>& mols,
       float* v = static_cast<float*>(dg->vals);
       if (dg->npm > 0) {
         if (dg->lo && dg->gga)
-          hipLaunchKernelGGL((k_eval_pm<10, true, 512>), dim3(dg->npm), dim3(512), 0, s,
+          hipLaunchKernelGGL((k_eval_pm<10, true>), dim3(dg->npm), dim3(512), 0, s,
                              dg->pmitems, dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v,
                              dg->phiT, dg->lo16);
         else if (dg->lo)
-          hipLaunchKernelGGL((k_eval_pm<4, true>), dim3(dg->npm), block, 0, s, dg->pmitems,
+          hipLaunchKernelGGL((k_eval_pm<4, true>), dim3(dg->npm), dim3(512), 0, s, dg->pmitems,
                              dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
         else if (dg->gga && !dg->radial)
-          hipLaunchKernelGGL((k_eval_pm<10, false, 512>), dim3(dg->npm), dim3(512), 0, s,
+          hipLaunchKernelGGL((k_eval_pm<10>), dim3(dg->npm), dim3(512), 0, s,
                              dg->pmitems, dg->groups, dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v,
                              dg->phiT, dg->lo16);
         else
-          hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), block, 0, s, dg->pmitems, dg->groups,
+          hipLaunchKernelGGL(k_eval_pm<4>, dim3(dg->npm), dim3(512), 0, s, dg->pmitems, dg->groups,
                              dg->fnl, dg->bas, dg->px, dg->py, dg->pz, v, dg->phiT, dg->lo16);
       }
       if (dg->presplit && dg->ndensity > 0)
