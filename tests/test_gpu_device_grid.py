@@ -25,7 +25,10 @@ def _gpu():
         pytest.skip("no GPU")
 
 
-@pytest.mark.parametrize("name", ["c0_pbe", "mol25_s103_pbe", "mol45_s104_pbe", "mol60_s106_pbe"])
+# mol195: groups of up to 190 nuclei, past the device Becke kernel's LDS
+# table of the 64 nearest
+@pytest.mark.parametrize("name", ["c0_pbe", "mol25_s103_pbe", "mol45_s104_pbe", "mol60_s106_pbe",
+                                  "mol195_s107_pbe"])
 def test_device_grid_matches_host(name):
     _gpu()
     from lio_amd import g2g
