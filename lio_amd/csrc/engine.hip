@@ -3031,7 +3031,8 @@ __global__ __launch_bounds__(256, 2) void k_build_persist(PersistArgs* pargs, in
 struct StGroup {
   int pt_off, npts;    // staged points
   int atom_off, natoms;  // nuclei (molecule-local, ascending) in gatoms
-  int mol, pad_;
+  int mol;
+  int sq_off;  // first entry of the group's natoms x natoms tables in near-first order
 };
 struct StMol {
   int atom_base, natom;  // atoms in the batch-wide atom arrays
@@ -3076,23 +3077,27 @@ constexpr int kBeckePts = 64, kBeckeNear = 64;
 // The near-first order of every staged group's nuclei: the nuclei by
 // distance from the group's first staged point, ties in list order (the
 // rank of q counts the nuclei strictly nearer, or as near and earlier), the
-// order the host's becke_order forms with the same operations; gord holds the
-// nucleus indices in that order.  One block per group, a thread per nucleus.
-__global__ __launch_bounds__(256) void k_becke_order(const StGroup* __restrict__ groups,
-                                                     const StMol* __restrict__ mols,
-                                                     const int* __restrict__ gatoms,
-                                                     const double* __restrict__ ax,
-                                                     const double* __restrict__ ay,
-                                                     const double* __restrict__ az,
-                                                     const double* __restrict__ sx,
-                                                     const double* __restrict__ sy,
-                                                     const double* __restrict__ sz,
-                                                     int* __restrict__ gord) {
+// order the host's becke_order forms with the same operations.  gord holds
+// the nucleus indices in that order, gpos the rank of each list position, and
+// the group's tables in that order follow: the nuclei's coordinates (ox, oy,
+// oz) and, for every list position j and rank t, the pair distance and size
+// term of nuclei j and gord[t] (td, tw, rows of natoms from sq_off), so that
+// the Becke products load by rank, with no index load in between.  One block
+// per group.
+__global__ __launch_bounds__(256) void k_becke_order(
+    const StGroup* __restrict__ groups, const StMol* __restrict__ mols,
+    const int* __restrict__ gatoms, const double* __restrict__ ax,
+    const double* __restrict__ ay, const double* __restrict__ az,
+    const double* __restrict__ aw, const double* __restrict__ rij,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, int* __restrict__ gord, int* __restrict__ gpos,
+    double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
+    double* __restrict__ td, double* __restrict__ tw) {
   const StGroup G = groups[blockIdx.x];
   if (G.npts == 0) return;
   const StMol m = mols[G.mol];
   const int* at = gatoms + G.atom_off;
-  const int k = G.natoms;
+  const int k = G.natoms, na = m.natom;
   const double* A = ax + m.atom_base;
   const double* B = ay + m.atom_base;
   const double* C = az + m.atom_base;
@@ -3105,6 +3110,23 @@ __global__ __launch_bounds__(256) void k_becke_order(const StGroup* __restrict__
       rank += (dr < dq || (dr == dq && r < q)) ? 1 : 0;
     }
     gord[G.atom_off + rank] = at[q];
+    gpos[G.atom_off + q] = rank;
+  }
+  __syncthreads();  // the order, written by the block, is read back below
+  const int* ord = gord + G.atom_off;
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    const int ak = ord[t];
+    ox[G.atom_off + t] = A[ak];
+    oy[G.atom_off + t] = B[ak];
+    oz[G.atom_off + t] = C[ak];
+  }
+  const double* D = rij + m.rij_off;
+  const double* W = aw + m.rij_off;
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    const int j = e / k, t = e % k;
+    const long long src = (long long)at[j] * na + ord[t];
+    td[(long long)G.sq_off + e] = D[src];
+    tw[(long long)G.sq_off + e] = W[src];
   }
 }
 
@@ -3192,8 +3214,9 @@ __global__ __launch_bounds__(kBeckePts) void k_becke_exact(
 // every factor in [0, 1], with P_own and P_n from k_becke_exact.  Lane l of
 // every wave takes point l; wave c sums the j of class c (j mod kBeckeLanes)
 // in list order, and the first wave adds the classes' sums in order.  Every
-// P_j other than P_own and P_n runs over the nuclei nearest-first (gord,
-// from the group's first point) and is dropped once its running product, an
+// P_j other than P_own and P_n runs over the nuclei nearest-first (the order
+// from the group's first point, with the coordinates and pair tables by rank
+// from k_becke_order) and is dropped once its running product, an
 // upper bound of P_j, falls below 2^-60 of the larger exact term of the sum
 // (P_own, P_n): a dropped term lies far below half an ulp of the sum.  The
 // distances to the first kBeckeNear nuclei of that order come from LDS
@@ -3207,8 +3230,10 @@ __global__ __launch_bounds__(kBeckePts) void k_becke_exact(
 __global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
     const StGroup* __restrict__ groups, const int2* __restrict__ items,
     const StMol* __restrict__ mols, const int* __restrict__ gatoms,
-    const int* __restrict__ gord, const double* __restrict__ ax, const double* __restrict__ ay,
-    const double* __restrict__ az, const double* __restrict__ aw, const double* __restrict__ rij,
+    const int* __restrict__ gpos, const double* __restrict__ ox, const double* __restrict__ oy,
+    const double* __restrict__ oz, const double* __restrict__ td, const double* __restrict__ tw,
+    const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
+    const double* __restrict__ aw, const double* __restrict__ rij,
     const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, const double2* __restrict__ bterm,
     const int2* __restrict__ bnuc, double* __restrict__ sw) {
@@ -3225,19 +3250,18 @@ __global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
   const int k = G.natoms, kn = min(k, kBeckeNear);
   const StMol m = mols[G.mol];
   const int* at = gatoms + G.atom_off;
-  const int* ord = gord + G.atom_off;
   const double* A = ax + m.atom_base;
   const double* B = ay + m.atom_base;
   const double* C = az + m.atom_base;
+  const double* Xo = ox + G.atom_off;  // the nuclei's coordinates by rank
+  const double* Yo = oy + G.atom_off;
+  const double* Zo = oz + G.atom_off;
   double x = 0.0, y = 0.0, z = 0.0;
   if (live) {
     x = sx[p];
     y = sy[p];
     z = sz[p];
-    for (int t = cls; t < kn; t += kBeckeLanes) {
-      const int ak = ord[t];
-      dnear[t][lane] = dist3(x, y, z, A[ak], B[ak], C[ak]);
-    }
+    for (int t = cls; t < kn; t += kBeckeLanes) dnear[t][lane] = dist3(x, y, z, Xo[t], Yo[t], Zo[t]);
   }
   __syncthreads();
   double mine = 0.0, sum = 0.0;
@@ -3267,12 +3291,14 @@ __global__ __launch_bounds__(kBeckePts * kBeckeLanes) void k_becke(
         const double* Dj = D + (long long)aj * na;
         const double* Wj = W + (long long)aj * na;
         if (becke_cell_w(dj, dn, Dj[an], Wj[an]) == 0.0) continue;
+        const int tj = gpos[G.atom_off + j];  // j's own rank, skipped
+        const double* Dr = td + G.sq_off + (long long)j * k;
+        const double* Wr = tw + G.sq_off + (long long)j * k;
         double prod = 1.0;
         for (int t = 0; t < k; ++t) {
-          const int ak = ord[t];
-          if (ak == aj) continue;
-          const double dk = t < kBeckeNear ? dnear[t][lane] : dist3(x, y, z, A[ak], B[ak], C[ak]);
-          prod *= becke_cell_w(dj, dk, Dj[ak], Wj[ak]);
+          if (t == tj) continue;
+          const double dk = t < kBeckeNear ? dnear[t][lane] : dist3(x, y, z, Xo[t], Yo[t], Zo[t]);
+          prod *= becke_cell_w(dj, dk, Dr[t], Wr[t]);
           if (prod < thr) {
             prod = 0.0;
             break;
@@ -3911,6 +3937,7 @@ Engine::~Engine() {
   drop_graph();
   release(grid_, (hipStream_t)stream_, device_);
   if (stage_) (void)hipFreeAsync(stage_, (hipStream_t)stream_);
+  if (becke_) (void)hipFreeAsync(becke_, (hipStream_t)stream_);
   if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
   for (auto& e : events_)
     if (e) (void)hipEventDestroy((hipEvent_t)e);
@@ -4757,6 +4784,19 @@ char* Engine::stage_buffer(size_t bytes) {
   return static_cast<char*>(stage_);
 }
 
+// The grow-only buffer of the Becke tables in near-first order, at least
+// `bytes`.
+char* Engine::becke_buffer(size_t bytes) {
+  if (becke_cap_ < bytes + 256) {
+    if (becke_) XCQ_HIP(hipFreeAsync(becke_, (hipStream_t)stream_));
+    becke_ = nullptr;
+    becke_cap_ = 0;
+    const size_t want = bytes + 256 + (bytes + 256) / 4;
+    becke_cap_ = pool_alloc(&becke_, want, bytes + 256, (hipStream_t)stream_, device_);
+  }
+  return static_cast<char*>(becke_);
+}
+
 // Becke weights, zero-weight removal and the group counts on the device,
 // then the final groups (at least min_points_per_cube points) are compacted
 // into the arena by the batch load.
@@ -4770,7 +4810,31 @@ std::vector<GridPartition> Engine::weigh_and_load(const std::vector<const Molecu
   auto up = [&](void* dst, const void* src, size_t bytes) {
     if (bytes) XCQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   };
-  up(d.groups, h.sg.data(), h.sg.size() * sizeof(StGroup));
+  // each group's natoms x natoms Becke tables in near-first order, in their
+  // own grow-only buffer (sized now that the groups are known)
+  std::vector<StGroup> sg = h.sg;
+  long long nsq = 0;
+  for (StGroup& G : sg) {
+    G.sq_off = (int)nsq;
+    nsq += (long long)G.natoms * G.natoms;
+  }
+  if (nsq > (1LL << 31) - 256) throw std::runtime_error("g2g: Becke tables too large");
+  const size_t nat = h.gatoms.size() + 1;
+  double *td, *tw, *ox, *oy, *oz;
+  int* gpos;
+  auto carve_becke = [&](Carver& c) {
+    td = c.take<double>((size_t)nsq + 1);
+    tw = c.take<double>((size_t)nsq + 1);
+    ox = c.take<double>(nat);
+    oy = c.take<double>(nat);
+    oz = c.take<double>(nat);
+    gpos = c.take<int>(nat);
+  };
+  Carver bsize{nullptr};
+  carve_becke(bsize);
+  Carver bc{becke_buffer(bsize.used)};
+  carve_becke(bc);
+  up(d.groups, sg.data(), sg.size() * sizeof(StGroup));
   up(d.mols, h.sm.data(), h.sm.size() * sizeof(StMol));
   // the Becke blocks heaviest first (cost ~ natoms^2), so that the longest
   // ones do not start last
@@ -4787,14 +4851,15 @@ std::vector<GridPartition> Engine::weigh_and_load(const std::vector<const Molecu
   up(d.rij, h.rij.data(), h.rij.size() * sizeof(double));
   if (nst > 0)
     hipLaunchKernelGGL(k_becke_order, dim3(nst), dim3(256), 0, s, d.groups, d.mols, d.gatoms,
-                       d.ax, d.ay, d.az, d.x, d.y, d.z, d.gord);
+                       d.ax, d.ay, d.az, d.aw, d.rij, d.x, d.y, d.z, d.gord, gpos, ox, oy, oz,
+                       td, tw);
   if (!h.bitems.empty()) {
     hipLaunchKernelGGL(k_becke_exact, dim3((unsigned)h.bitems.size()), dim3(kBeckePts), 0, s,
                        d.groups, d.bitems, d.mols, d.gatoms, d.owner, d.ax, d.ay, d.az, d.aw,
                        d.rij, d.x, d.y, d.z, d.bterm, d.bnuc);
     hipLaunchKernelGGL(k_becke, dim3((unsigned)h.bitems.size()), dim3(kBeckePts * kBeckeLanes), 0,
-                       s, d.groups, d.bitems, d.mols, d.gatoms, d.gord, d.ax, d.ay, d.az, d.aw,
-                       d.rij, d.x, d.y, d.z, d.bterm, d.bnuc, d.w);
+                       s, d.groups, d.bitems, d.mols, d.gatoms, gpos, ox, oy, oz, td, tw, d.ax,
+                       d.ay, d.az, d.aw, d.rij, d.x, d.y, d.z, d.bterm, d.bnuc, d.w);
   }
   const int drop = opt.remove_zero_weights ? 1 : 0;
   if (nst > 0)
@@ -4871,7 +4936,7 @@ std::vector<GridPartition> Engine::load_staged(const std::vector<const Molecule*
       G.atom_off = (int)h.gatoms.size();
       G.natoms = st.atom_off[g + 1] - st.atom_off[g];
       G.mol = k;
-      G.pad_ = 0;
+      G.sq_off = 0;
       h.gatoms.insert(h.gatoms.end(), st.atoms.begin() + st.atom_off[g],
                       st.atoms.begin() + st.atom_off[g + 1]);
       const int b = gp.pt_off[g];
@@ -5160,7 +5225,7 @@ std::vector<GridPartition> Engine::load_generated(const std::vector<const Molecu
     G.atom_off = (int)h.gatoms.size();
     G.natoms = bi.nnuc;
     G.mol = k;
-    G.pad_ = 0;
+    G.sq_off = 0;
     h.gatoms.insert(h.gatoms.end(), pa.begin() + it.aoff, pa.begin() + it.aoff + bi.nnuc);
     for (int o2 = 0; o2 < G.npts; o2 += kBeckePts)
       h.bitems.push_back(make_int2((int)h.sg.size(), G.pt_off + o2));
@@ -5626,6 +5691,9 @@ void Engine::unload() {
   if (stage_) (void)hipFreeAsync(stage_, s);
   stage_ = nullptr;
   stage_cap_ = 0;
+  if (becke_) (void)hipFreeAsync(becke_, s);
+  becke_ = nullptr;
+  becke_cap_ = 0;
   // the frees have run: the memory is back in the pool for any stream
   XCQ_HIP(hipStreamSynchronize(s));
 }

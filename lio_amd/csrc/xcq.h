@@ -347,12 +347,15 @@ class Engine {
   void* graph_ = nullptr;
   void* stage_ = nullptr;     // device grid rebuild staging (kept, grow-only)
   size_t stage_cap_ = 0;
+  void* becke_ = nullptr;     // Becke tables in near-first order (kept, grow-only)
+  size_t becke_cap_ = 0;
   double load_ms_[4] = {};
   bool pending_ = false;      // an asynchronous build is queued and not yet collected
   void check_not_pending(const char* what) const;
   Tuning tuning_;
   void drop_graph();
   char* stage_buffer(size_t bytes);
+  char* becke_buffer(size_t bytes);
   std::vector<GridPartition> weigh_and_load(const std::vector<const Molecule*>& mols,
                                             const struct StageHost& h, const struct StageDev& d,
                                             const Options& opt, Precision prec, double t_start);
